@@ -1,0 +1,145 @@
+/*
+ * wr_hip.h -- C ABI of the MI355X (gfx950) pyramid Lucas-Kanade registration
+ * library (libwrhip.so).
+ *
+ * The reference has no FFI for this path: its backend "plugin" is the module
+ * pair getMotion / correctMotion in
+ *   src/wholistic_registration/utils/calFlow3d_Wei_v1.py:295-673 (getMotion)
+ *   src/wholistic_registration/utils/calFlow3d_Wei_v1.py:676-706 (correctMotion)
+ * resolved by name from wbi_registration_3d
+ * (src/wholistic_registration/utils/registration.py:320-326).  Each entry point
+ * below replaces one of those functions; the Python module
+ * wholistic_registration_amd.calFlow3d_Wei_v1 binds them with ctypes and keeps
+ * the reference's Python signatures.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer on the context's device, in
+ *     C order (Z, Y, X): x is the fastest axis.  This is the memory order of
+ *     the (Z, Y, X) volumes the reference harness transposes into its (x, y, z)
+ *     views (registration.py:251,313), so no transpose is needed.
+ *   - Motion fields are (Z, Y, X, 3) float32, component c displacing the
+ *     reference's axis c (0 = x, 1 = y, 2 = z).
+ *   - Masks are float32 (Z, Y, X); a value > 0 after resampling means
+ *     "excluded" (calFlow3d_Wei_v1.py:422-423,473-474).  NULL = no mask.
+ *   - All work is enqueued on `stream` (NULL = the default stream).
+ *     wr_get_motion returns after the stream has drained, because its return
+ *     value includes host-side error logs.  wr_correct_motion is asynchronous.
+ *   - Return value: WR_OK or a negative WR_E* code; wr_last_error() gives the
+ *     message (thread-local).
+ */
+#ifndef WR_HIP_H
+#define WR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WR_OK 0
+#define WR_E_ARG -1      /* invalid argument (reference: ValueError)           */
+#define WR_E_KEY -2      /* missing parameter (reference: KeyError)            */
+#define WR_E_HIP -3      /* HIP runtime error                                   */
+#define WR_E_MEM -4      /* device allocation failed                           */
+
+#define WR_F32 0         /* input volume dtypes */
+#define WR_U16 1
+#define WR_F64 2         /* accepted by wr_imresize only */
+
+#define WR_MAX_LEVELS 16
+
+typedef struct wr_ctx wr_ctx;
+
+/* Solver parameters: the `option` keys getMotion reads
+ * (calFlow3d_Wei_v1.py:317-324,429,453,523). */
+typedef struct wr_params {
+    int layer;              /* option["layer"]: levels layer..0 are visited     */
+    int iter;               /* option["iter"]: iterations per level             */
+    int r;                  /* option["r"]: patch half-width, (2r+1)^2 x 1      */
+    int reserved0;
+    double zRatio;          /* option["zRatio"]                                 */
+    double movRange;        /* option["movRange"]: per-update step cap          */
+    double smoothPenalty;   /* option["smoothPenalty"] (already normalised)     */
+    double tol;             /* option["tol"]: error-change stop tolerance       */
+    /* Optional forced schedule (NULL = the data-dependent stop rules of
+     * calFlow3d_Wei_v1.py:515-530,651-654).  Indexed by level 0..layer:
+     * forced_records[L] error records, forced_updates[L] updates. */
+    const int* forced_records;
+    const int* forced_updates;
+    int tile_patches_x;     /* 0 = autotuned; patches per workgroup along x   */
+    int tile_bands_y;       /* 0 = autotuned; patch rows per workgroup         */
+} wr_params;
+
+/* Per-level iteration log, filled on the host when wr_get_motion returns
+ * (the lists of error_log["layer_L"], calFlow3d_Wei_v1.py:329-335,505-507,643).
+ * Arrays are [level][iter] with level 0..layer, row stride `iter`. */
+typedef struct wr_log {
+    int records[WR_MAX_LEVELS];   /* len(currentError) per level            */
+    int updates[WR_MAX_LEVELS];   /* len(max_diff_motion) per level         */
+    int nans[WR_MAX_LEVELS];      /* NaN components zeroed by the solve     */
+    double* diff_error;           /* host, (layer+1)*iter, may be NULL      */
+    double* penalty_error;
+    double* current_error;
+    double* max_diff_motion;
+    double final_error;           /* currentError returned by getMotion     */
+} wr_log;
+
+/* Create / destroy a context bound to HIP device `device`.  The context owns
+ * the solver workspace, grown on demand and reused across calls. */
+wr_ctx* wr_create(int device);
+void wr_destroy(wr_ctx* ctx);
+
+/* Thread-local message of the last failed call. */
+const char* wr_last_error(void);
+
+/* Replaces getMotion(dat_mov, dat_ref, option)
+ * (calFlow3d_Wei_v1.py:295-673).
+ *   mov, ref      (Z,Y,X) volumes of dtype `dtype` (WR_F32 or WR_U16)
+ *   mask_ref      (Z,Y,X) float32 or NULL   (option["mask_ref"])
+ *   mask_mov      (Z,Y,X) float32 or NULL   (option["mask_mov"])
+ *   motion_init   (Z,Y,X,3) float32 or NULL (option["motion"])
+ *   motion_out    (Z,Y,X,3) float32, the level-0 motion field
+ *   coords_out    (Z,Y,X,3) float32 or NULL: motion + voxel grid
+ *   log           host struct or NULL */
+int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype,
+                  int nx, int ny, int nz,
+                  const float* mask_ref, const float* mask_mov,
+                  const float* motion_init, const wr_params* params,
+                  float* motion_out, float* coords_out, wr_log* log,
+                  void* stream);
+
+/* Replaces correctMotion(data_raw, motion_field)
+ * (calFlow3d_Wei_v1.py:676-706): out(p) = trilinear(in, p + motion(p)),
+ * clamp to edge, float32 out. */
+int wr_correct_motion(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz,
+                      const float* motion, float* out, void* stream);
+
+/* Pyramid level image as the reference builds it: imresize(vol,
+ * output_shape=(int(nx/2^level), int(ny/2^level), nz)) (bicubic,
+ * antialiased, imresize.py:148-191; calFlow3d_Wei_v1.py:341-351), stored as
+ * float32.  out holds (nz, ny_L, nx_L). */
+int wr_pyramid_level(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz,
+                     int level, float* out, void* stream);
+
+/* MATLAB-style resize of the x and y axes, z kept: imresize(vol,
+ * output_shape=(nx_out, ny_out, nz), method) (imresize.py:148-191), float64
+ * result in (nz, ny_out, nx_out).  method: 0 = bicubic, 1 = bilinear. */
+int wr_imresize(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz,
+                int nx_out, int ny_out, int method, double* out, void* stream);
+
+/* Launch statistics of the last wr_get_motion on this context, for
+ * measurement: number of LK passes launched and executed per level, the LK
+ * kernel's tile shape, and the grid size. */
+typedef struct wr_stats {
+    int lk_launches[WR_MAX_LEVELS];
+    int tile_cols;                /* owned columns per workgroup            */
+    int tile_rows;                /* owned rows per workgroup               */
+    int lk_blocks[WR_MAX_LEVELS]; /* workgroups per LK pass                  */
+} wr_stats;
+int wr_get_stats(wr_ctx* ctx, wr_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WR_HIP_H */

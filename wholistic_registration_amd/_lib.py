@@ -1,0 +1,134 @@
+"""ctypes binding of libwrhip.so (declared in include/wr_hip.h).
+
+The library is required: every entry point raises when it is not built or
+cannot be loaded.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import threading
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libwrhip.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "wr_hip.h")
+
+WR_OK, WR_E_ARG, WR_E_KEY, WR_E_HIP, WR_E_MEM = 0, -1, -2, -3, -4
+WR_F32, WR_U16, WR_F64 = 0, 1, 2
+WR_MAX_LEVELS = 16
+
+
+class WrParams(ctypes.Structure):
+    _fields_ = [
+        ("layer", ctypes.c_int),
+        ("iter", ctypes.c_int),
+        ("r", ctypes.c_int),
+        ("reserved0", ctypes.c_int),
+        ("zRatio", ctypes.c_double),
+        ("movRange", ctypes.c_double),
+        ("smoothPenalty", ctypes.c_double),
+        ("tol", ctypes.c_double),
+        ("forced_records", ctypes.POINTER(ctypes.c_int)),
+        ("forced_updates", ctypes.POINTER(ctypes.c_int)),
+        ("tile_patches_x", ctypes.c_int),
+        ("tile_bands_y", ctypes.c_int),
+    ]
+
+
+class WrLog(ctypes.Structure):
+    _fields_ = [
+        ("records", ctypes.c_int * WR_MAX_LEVELS),
+        ("updates", ctypes.c_int * WR_MAX_LEVELS),
+        ("nans", ctypes.c_int * WR_MAX_LEVELS),
+        ("diff_error", ctypes.POINTER(ctypes.c_double)),
+        ("penalty_error", ctypes.POINTER(ctypes.c_double)),
+        ("current_error", ctypes.POINTER(ctypes.c_double)),
+        ("max_diff_motion", ctypes.POINTER(ctypes.c_double)),
+        ("final_error", ctypes.c_double),
+    ]
+
+
+class WrStats(ctypes.Structure):
+    _fields_ = [
+        ("lk_launches", ctypes.c_int * WR_MAX_LEVELS),
+        ("tile_cols", ctypes.c_int),
+        ("tile_rows", ctypes.c_int),
+        ("lk_blocks", ctypes.c_int * WR_MAX_LEVELS),
+    ]
+
+
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_SIGNATURES = {
+    "wr_create": ([_I], _VP),
+    "wr_destroy": ([_VP], None),
+    "wr_last_error": ([], ctypes.c_char_p),
+    "wr_get_motion": ([_VP, _VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP, ctypes.POINTER(WrParams),
+                       _VP, _VP, ctypes.POINTER(WrLog), _VP], _I),
+    "wr_correct_motion": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP], _I),
+    "wr_pyramid_level": ([_VP, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
+    "wr_imresize": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
+    "wr_get_stats": ([_VP, ctypes.POINTER(WrStats)], _I),
+}
+
+_lib = None
+_lock = threading.Lock()
+_ctx = {}
+
+
+def header_symbols(path=HEADER):
+    """Function names declared in the C header."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wr_[a-z_]+)\s*\(", text)))
+
+
+def load():
+    """Load libwrhip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m wholistic_registration_amd.build_lib`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+        return lib
+
+
+def last_error():
+    msg = load().wr_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc, what):
+    if rc == WR_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if rc == WR_E_ARG:
+        raise ValueError(msg)
+    if rc == WR_E_KEY:
+        raise KeyError(msg)
+    if rc == WR_E_MEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def context(device=0):
+    """The library context of one HIP device (created on first use)."""
+    lib = load()
+    with _lock:
+        h = _ctx.get(device)
+        if h is None:
+            h = lib.wr_create(device)
+            if not h:
+                raise RuntimeError(f"wr_create({device}): {last_error()}")
+            _ctx[device] = h
+        return h

@@ -1,0 +1,414 @@
+// C ABI of libwrhip.so and the host-side driver of getMotion.
+//
+// wr_get_motion enqueues the whole pyramid (calFlow3d_Wei_v1.py:327-673) on
+// one stream: per level the two level images and masks (imresize from full
+// resolution, :346-351,422-423), the level's initial field (warm start
+// :362-380 or the bilinear x2 upsample of the previous level :381-410), and
+// `iter` launches of the iteration sequence whose stop decisions are taken on
+// the device.  The host waits once, at the end, for the logs.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "wr_internal.h"
+
+namespace wr {
+
+namespace {
+thread_local std::string g_error;
+
+__global__ void k_fill(float* p, size_t n, float v) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void fill(float* p, size_t n, float v, hipStream_t s) {
+    k_fill<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(p, n, v);
+}
+
+// Number of control points along an axis: len(range(r, n - 1, 2r + 1))
+// (calFlow3d:434-435).
+int n_cp(int n, int r) {
+    const int p = 2 * r + 1;
+    const int span = (n - 1) - r;
+    return span > 0 ? (span + p - 1) / p : 0;
+}
+
+struct Plan2 {
+    Table tx, ty;
+    bool x_first;
+};
+
+// Axis order of imresize: ascending scale, stable (imresize.py:170-171,185-187).
+bool x_before_y(int nx_in, int nx_out, int ny_in, int ny_out) {
+    const double sx = 1.0 * nx_out / nx_in, sy = 1.0 * ny_out / ny_in;
+    return sx <= sy;
+}
+
+// in (nz, ny_in, nx_in) -> out (nz, ny_out, nx_out) through `tmp` (float64).
+void resample2(const void* in, Elem in_t, int stride, int comp, void* out, OutMode om, double div,
+               const Plan2& pl, const double* w, const int* idx, int nz, double* tmp, hipStream_t s) {
+    const Table& tx = pl.tx;
+    const Table& ty = pl.ty;
+    if (tx.identity && ty.identity) {
+        launch_convert(in, in_t, stride, comp, out, om, div, static_cast<size_t>(tx.n_in) * ty.n_in * nz, s);
+    } else if (ty.identity) {
+        launch_resample_x(in, in_t, stride, comp, out, om, div, tx, w, idx, ty.n_in, nz, s);
+    } else if (tx.identity) {
+        launch_resample_y(in, in_t, stride, comp, out, om, div, ty, w, idx, tx.n_in, nz, s);
+    } else if (pl.x_first) {
+        launch_resample_x(in, in_t, stride, comp, tmp, OUT_F64, 1.0, tx, w, idx, ty.n_in, nz, s);
+        launch_resample_y(tmp, E_F64, 1, 0, out, om, div, ty, w, idx, tx.n_out, nz, s);
+    } else {
+        launch_resample_y(in, in_t, stride, comp, tmp, OUT_F64, 1.0, ty, w, idx, tx.n_in, nz, s);
+        launch_resample_x(tmp, E_F64, 1, 0, out, om, div, tx, w, idx, ty.n_out, nz, s);
+    }
+}
+
+size_t tmp_bytes(const Plan2& pl, int nz) {
+    if (pl.tx.identity || pl.ty.identity) return 0;
+    const size_t a = pl.x_first ? static_cast<size_t>(pl.tx.n_out) * pl.ty.n_in
+                                : static_cast<size_t>(pl.tx.n_in) * pl.ty.n_out;
+    return a * nz * sizeof(double);
+}
+
+Plan2 make_plan(int nx_in, int nx_out, int ny_in, int ny_out, Kernel k, std::vector<double>& wa,
+                std::vector<int>& ia) {
+    Plan2 p;
+    p.tx = build_table(nx_in, nx_out, k, wa, ia);
+    p.ty = build_table(ny_in, ny_out, k, wa, ia);
+    p.x_first = x_before_y(nx_in, nx_out, ny_in, ny_out);
+    return p;
+}
+
+int upload_tables(Context& c, const std::vector<double>& wa, const std::vector<int>& ia) {
+    if (int rc = grow(c.tab_w, std::max<size_t>(wa.size(), 1) * sizeof(double))) return rc;
+    if (int rc = grow(c.tab_i, std::max<size_t>(ia.size(), 1) * sizeof(int))) return rc;
+    if (!wa.empty()) WR_HIP_CHECK(hipMemcpy(c.tab_w.p, wa.data(), wa.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!ia.empty()) WR_HIP_CHECK(hipMemcpy(c.tab_i.p, ia.data(), ia.size() * sizeof(int), hipMemcpyHostToDevice));
+    return WR_OK;
+}
+
+Elem elem_of(int dtype) { return dtype == WR_U16 ? E_U16 : (dtype == WR_F64 ? E_F64 : E_F32); }
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_error = msg; }
+
+int grow(DevBuf& b, size_t bytes) {
+    if (bytes <= b.bytes && b.p) return WR_OK;
+    if (b.p) {
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    const size_t want = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        set_error("device allocation of " + std::to_string(want) + " bytes failed");
+        return WR_E_MEM;
+    }
+    b.bytes = want;
+    return WR_OK;
+}
+
+void release(DevBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+}  // namespace wr
+
+using namespace wr;
+
+#define WR_TRY(expr)             \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != WR_OK) return rc_; \
+    } while (0)
+
+#define WR_ARG(cond, msg)         \
+    do {                          \
+        if (!(cond)) {            \
+            set_error(msg);       \
+            return WR_E_ARG;      \
+        }                         \
+    } while (0)
+
+struct wr_ctx {
+    Context c;
+};
+
+extern "C" {
+
+wr_ctx* wr_create(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+        set_error("wr_create: no HIP device " + std::to_string(device));
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        set_error("wr_create: hipSetDevice failed");
+        return nullptr;
+    }
+    auto* ctx = new wr_ctx;
+    ctx->c.device = device;
+    return ctx;
+}
+
+void wr_destroy(wr_ctx* ctx) {
+    if (!ctx) return;
+    Context& c = ctx->c;
+    (void)hipSetDevice(c.device);
+    for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
+                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.tab_w, &c.tab_i})
+        release(*b);
+    delete ctx;
+}
+
+const char* wr_last_error(void) { return g_error.c_str(); }
+
+int wr_get_stats(wr_ctx* ctx, wr_stats* out) {
+    WR_ARG(ctx && out, "wr_get_stats: null argument");
+    *out = ctx->c.stats;
+    return WR_OK;
+}
+
+int wr_imresize(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz, int nx_out, int ny_out,
+                int method, double* out, void* stream) {
+    WR_ARG(ctx && in && out, "wr_imresize: null argument");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0 && nx_out > 0 && ny_out > 0, "wr_imresize: empty shape");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16 || dtype == WR_F64, "wr_imresize: unsupported dtype");
+    WR_ARG(method == 0 || method == 1, "unidentified kernel method supplied");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<double> wa;
+    std::vector<int> ia;
+    const Plan2 pl = make_plan(nx, nx_out, ny, ny_out, method == 0 ? KERNEL_CUBIC : KERNEL_TENT, wa, ia);
+    WR_TRY(upload_tables(c, wa, ia));
+    WR_TRY(grow(c.tmp, tmp_bytes(pl, nz)));
+    resample2(in, elem_of(dtype), 1, 0, out, OUT_F64, 1.0, pl, c.tab_w.as<double>(), c.tab_i.as<int>(), nz,
+              c.tmp.as<double>(), s);
+    WR_HIP_CHECK(hipGetLastError());
+    WR_HIP_CHECK(hipStreamSynchronize(s));
+    return WR_OK;
+}
+
+int wr_pyramid_level(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz, int level, float* out,
+                     void* stream) {
+    WR_ARG(ctx && in && out, "wr_pyramid_level: null argument");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_pyramid_level: dtype must be WR_F32 or WR_U16");
+    WR_ARG(level >= 0 && level < WR_MAX_LEVELS, "wr_pyramid_level: level out of range");
+    const int lx = static_cast<int>(nx / std::ldexp(1.0, level)), ly = static_cast<int>(ny / std::ldexp(1.0, level));
+    WR_ARG(lx > 0 && ly > 0 && nz > 0, "wr_pyramid_level: level has an empty axis");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::vector<double> wa;
+    std::vector<int> ia;
+    const Plan2 pl = make_plan(nx, lx, ny, ly, KERNEL_CUBIC, wa, ia);
+    WR_TRY(upload_tables(c, wa, ia));
+    WR_TRY(grow(c.tmp, tmp_bytes(pl, nz)));
+    resample2(in, elem_of(dtype), 1, 0, out, OUT_F32, 1.0, pl, c.tab_w.as<double>(), c.tab_i.as<int>(), nz,
+              c.tmp.as<double>(), s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_correct_motion(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz, const float* motion,
+                      float* out, void* stream) {
+    WR_ARG(ctx && in && motion && out, "wr_correct_motion: null argument");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_correct_motion: empty volume");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_correct_motion: dtype must be WR_F32 or WR_U16");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_warp(in, elem_of(dtype), nx, ny, nz, motion, out, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
+                  const float* mask_ref, const float* mask_mov, const float* motion_init, const wr_params* prm,
+                  float* motion_out, float* coords_out, wr_log* log, void* stream) {
+    WR_ARG(ctx && mov && ref && prm && motion_out, "wr_get_motion: null argument");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_get_motion: dtype must be WR_F32 or WR_U16");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_get_motion: empty volume");
+    const int layer = prm->layer, iter = prm->iter, r = prm->r;
+    WR_ARG(layer >= 0 && layer < WR_MAX_LEVELS, "wr_get_motion: layer out of range");
+    WR_ARG(iter >= 1, "wr_get_motion: iter must be at least 1");
+    WR_ARG(r >= 0, "wr_get_motion: r must be non-negative");
+    WR_ARG(prm->zRatio != 0.0 && std::isfinite(prm->zRatio), "wr_get_motion: zRatio must be finite, non-zero");
+    WR_ARG(prm->movRange > 0.0, "wr_get_motion: movRange must be positive");
+    const int p = 2 * r + 1;
+
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Elem et = elem_of(dtype);
+
+    // ---- geometry and tables of every level
+    std::vector<LevelGeom> geo(layer + 1);
+    std::vector<LkTile> tiles(layer + 1);
+    std::vector<Plan2> down(layer + 1), up(layer + 1);
+    std::vector<double> wa;
+    std::vector<int> ia;
+    size_t tmp_need = 0, ncp_max = 0, nblk_max = 0, ncpb_max = 0, nvox_prev = 0;
+    for (int L = layer; L >= 0; --L) {
+        LevelGeom& g = geo[L];
+        g.nx = static_cast<int>(nx / std::ldexp(1.0, L));
+        g.ny = static_cast<int>(ny / std::ldexp(1.0, L));
+        g.nz = nz;
+        WR_ARG(g.nx > 0 && g.ny > 0, "wr_get_motion: pyramid level " + std::to_string(L) + " has an empty axis");
+        g.r = r;
+        g.p = p;
+        g.ncx = n_cp(g.nx, r);
+        g.ncy = n_cp(g.ny, r);
+        WR_ARG(g.ncx > 0 && g.ncy > 0,
+               "wr_get_motion: pyramid level " + std::to_string(L) + " is smaller than one patch");
+        g.extra_x = std::max(0, g.ncx * p - g.nx);
+        g.extra_y = std::max(0, g.ncy * p - g.ny);
+        g.z_ratio_d = prm->zRatio / std::ldexp(1.0, L);
+        g.z_ratio = static_cast<float>(g.z_ratio_d);
+        g.mov_range = prm->movRange;
+        tiles[L] = plan_tile(g, prm->tile_patches_x, prm->tile_bands_y);
+        down[L] = make_plan(nx, g.nx, ny, g.ny, KERNEL_CUBIC, wa, ia);
+        tmp_need = std::max(tmp_need, tmp_bytes(down[L], nz));
+        if (L < layer) {
+            up[L] = make_plan(geo[L + 1].nx, g.nx, geo[L + 1].ny, g.ny, KERNEL_TENT, wa, ia);
+            tmp_need = std::max(tmp_need, tmp_bytes(up[L], nz));
+            nvox_prev = std::max(nvox_prev, static_cast<size_t>(geo[L + 1].nx) * geo[L + 1].ny * nz);
+        }
+        const size_t ncp = static_cast<size_t>(g.ncx) * g.ncy * nz;
+        ncp_max = std::max(ncp_max, ncp);
+        nblk_max = std::max(nblk_max, static_cast<size_t>(tiles[L].blocks));
+        ncpb_max = std::max(ncpb_max, (ncp + 255) / 256);
+    }
+    if (prm->forced_records || prm->forced_updates)
+        WR_ARG(prm->forced_records && prm->forced_updates, "wr_get_motion: forced schedule needs both arrays");
+
+    // ---- workspace
+    const size_t n0 = static_cast<size_t>(nx) * ny * nz;
+    WR_TRY(upload_tables(c, wa, ia));
+    WR_TRY(grow(c.tmp, tmp_need));
+    if (layer > 0 || dtype != WR_F32) {
+        WR_TRY(grow(c.mov_l, n0 * sizeof(float)));
+        WR_TRY(grow(c.ref_l, n0 * sizeof(float)));
+    }
+    if (mask_ref) WR_TRY(grow(c.mref_l, n0));
+    if (mask_mov) WR_TRY(grow(c.mmov_l, n0 * sizeof(float)));
+    WR_TRY(grow(c.m0, 3 * n0 * sizeof(float)));
+    if (layer > 0) WR_TRY(grow(c.mprev, 3 * nvox_prev * sizeof(float)));
+    WR_TRY(grow(c.cp, 3 * ncp_max * sizeof(float)));
+    WR_TRY(grow(c.dnei, 3 * ncp_max * sizeof(float)));
+    WR_TRY(grow(c.sums, 9 * ncp_max * sizeof(float)));
+    WR_TRY(grow(c.errpart, nblk_max * sizeof(double)));
+    WR_TRY(grow(c.penpart, ncpb_max * sizeof(double)));
+    WR_TRY(grow(c.ctl, sizeof(Ctl)));
+    WR_TRY(grow(c.log, static_cast<size_t>(layer + 1) * iter * 4 * sizeof(double)));
+    WR_TRY(grow(c.counts, static_cast<size_t>(layer + 1) * 3 * sizeof(int)));
+    fill(c.log.as<float>(), static_cast<size_t>(layer + 1) * iter * 8, 0.0f, s);
+    fill(c.counts.as<float>(), static_cast<size_t>(layer + 1) * 3, 0.0f, s);
+
+    const double* tw = c.tab_w.as<double>();
+    const int* ti = c.tab_i.as<int>();
+    double* tmp = c.tmp.as<double>();
+    float* m0 = c.m0.as<float>();
+    float* mprev = c.mprev.as<float>();
+    std::memset(&c.stats, 0, sizeof(c.stats));
+
+    for (int L = layer; L >= 0; --L) {
+        const LevelGeom& g = geo[L];
+        const LkTile& t = tiles[L];
+        const size_t nv = static_cast<size_t>(g.nx) * g.ny * nz;
+
+        // level images (float32) and masks
+        const float* mov_l;
+        const float* ref_l;
+        if (L == 0 && dtype == WR_F32) {
+            mov_l = static_cast<const float*>(mov);
+            ref_l = static_cast<const float*>(ref);
+        } else {
+            resample2(mov, et, 1, 0, c.mov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, s);
+            resample2(ref, et, 1, 0, c.ref_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, s);
+            mov_l = c.mov_l.as<float>();
+            ref_l = c.ref_l.as<float>();
+        }
+        if (mask_ref) resample2(mask_ref, E_F32, 1, 0, c.mref_l.p, OUT_MASK, 1.0, down[L], tw, ti, nz, tmp, s);
+        if (mask_mov) resample2(mask_mov, E_F32, 1, 0, c.mmov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, s);
+
+        // initial dense field of the level, 3 planes
+        if (L == layer) {
+            if (motion_init) {
+                const double div[3] = {1.0 * nx / g.nx, 1.0 * ny / g.ny, 1.0 * nz / g.nz};
+                for (int q = 0; q < 3; ++q)
+                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q], down[L], tw, ti, nz, tmp, s);
+            } else {
+                fill(m0, 3 * nv, 0.0f, s);
+            }
+        } else {
+            const size_t np = static_cast<size_t>(geo[L + 1].nx) * geo[L + 1].ny * nz;
+            for (int q = 0; q < 3; ++q)
+                resample2(mprev + q * np, E_F32, 1, 0, m0 + q * nv, OUT_F32, q < 2 ? 0.5 : 1.0, up[L], tw, ti, nz, tmp,
+                          s);
+        }
+        launch_cp_init(g, m0, c.cp.as<float>(), s);
+        const int frec = prm->forced_records ? prm->forced_records[L] : -1;
+        const int fupd = prm->forced_updates ? prm->forced_updates[L] : -1;
+        launch_ctl_reset(c.ctl.as<Ctl>(), iter, frec, fupd, prm->tol, prm->smoothPenalty * p * p,
+                         static_cast<double>(nv), s);
+
+        LkBuffers b;
+        b.mov = mov_l;
+        b.ref = ref_l;
+        b.mref = mask_ref ? c.mref_l.as<uint8_t>() : nullptr;
+        b.mmov = mask_mov ? c.mmov_l.as<float>() : nullptr;
+        b.m0 = m0;
+        b.cp = c.cp.as<float>();
+        b.dnei = c.dnei.as<float>();
+        b.sums = c.sums.as<float>();
+        b.errpart = c.errpart.as<double>();
+        b.penpart = c.penpart.as<double>();
+        b.ctl = c.ctl.as<Ctl>();
+        b.log = c.log.as<double>() + static_cast<size_t>(L) * iter * 4;
+        b.counts = c.counts.as<int>() + L * 3;
+        for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, L, k, iter, s));
+        c.stats.lk_launches[L] = iter;
+        c.stats.lk_blocks[L] = static_cast<int>(t.blocks);
+        if (L == 0) {
+            c.stats.tile_cols = t.cols;
+            c.stats.tile_rows = t.rows;
+        }
+        if (L > 0) launch_level_end(g, b, L, iter, mprev, nullptr, nullptr, s);
+        else launch_level_end(g, b, L, iter, nullptr, motion_out, coords_out, s);
+        WR_HIP_CHECK(hipGetLastError());
+    }
+
+    // ---- logs back to the host
+    std::vector<double> hlog(static_cast<size_t>(layer + 1) * iter * 4);
+    std::vector<int> hcnt(static_cast<size_t>(layer + 1) * 3);
+    WR_HIP_CHECK(hipMemcpyAsync(hlog.data(), c.log.p, hlog.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipMemcpyAsync(hcnt.data(), c.counts.p, hcnt.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipStreamSynchronize(s));
+    if (log) {
+        for (int L = 0; L <= layer; ++L) {
+            log->records[L] = hcnt[L * 3 + 0];
+            log->updates[L] = hcnt[L * 3 + 1];
+            log->nans[L] = hcnt[L * 3 + 2];
+            for (int k = 0; k < iter; ++k) {
+                const double* row = &hlog[(static_cast<size_t>(L) * iter + k) * 4];
+                const size_t o = static_cast<size_t>(L) * iter + k;
+                if (log->diff_error) log->diff_error[o] = row[0];
+                if (log->penalty_error) log->penalty_error[o] = row[1];
+                if (log->current_error) log->current_error[o] = row[2];
+                if (log->max_diff_motion) log->max_diff_motion[o] = row[3];
+            }
+        }
+        const int n = hcnt[0];
+        log->final_error = n > 0 ? hlog[static_cast<size_t>(n - 1) * 4 + 2] : NAN;
+    }
+    return WR_OK;
+}
+
+}  // extern "C"

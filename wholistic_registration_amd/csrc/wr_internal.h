@@ -1,0 +1,151 @@
+// Internal declarations shared by the translation units of libwrhip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wr_hip.h"
+
+namespace wr {
+
+// ---------------------------------------------------------------- errors
+
+void set_error(const std::string& msg);
+
+#define WR_HIP_CHECK(expr)                                                       \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            ::wr::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));  \
+            return WR_E_HIP;                                                     \
+        }                                                                        \
+    } while (0)
+
+// ---------------------------------------------------------------- buffers
+
+// A device allocation that only grows; reused across calls of one context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+int grow(DevBuf& b, size_t bytes);   // WR_OK or WR_E_MEM
+void release(DevBuf& b);
+
+// ---------------------------------------------------------------- resample
+
+// One axis of a MATLAB-style resize (imresize.py:60-82): output j is
+// sum_k w[j*K + k] * src[idx[j*K + k]].  Offsets into the table arena.
+struct Table {
+    int n_in = 0, n_out = 0, k = 0;
+    size_t w_off = 0;     // doubles
+    size_t i_off = 0;     // ints
+    bool identity = false;
+};
+
+enum Kernel { KERNEL_CUBIC = 0, KERNEL_TENT = 1 };
+
+// Builds the table exactly as the reference does (float64, numpy row-sum
+// order) and appends it to the host arena.
+Table build_table(int n_in, int n_out, Kernel kern, std::vector<double>& w_arena,
+                  std::vector<int>& i_arena);
+
+enum Elem { E_F32 = 0, E_U16 = 1, E_F64 = 2 };
+enum OutMode { OUT_F64 = 0, OUT_F32 = 1, OUT_MASK = 2 };
+
+// Resample along x: in (nz, ny, t.n_in) with element stride `in_stride`
+// (3 for one component of an interleaved field) and component offset
+// `in_comp`; out (nz, ny, t.n_out).
+void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
+                       OutMode om, double divisor, const Table& t, const double* w,
+                       const int* idx, int ny, int nz, hipStream_t s);
+// Resample along y: in (nz, t.n_in, nx) -> out (nz, t.n_out, nx).
+void launch_resample_y(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
+                       OutMode om, double divisor, const Table& t, const double* w,
+                       const int* idx, int nx, int nz, hipStream_t s);
+// Elementwise conversion (identity resample): out = convert(in).
+void launch_convert(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
+                    OutMode om, double divisor, size_t n, hipStream_t s);
+
+// ---------------------------------------------------------------- LK solver
+
+// Device control block of one level (one per context, reset per level).
+struct Ctl {
+    int done;            // level finished: every later kernel of the level exits
+    int rec;             // error records so far (len(currentError))
+    int upd;             // updates so far (len(max_diff_motion))
+    int nan;             // NaN components zeroed by the solve
+    int iter;            // option["iter"]
+    int frec, fupd;      // forced schedule of this level, -1 = natural
+    unsigned maxdiff;    // float bits of max |delta motion| of the current update
+    double old_err[3];   // the last three accepted errors (calFlow3d:426,528-530)
+    double tol;
+    double s_pen;        // smoothPenalty * (2r+1)^2
+    double inv_n;        // 1 / (x*y*z)
+    double last_err;
+};
+
+struct LevelGeom {
+    int nx, ny, nz;      // level size
+    int r, p;            // patch half width, p = 2r+1
+    int ncx, ncy;        // control points per row / column
+    int extra_x, extra_y;  // clamped repeats of the last column / row
+    float z_ratio;       // zRatio / 2^level
+    double z_ratio_d;
+    double mov_range;
+};
+
+struct LkTile {
+    int np_x;            // patches per workgroup along x
+    int nb_y;            // patch rows per workgroup
+    int cols;            // np_x * p owned columns
+    int rows;            // nb_y * p owned rows
+    int nbx, nby;        // workgroups along x, y
+    long long blocks;    // nbx * nby * nz
+};
+
+LkTile plan_tile(const LevelGeom& g, int np_x, int nb_y);
+
+struct LkBuffers {
+    const float* mov;        // level moving image (nz, ny, nx)
+    const float* ref;        // level template
+    const uint8_t* mref;     // level mask_ref (> 0 after resampling) or null
+    const float* mmov;       // level mask_mov (continuous) or null
+    const float* m0;         // dense field at level start, 3 planes, or null
+    float* cp;               // control-point field, 3 planes of ncx*ncy*nz
+    float* dnei;             // neighbour differences, 3 planes
+    float* sums;             // 9 planes of per-patch sums
+    double* errpart;         // per-workgroup sum of It^2
+    double* penpart;         // per-workgroup sum of |D|^2
+    Ctl* ctl;
+    double* log;             // [level][iter][4]: diff, pen, cur, maxdiff
+    int* counts;             // [level][3]: rec, upd, nan
+};
+
+void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
+                      double inv_n, hipStream_t s);
+void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, hipStream_t s);
+int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int level,
+                     int k, int iter, hipStream_t s);
+void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter,
+                      float* dense_soa, float* motion_aos, float* coords_aos, hipStream_t s);
+
+// ---------------------------------------------------------------- warp
+
+void launch_warp(const void* in, Elem in_t, int nx, int ny, int nz, const float* motion,
+                 float* out, hipStream_t s);
+
+// ---------------------------------------------------------------- context
+
+struct Context {
+    int device = 0;
+    DevBuf mov_l, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;
+    DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts;
+    DevBuf tab_w, tab_i;
+    wr_stats stats{};
+};
+
+}  // namespace wr
