@@ -1,0 +1,239 @@
+#!/usr/bin/env python
+"""Registered voxels/s of one full registration (getMotion + correctMotion)
+on one MI355X.
+
+Workload (default): a uint16 pair of 96 x 1024 x 2048 (z, y, x) voxels,
+400k Gaussian blobs, a smooth 8-mode displacement field with max |d| = 8
+voxels, a random ellipsoid mask covering 10 % of the voxels as mask_ref;
+layer 4 (five scales), 33x33x1 patches (r = 16), 20 iterations per level,
+float32 throughout.  One step = getMotion + correctMotion of the moving
+volume with the inputs already resident in HBM.  Data are synthetic,
+generated on the device from fixed seeds.
+
+Prints ONE JSON line (rank 0).  The roofline object is for the dominant
+kernel (the fused LK pass at level 0): algorithmic bytes per launch
+((2 * 4 + 1) B per level-0 voxel: moving and template float32 level images
+plus the mask byte) divided by its average duration, measured with HIP
+events recorded on the launch stream inside the timed steps.  cpu_baseline
+times the numpy/scipy oracle (the reference's CPU path restated) on a crop of
+the same pair, one thread.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # name: (z, y, x), blobs, modes, max|d|, mask fraction, layer, r, iter, seed
+    "plain_96x1024x2048_u16": dict(zyx=(96, 1024, 2048), n_blobs=400_000, n_modes=8, dmax=8.0,
+                                   mask_fraction=0.10, layer=4, r=16, iter=20, seed=1, u16=True),
+    "plain_64x128x128_f32": dict(zyx=(64, 128, 128), n_blobs=2000, n_modes=4, dmax=3.0,
+                                 mask_fraction=0.0, layer=3, r=8, iter=10, seed=0, u16=False),
+}
+DEFAULT = "plain_96x1024x2048_u16"
+HBM_PEAK_GBS = 8000.0
+RAW_PENALTY = 0.08
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_pair_device(w, dev):
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import synth
+    rng = np.random.default_rng(w["seed"])
+    zyx = w["zyx"]
+    blobs = synth.draw_blobs(rng, zyx, w["n_blobs"])
+    modes = synth.draw_modes(rng, w["n_modes"])
+    clean = synth.render_template_torch(zyx, blobs, dev)
+    d = synth.field_torch(zyx, modes, w["dmax"], dev)
+    warped = D.correct_motion(clean, (-d).contiguous())
+    del d
+    g = torch.Generator(device=dev)
+    g.manual_seed(w["seed"])
+    tmpl = torch.poisson(clean, generator=g)
+    mov = torch.poisson(warped.clamp_min(0), generator=g)
+    del clean, warped
+    if w["u16"]:
+        tmpl = tmpl.round_().clamp_(0, 65535).to(torch.uint16)
+        mov = mov.round_().clamp_(0, 65535).to(torch.uint16)
+    mask = None
+    if w["mask_fraction"] > 0:
+        mask = synth.ellipsoid_mask_torch(rng, zyx, w["mask_fraction"], dev).float().contiguous()
+    torch.cuda.synchronize()
+    return tmpl.contiguous(), mov.contiguous(), mask
+
+
+def smooth_penalty(tmpl, mask):
+    """getSmPnltNormFctr (preprocess.py:42-66) on the float32 template, in
+    float64 (the harness converts uint16 to float32 first)."""
+    import torch
+    ref = tmpl.float()
+    m = mask > 0 if mask is not None else torch.zeros_like(ref, dtype=torch.bool)
+    gx = ((ref[..., 2:] - ref[..., :-2]) / 2).double()
+    gy = ((ref[:, 2:, :] - ref[:, :-2, :]) / 2).double()
+    mx = (gx ** 2)[~m[..., 1:-1]].mean()
+    my = (gy ** 2)[~m[:, 1:-1, :]].mean()
+    return float((mx + my) / 2)
+
+
+def algorithmic_bytes(w, records, s_in, s_p=4, s_m=1):
+    """Bytes per registration: LK passes, pyramid, level-0 conversion, final
+    warp with motion write."""
+    nz, ny, nx = w["zyx"]
+    n0 = nx * ny * nz
+    tot = 0.0
+    for L in range(w["layer"] + 1):
+        nl = int(nx / 2 ** L) * int(ny / 2 ** L) * nz
+        tot += nl * records[L] * (2 * s_p + s_m)
+        if L >= 1:
+            tot += 2 * s_in * n0 + nl * 2 * s_p
+    if s_p != s_in:
+        tot += 2 * (s_in + s_p) * n0
+    tot += (s_in + 4 + 12) * n0
+    return tot
+
+
+def cpu_baseline(tmpl, mov, mask, w, crop):
+    """The oracle on a crop, one thread (scipy.ndimage is single-threaded)."""
+    from oracle import lk_oracle as O
+    cz, cy, cx = crop
+    nz, ny, nx = w["zyx"]
+    z0, y0, x0 = (nz - cz) // 2, (ny - cy) // 2, (nx - cx) // 2
+    sl = (slice(z0, z0 + cz), slice(y0, y0 + cy), slice(x0, x0 + cx))
+    ref = tmpl[sl].float().cpu().numpy().transpose(2, 1, 0)
+    mv = mov[sl].float().cpu().numpy().transpose(2, 1, 0)
+    mk = (mask[sl].cpu().numpy() > 0).transpose(2, 1, 0) if mask is not None else \
+        np.zeros(ref.shape, bool)
+    opt = dict(layer=w["layer"], iter=w["iter"], r=w["r"], zRatio=1.0, movRange=5.0, tol=1e-3,
+               save_ite=10 ** 9, mask_ref=mk, mask_mov=np.zeros(ref.shape, bool),
+               smoothPenalty=O.smooth_penalty_factor(ref, mk) * RAW_PENALTY)
+    t0 = time.perf_counter()
+    m, _, _, _ = O.get_motion(mv, ref, opt, keep_images=False)
+    O.correct_motion(mv, m)
+    dt = time.perf_counter() - t0
+    nvox = cx * cy * cz
+    return {"value": nvox / dt, "unit": "registered voxels/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (numpy/scipy restatement, ref-dtype mode, direct per-patch window "
+                      f"sums) on the centre crop {cz}x{cy}x{cx} (z,y,x) of the same pair, same "
+                      f"parameters, one thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-crop", default="4,512,1024", help="z,y,x of the CPU-baseline crop")
+    ap.add_argument("--tile-patches-x", type=int, default=0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames or z-slabs "
+              f"shard without communication, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from wholistic_registration_amd import _lib
+    from wholistic_registration_amd import device as D
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    w = WORKLOADS[args.workload]
+    nz, ny, nx = w["zyx"]
+    n0 = nx * ny * nz
+
+    log(f"[bench] generating {args.workload} on the device")
+    tmpl, mov, mask = make_pair_device(w, dev)
+    sp = smooth_penalty(tmpl, mask) * RAW_PENALTY
+    s_in = 2 if tmpl.dtype == torch.uint16 else 4
+    motion = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device=dev)
+    warped = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    params = dict(layer=w["layer"], iter=w["iter"], r=w["r"], zRatio=1.0, movRange=5.0,
+                  smoothPenalty=sp, tol=1e-3, tile_patches_x=args.tile_patches_x)
+
+    def step(timing):
+        res = D.get_motion(mov, tmpl, mask_ref=mask, motion_out=motion, timing=timing, **params)
+        D.correct_motion(mov, motion, out=warped)
+        return res
+
+    for i in range(args.warmup):
+        res = step(False)
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i}: records {res.records[::-1]} (levels {w['layer']}..0)")
+    torch.cuda.synchronize()
+    times, lk_ms, lk_n, recs = [], 0.0, 0, None
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = step(True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        st = D.stats()
+        lk_ms += st.lk_ms[0]
+        lk_n += st.lk_exec[0]
+        recs = res.records
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms, records {res.records[::-1]}")
+    total = time.perf_counter() - t_all
+    ms_per_step = 1e3 * total / args.steps
+    value = n0 / (total / args.steps)
+
+    # roofline of the level-0 LK pass
+    bytes_per_launch = (2 * 4 + (1 if mask is not None else 0)) * n0
+    avg_ms = lk_ms / max(lk_n, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    alg = algorithmic_bytes(w, recs, s_in, s_m=1 if mask is not None else 0)
+    out = {
+        "metric": "registered voxels/s at EPE<=0.1 vox vs FP64 oracle (plain FP32 path)",
+        "value": value, "unit": "registered voxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "zyx": list(w["zyx"]), "input_dtype": str(tmpl.dtype),
+                   "layer": w["layer"], "r": w["r"], "iter": w["iter"], "mask": mask is not None,
+                   "records_per_level": {f"L{L}": recs[L] for L in range(w["layer"], -1, -1)},
+                   "tile_cols": D.stats().tile_cols},
+        "roofline": {"bound": "hbm", "kernel": "k_lk_pass (level 0)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "bytes_per_launch": bytes_per_launch},
+        "pipeline_gbs": alg / (total / args.steps) / 1e9,
+        "algorithmic_bytes_per_step": alg,
+    }
+    if not args.no_cpu_baseline:
+        crop = tuple(int(v) for v in args.cpu_crop.split(","))
+        log(f"[bench] cpu baseline on crop {crop}")
+        out["cpu_baseline"] = cpu_baseline(tmpl, mov, mask, w, crop)
+    print(json.dumps(out), flush=True)
+    _ = _lib, math
+
+
+if __name__ == "__main__":
+    main()

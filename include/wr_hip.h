@@ -56,7 +56,7 @@ typedef struct wr_params {
     int layer;              /* option["layer"]: levels layer..0 are visited     */
     int iter;               /* option["iter"]: iterations per level             */
     int r;                  /* option["r"]: patch half-width, (2r+1)^2 x 1      */
-    int reserved0;
+    int timing;             /* 1 = time every LK pass with HIP events         */
     double zRatio;          /* option["zRatio"]                                 */
     double movRange;        /* option["movRange"]: per-update step cap          */
     double smoothPenalty;   /* option["smoothPenalty"] (already normalised)     */
@@ -128,13 +128,16 @@ int wr_imresize(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz,
                 int nx_out, int ny_out, int method, double* out, void* stream);
 
 /* Launch statistics of the last wr_get_motion on this context, for
- * measurement: number of LK passes launched and executed per level, the LK
- * kernel's tile shape, and the grid size. */
+ * measurement: LK passes launched per level, the LK kernel's tile shape and
+ * grid size, and (when params->timing was 1) the number and summed HIP-event
+ * duration of the passes that did work. */
 typedef struct wr_stats {
     int lk_launches[WR_MAX_LEVELS];
     int tile_cols;                /* owned columns per workgroup            */
     int tile_rows;                /* owned rows per workgroup               */
     int lk_blocks[WR_MAX_LEVELS]; /* workgroups per LK pass                  */
+    int lk_exec[WR_MAX_LEVELS];   /* LK passes that did work (timing = 1)   */
+    double lk_ms[WR_MAX_LEVELS];  /* summed duration of those passes, ms    */
 } wr_stats;
 int wr_get_stats(wr_ctx* ctx, wr_stats* out);
 

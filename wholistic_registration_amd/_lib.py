@@ -25,7 +25,7 @@ class WrParams(ctypes.Structure):
         ("layer", ctypes.c_int),
         ("iter", ctypes.c_int),
         ("r", ctypes.c_int),
-        ("reserved0", ctypes.c_int),
+        ("timing", ctypes.c_int),
         ("zRatio", ctypes.c_double),
         ("movRange", ctypes.c_double),
         ("smoothPenalty", ctypes.c_double),
@@ -56,6 +56,8 @@ class WrStats(ctypes.Structure):
         ("tile_cols", ctypes.c_int),
         ("tile_rows", ctypes.c_int),
         ("lk_blocks", ctypes.c_int * WR_MAX_LEVELS),
+        ("lk_exec", ctypes.c_int * WR_MAX_LEVELS),
+        ("lk_ms", ctypes.c_double * WR_MAX_LEVELS),
     ]
 
 

@@ -165,6 +165,7 @@ void wr_destroy(wr_ctx* ctx) {
     for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
                       &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.tab_w, &c.tab_i})
         release(*b);
+    for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     delete ctx;
 }
 
@@ -311,6 +312,15 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     fill(c.log.as<float>(), static_cast<size_t>(layer + 1) * iter * 8, 0.0f, s);
     fill(c.counts.as<float>(), static_cast<size_t>(layer + 1) * 3, 0.0f, s);
 
+    const bool timing = prm->timing != 0;
+    if (timing) {
+        const size_t need = static_cast<size_t>(layer + 1) * iter * 2;
+        while (c.events.size() < need) {
+            hipEvent_t e;
+            WR_HIP_CHECK(hipEventCreate(&e));
+            c.events.push_back(e);
+        }
+    }
     const double* tw = c.tab_w.as<double>();
     const int* ti = c.tab_i.as<int>();
     double* tmp = c.tmp.as<double>();
@@ -373,6 +383,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         b.ctl = c.ctl.as<Ctl>();
         b.log = c.log.as<double>() + static_cast<size_t>(L) * iter * 4;
         b.counts = c.counts.as<int>() + L * 3;
+        b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
         for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, L, k, iter, s));
         c.stats.lk_launches[L] = iter;
         c.stats.lk_blocks[L] = static_cast<int>(t.blocks);
@@ -391,6 +402,20 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     WR_HIP_CHECK(hipMemcpyAsync(hlog.data(), c.log.p, hlog.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     WR_HIP_CHECK(hipMemcpyAsync(hcnt.data(), c.counts.p, hcnt.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     WR_HIP_CHECK(hipStreamSynchronize(s));
+    if (timing) {
+        for (int L = 0; L <= layer; ++L) {
+            const int n = std::min(hcnt[L * 3 + 0], iter);
+            double tot = 0.0;
+            for (int k = 0; k < n; ++k) {
+                float ms = 0.0f;
+                const size_t e = (static_cast<size_t>(L) * iter + k) * 2;
+                WR_HIP_CHECK(hipEventElapsedTime(&ms, c.events[e], c.events[e + 1]));
+                tot += ms;
+            }
+            c.stats.lk_exec[L] = n;
+            c.stats.lk_ms[L] = tot;
+        }
+    }
     if (log) {
         for (int L = 0; L <= layer; ++L) {
             log->records[L] = hcnt[L * 3 + 0];

@@ -690,6 +690,7 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     const int threads = std::min(kMaxThreads, ((t.cols + 2 + 63) / 64) * 64);
     const dim3 grid(static_cast<unsigned>(t.blocks));
     const bool mr = b.mref != nullptr, mm = b.mmov != nullptr;
+    if (b.events) (void)hipEventRecord(b.events[2 * k], s);
     if (k == 0) {
         if (mr && mm) k_lk_pass<true, true, true><<<grid, threads, 0, s>>>(a);
         else if (mr) k_lk_pass<true, true, false><<<grid, threads, 0, s>>>(a);
@@ -701,6 +702,7 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
         else if (mm) k_lk_pass<false, false, true><<<grid, threads, 0, s>>>(a);
         else k_lk_pass<false, false, false><<<grid, threads, 0, s>>>(a);
     }
+    if (b.events) (void)hipEventRecord(b.events[2 * k + 1], s);
     k_err_finish<<<1, 1024, 0, s>>>(b.ctl, b.errpart, t.blocks, b.penpart, cpb, log_row + static_cast<size_t>(0),
                                     b.counts);
     k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d);

@@ -123,6 +123,7 @@ struct LkBuffers {
     Ctl* ctl;
     double* log;             // [level][iter][4]: diff, pen, cur, maxdiff
     int* counts;             // [level][3]: rec, upd, nan
+    hipEvent_t* events;      // null, or 2*iter events of this level bracketing the LK passes
 };
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
@@ -145,6 +146,7 @@ struct Context {
     DevBuf mov_l, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;
     DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts;
     DevBuf tab_w, tab_i;
+    std::vector<hipEvent_t> events;
     wr_stats stats{};
 };
 

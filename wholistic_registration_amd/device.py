@@ -90,7 +90,7 @@ class MotionResult:
 
 def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
-               schedule=None, tile_patches_x=0, motion_out=None, device=None):
+               schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
@@ -118,6 +118,7 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     prm.zRatio, prm.movRange = float(zRatio), float(movRange)
     prm.smoothPenalty, prm.tol = float(smoothPenalty), float(tol)
     prm.tile_patches_x = int(tile_patches_x)
+    prm.timing = 1 if timing else 0
     keep = []
     if schedule is not None:
         rec = (ctypes.c_int * (layer + 1))(*[schedule[L][0] for L in range(layer + 1)])
@@ -148,6 +149,14 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
         nans=[log.nans[L] for L in range(nlev)],
         diff_error=bufs["diff"], penalty_error=bufs["pen"], current_error=bufs["cur"],
         max_diff_motion=bufs["md"], final_error=float(log.final_error), layer=layer)
+
+
+def stats(device=0):
+    """Launch statistics of the last get_motion on ``device`` (HIP-event
+    durations of the LK passes when it ran with ``timing=True``)."""
+    st = _lib.WrStats()
+    _lib.check(_lib.load().wr_get_stats(_lib.context(device), ctypes.byref(st)), "wr_get_stats")
+    return st
 
 
 def correct_motion(vol, motion, out=None):
