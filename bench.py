@@ -61,7 +61,6 @@ def make_pair_device(w, dev):
     clean = synth.render_template_torch(zyx, blobs, dev)
     d = synth.field_torch(zyx, modes, w["dmax"], dev)
     warped = D.correct_motion(clean, (-d).contiguous())
-    del d
     g = torch.Generator(device=dev)
     g.manual_seed(w["seed"])
     tmpl = torch.poisson(clean, generator=g)
@@ -74,7 +73,7 @@ def make_pair_device(w, dev):
     if w["mask_fraction"] > 0:
         mask = synth.ellipsoid_mask_torch(rng, zyx, w["mask_fraction"], dev).float().contiguous()
     torch.cuda.synchronize()
-    return tmpl.contiguous(), mov.contiguous(), mask
+    return tmpl.contiguous(), mov.contiguous(), mask, d
 
 
 def smooth_penalty(tmpl, mask):
@@ -107,9 +106,16 @@ def algorithmic_bytes(w, records, s_in, s_p=4, s_m=1):
     return tot
 
 
-def cpu_baseline(tmpl, mov, mask, w, crop):
-    """The oracle on a crop, one thread (scipy.ndimage is single-threaded)."""
+def crop_checks(tmpl, mov, mask, w, crop, want_cpu, want_epe):
+    """On the centre crop of the pair: time the oracle in the reference's
+    dtype flow (the CPU baseline, one thread: scipy.ndimage is
+    single-threaded), and compare the GPU field with the all-float64 oracle
+    on the same crop (the EPE <= 0.1 voxel gate)."""
+    import torch
+
     from oracle import lk_oracle as O
+    from oracle.metrics import epe_stats
+    from wholistic_registration_amd import device as D
     cz, cy, cx = crop
     nz, ny, nx = w["zyx"]
     z0, y0, x0 = (nz - cz) // 2, (ny - cy) // 2, (nx - cx) // 2
@@ -118,18 +124,52 @@ def cpu_baseline(tmpl, mov, mask, w, crop):
     mv = mov[sl].float().cpu().numpy().transpose(2, 1, 0)
     mk = (mask[sl].cpu().numpy() > 0).transpose(2, 1, 0) if mask is not None else \
         np.zeros(ref.shape, bool)
+    sp = O.smooth_penalty_factor(ref, mk) * RAW_PENALTY
     opt = dict(layer=w["layer"], iter=w["iter"], r=w["r"], zRatio=1.0, movRange=5.0, tol=1e-3,
-               save_ite=10 ** 9, mask_ref=mk, mask_mov=np.zeros(ref.shape, bool),
-               smoothPenalty=O.smooth_penalty_factor(ref, mk) * RAW_PENALTY)
-    t0 = time.perf_counter()
-    m, _, _, _ = O.get_motion(mv, ref, opt, keep_images=False)
-    O.correct_motion(mv, m)
-    dt = time.perf_counter() - t0
+               save_ite=10 ** 9, mask_ref=mk, mask_mov=np.zeros(ref.shape, bool), smoothPenalty=sp)
+    out = {}
     nvox = cx * cy * cz
-    return {"value": nvox / dt, "unit": "registered voxels/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (numpy/scipy restatement, ref-dtype mode, direct per-patch window "
-                      f"sums) on the centre crop {cz}x{cy}x{cx} (z,y,x) of the same pair, same "
-                      f"parameters, one thread, {dt:.1f} s"}
+    if want_cpu:
+        t0 = time.perf_counter()
+        m, _, _, _ = O.get_motion(mv, ref, opt, keep_images=False)
+        O.correct_motion(mv, m)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": nvox / dt, "unit": "registered voxels/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (numpy/scipy restatement of the reference, its dtype flow, direct "
+                      f"per-patch window sums) on the centre crop {cz}x{cy}x{cx} (z,y,x) of the "
+                      f"same pair, same parameters, one thread, {dt:.1f} s"}
+    if want_epe:
+        m64, _, _, _ = O.get_motion(mv, ref, opt, mode="fp64", keep_images=False)
+        dev = tmpl.device
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(dev)
+        mref_t = t(mk.astype(np.float32)) if mk.any() else None
+        res = D.get_motion(t(mv), t(ref), mask_ref=mref_t, layer=w["layer"], iter=w["iter"],
+                           r=w["r"], zRatio=1.0, movRange=5.0, smoothPenalty=sp, tol=1e-3)
+        got = res.motion.cpu().numpy().transpose(2, 1, 0, 3)
+        out["epe_vs_fp64"] = dict(epe_stats(got, m64),
+                                  sample=f"GPU float32 path vs all-float64 oracle on the centre "
+                                         f"crop {cz}x{cy}x{cx} (z,y,x), both free-running")
+    return out
+
+
+def truth_epe(motion, d, mask):
+    """Endpoint error of the GPU field against the synthetic registration
+    field m = d(p + m) (fixed point, evaluated with the library's warp)."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    comps = [d[..., c].contiguous() for c in range(3)]
+    m = d.clone()
+    for _ in range(25):
+        m = torch.stack([D.correct_motion(comps[c], m) for c in range(3)], dim=-1).contiguous()
+    e = torch.sqrt(((motion - m) ** 2).sum(-1))
+    keep = mask <= 0 if mask is not None else torch.ones_like(e, dtype=torch.bool)
+    ek = e[keep].float()
+    q = torch.quantile(ek[:: max(1, ek.numel() // 4_000_000)], torch.tensor([0.5, 0.99], device=e.device))
+    return {"epe_mean": float(ek.mean()), "epe_median": float(q[0]), "epe_p99": float(q[1]),
+            "epe_max": float(ek.max()),
+            "sample": "full volume outside the mask, against the generating field"}
 
 
 def main():
@@ -139,6 +179,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-epe-check", action="store_true",
+                    help="skip the float64-oracle endpoint-error check on the crop")
     ap.add_argument("--cpu-crop", default="4,512,1024", help="z,y,x of the CPU-baseline crop")
     ap.add_argument("--tile-patches-x", type=int, default=0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -171,7 +213,7 @@ def main():
     n0 = nx * ny * nz
 
     log(f"[bench] generating {args.workload} on the device")
-    tmpl, mov, mask = make_pair_device(w, dev)
+    tmpl, mov, mask, field = make_pair_device(w, dev)
     sp = smooth_penalty(tmpl, mask) * RAW_PENALTY
     s_in = 2 if tmpl.dtype == torch.uint16 else 4
     motion = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device=dev)
@@ -227,10 +269,12 @@ def main():
         "pipeline_gbs": alg / (total / args.steps) / 1e9,
         "algorithmic_bytes_per_step": alg,
     }
-    if not args.no_cpu_baseline:
+    out["epe_vs_truth"] = truth_epe(motion, field, mask)
+    if not (args.no_cpu_baseline and args.no_epe_check):
         crop = tuple(int(v) for v in args.cpu_crop.split(","))
-        log(f"[bench] cpu baseline on crop {crop}")
-        out["cpu_baseline"] = cpu_baseline(tmpl, mov, mask, w, crop)
+        log(f"[bench] oracle runs on crop {crop}")
+        out.update(crop_checks(tmpl, mov, mask, w, crop, not args.no_cpu_baseline,
+                               not args.no_epe_check))
     print(json.dumps(out), flush=True)
     _ = _lib, math
 

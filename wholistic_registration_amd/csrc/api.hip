@@ -46,25 +46,29 @@ bool x_before_y(int nx_in, int nx_out, int ny_in, int ny_out) {
     return sx <= sy;
 }
 
-// in (nz, ny_in, nx_in) -> out (nz, ny_out, nx_out) through `tmp` (float64).
+// in (nz, ny_in, nx_in) -> out (nz, ny_out, nx_out) through `tmp` (float64);
+// `ol` places the output rows (pitch and leading pad, in elements).
 void resample2(const void* in, Elem in_t, int stride, int comp, void* out, OutMode om, double div,
-               const Plan2& pl, const double* w, const int* idx, int nz, double* tmp, hipStream_t s) {
+               const Plan2& pl, const double* w, const int* idx, int nz, double* tmp, OutLayout ol,
+               hipStream_t s) {
     const Table& tx = pl.tx;
     const Table& ty = pl.ty;
     if (tx.identity && ty.identity) {
-        launch_convert(in, in_t, stride, comp, out, om, div, static_cast<size_t>(tx.n_in) * ty.n_in * nz, s);
+        launch_convert(in, in_t, stride, comp, out, om, div, tx.n_in, static_cast<long long>(ty.n_in) * nz, ol, s);
     } else if (ty.identity) {
-        launch_resample_x(in, in_t, stride, comp, out, om, div, tx, w, idx, ty.n_in, nz, s);
+        launch_resample_x(in, in_t, stride, comp, out, om, div, tx, w, idx, ty.n_in, nz, ol, s);
     } else if (tx.identity) {
-        launch_resample_y(in, in_t, stride, comp, out, om, div, ty, w, idx, tx.n_in, nz, s);
+        launch_resample_y(in, in_t, stride, comp, out, om, div, ty, w, idx, tx.n_in, nz, ol, s);
     } else if (pl.x_first) {
-        launch_resample_x(in, in_t, stride, comp, tmp, OUT_F64, 1.0, tx, w, idx, ty.n_in, nz, s);
-        launch_resample_y(tmp, E_F64, 1, 0, out, om, div, ty, w, idx, tx.n_out, nz, s);
+        launch_resample_x(in, in_t, stride, comp, tmp, OUT_F64, 1.0, tx, w, idx, ty.n_in, nz, OutLayout{tx.n_out, 0}, s);
+        launch_resample_y(tmp, E_F64, 1, 0, out, om, div, ty, w, idx, tx.n_out, nz, ol, s);
     } else {
-        launch_resample_y(in, in_t, stride, comp, tmp, OUT_F64, 1.0, ty, w, idx, tx.n_in, nz, s);
-        launch_resample_x(tmp, E_F64, 1, 0, out, om, div, tx, w, idx, ty.n_out, nz, s);
+        launch_resample_y(in, in_t, stride, comp, tmp, OUT_F64, 1.0, ty, w, idx, tx.n_in, nz, OutLayout{tx.n_in, 0}, s);
+        launch_resample_x(tmp, E_F64, 1, 0, out, om, div, tx, w, idx, ty.n_out, nz, ol, s);
     }
 }
+
+OutLayout dense(int nx) { return OutLayout{nx, 0}; }
 
 size_t tmp_bytes(const Plan2& pl, int nz) {
     if (pl.tx.identity || pl.ty.identity) return 0;
@@ -163,7 +167,7 @@ void wr_destroy(wr_ctx* ctx) {
     Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
-                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.tab_w, &c.tab_i})
+                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     delete ctx;
@@ -183,6 +187,7 @@ int wr_imresize(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz, 
     WR_ARG(nx > 0 && ny > 0 && nz > 0 && nx_out > 0 && ny_out > 0, "wr_imresize: empty shape");
     WR_ARG(dtype == WR_F32 || dtype == WR_U16 || dtype == WR_F64, "wr_imresize: unsupported dtype");
     WR_ARG(method == 0 || method == 1, "unidentified kernel method supplied");
+    WR_ARG(nx <= kMaxRow && nx_out <= kMaxRow, "wr_imresize: x extent above 8192 voxels");
     Context& c = ctx->c;
     WR_HIP_CHECK(hipSetDevice(c.device));
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -192,7 +197,7 @@ int wr_imresize(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz, 
     WR_TRY(upload_tables(c, wa, ia));
     WR_TRY(grow(c.tmp, tmp_bytes(pl, nz)));
     resample2(in, elem_of(dtype), 1, 0, out, OUT_F64, 1.0, pl, c.tab_w.as<double>(), c.tab_i.as<int>(), nz,
-              c.tmp.as<double>(), s);
+              c.tmp.as<double>(), dense(nx_out), s);
     WR_HIP_CHECK(hipGetLastError());
     WR_HIP_CHECK(hipStreamSynchronize(s));
     return WR_OK;
@@ -205,6 +210,7 @@ int wr_pyramid_level(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int
     WR_ARG(level >= 0 && level < WR_MAX_LEVELS, "wr_pyramid_level: level out of range");
     const int lx = static_cast<int>(nx / std::ldexp(1.0, level)), ly = static_cast<int>(ny / std::ldexp(1.0, level));
     WR_ARG(lx > 0 && ly > 0 && nz > 0, "wr_pyramid_level: level has an empty axis");
+    WR_ARG(nx <= kMaxRow, "wr_pyramid_level: x extent above 8192 voxels");
     Context& c = ctx->c;
     WR_HIP_CHECK(hipSetDevice(c.device));
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -214,7 +220,7 @@ int wr_pyramid_level(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int
     WR_TRY(upload_tables(c, wa, ia));
     WR_TRY(grow(c.tmp, tmp_bytes(pl, nz)));
     resample2(in, elem_of(dtype), 1, 0, out, OUT_F32, 1.0, pl, c.tab_w.as<double>(), c.tab_i.as<int>(), nz,
-              c.tmp.as<double>(), s);
+              c.tmp.as<double>(), dense(lx), s);
     WR_HIP_CHECK(hipGetLastError());
     return WR_OK;
 }
@@ -236,6 +242,9 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     WR_ARG(ctx && mov && ref && prm && motion_out, "wr_get_motion: null argument");
     WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_get_motion: dtype must be WR_F32 or WR_U16");
     WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_get_motion: empty volume");
+    // the LK kernels index the three field planes with 32-bit offsets
+    WR_ARG(3.0 * nx * ny * nz < 2147483647.0, "wr_get_motion: volume exceeds 715M voxels");
+    WR_ARG(nx <= kMaxRow, "wr_get_motion: x extent above 8192 voxels");
     const int layer = prm->layer, iter = prm->iter, r = prm->r;
     WR_ARG(layer >= 0 && layer < WR_MAX_LEVELS, "wr_get_motion: layer out of range");
     WR_ARG(iter >= 1, "wr_get_motion: iter must be at least 1");
@@ -293,12 +302,15 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     const size_t n0 = static_cast<size_t>(nx) * ny * nz;
     WR_TRY(upload_tables(c, wa, ia));
     WR_TRY(grow(c.tmp, tmp_need));
-    if (layer > 0 || dtype != WR_F32) {
-        WR_TRY(grow(c.mov_l, n0 * sizeof(float)));
-        WR_TRY(grow(c.ref_l, n0 * sizeof(float)));
-    }
+    // the moving image (and mask_mov) of every level live in rows padded by
+    // kPad replicated voxels on both x ends; the template is dense
+    const int pitch0 = nx + 2 * kPad;
+    const size_t padded0 = static_cast<size_t>(pitch0) * ny * nz;
+    WR_ARG(3.0 * padded0 < 2147483647.0, "wr_get_motion: volume exceeds 715M voxels");
+    WR_TRY(grow(c.mov_l, padded0 * sizeof(float)));
+    if (layer > 0 || dtype != WR_F32) WR_TRY(grow(c.ref_l, n0 * sizeof(float)));
     if (mask_ref) WR_TRY(grow(c.mref_l, n0));
-    if (mask_mov) WR_TRY(grow(c.mmov_l, n0 * sizeof(float)));
+    if (mask_mov) WR_TRY(grow(c.mmov_l, padded0 * sizeof(float)));
     WR_TRY(grow(c.m0, 3 * n0 * sizeof(float)));
     if (layer > 0) WR_TRY(grow(c.mprev, 3 * nvox_prev * sizeof(float)));
     WR_TRY(grow(c.cp, 3 * ncp_max * sizeof(float)));
@@ -306,6 +318,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     WR_TRY(grow(c.sums, 9 * ncp_max * sizeof(float)));
     WR_TRY(grow(c.errpart, nblk_max * sizeof(double)));
     WR_TRY(grow(c.penpart, ncpb_max * sizeof(double)));
+    WR_TRY(grow(c.mdpart, mdpart_slots(static_cast<long long>(ncp_max)) * sizeof(float)));
     WR_TRY(grow(c.ctl, sizeof(Ctl)));
     WR_TRY(grow(c.log, static_cast<size_t>(layer + 1) * iter * 4 * sizeof(double)));
     WR_TRY(grow(c.counts, static_cast<size_t>(layer + 1) * 3 * sizeof(int)));
@@ -334,26 +347,32 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         const size_t nv = static_cast<size_t>(g.nx) * g.ny * nz;
 
         // level images (float32) and masks
-        const float* mov_l;
+        const int pitch = g.nx + 2 * kPad;
+        const long long rows = static_cast<long long>(g.ny) * nz;
+        const OutLayout padded{pitch, kPad};
+        resample2(mov, et, 1, 0, c.mov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, s);
+        launch_pad_rows(c.mov_l.as<float>(), rows, g.nx, pitch, s);
         const float* ref_l;
         if (L == 0 && dtype == WR_F32) {
-            mov_l = static_cast<const float*>(mov);
             ref_l = static_cast<const float*>(ref);
         } else {
-            resample2(mov, et, 1, 0, c.mov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, s);
-            resample2(ref, et, 1, 0, c.ref_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, s);
-            mov_l = c.mov_l.as<float>();
+            resample2(ref, et, 1, 0, c.ref_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), s);
             ref_l = c.ref_l.as<float>();
         }
-        if (mask_ref) resample2(mask_ref, E_F32, 1, 0, c.mref_l.p, OUT_MASK, 1.0, down[L], tw, ti, nz, tmp, s);
-        if (mask_mov) resample2(mask_mov, E_F32, 1, 0, c.mmov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, s);
+        if (mask_ref)
+            resample2(mask_ref, E_F32, 1, 0, c.mref_l.p, OUT_MASK, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), s);
+        if (mask_mov) {
+            resample2(mask_mov, E_F32, 1, 0, c.mmov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, s);
+            launch_pad_rows(c.mmov_l.as<float>(), rows, g.nx, pitch, s);
+        }
 
         // initial dense field of the level, 3 planes
         if (L == layer) {
             if (motion_init) {
                 const double div[3] = {1.0 * nx / g.nx, 1.0 * ny / g.ny, 1.0 * nz / g.nz};
                 for (int q = 0; q < 3; ++q)
-                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q], down[L], tw, ti, nz, tmp, s);
+                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q], down[L], tw, ti, nz, tmp,
+                              dense(g.nx), s);
             } else {
                 fill(m0, 3 * nv, 0.0f, s);
             }
@@ -361,7 +380,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             const size_t np = static_cast<size_t>(geo[L + 1].nx) * geo[L + 1].ny * nz;
             for (int q = 0; q < 3; ++q)
                 resample2(mprev + q * np, E_F32, 1, 0, m0 + q * nv, OUT_F32, q < 2 ? 0.5 : 1.0, up[L], tw, ti, nz, tmp,
-                          s);
+                          dense(g.nx), s);
         }
         launch_cp_init(g, m0, c.cp.as<float>(), s);
         const int frec = prm->forced_records ? prm->forced_records[L] : -1;
@@ -370,10 +389,11 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
                          static_cast<double>(nv), s);
 
         LkBuffers b;
-        b.mov = mov_l;
+        b.pitch = pitch;
+        b.mov = c.mov_l.as<float>() + kPad;
         b.ref = ref_l;
         b.mref = mask_ref ? c.mref_l.as<uint8_t>() : nullptr;
-        b.mmov = mask_mov ? c.mmov_l.as<float>() : nullptr;
+        b.mmov = mask_mov ? c.mmov_l.as<float>() + kPad : nullptr;
         b.m0 = m0;
         b.cp = c.cp.as<float>();
         b.dnei = c.dnei.as<float>();
@@ -383,6 +403,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         b.ctl = c.ctl.as<Ctl>();
         b.log = c.log.as<double>() + static_cast<size_t>(L) * iter * 4;
         b.counts = c.counts.as<int>() + L * 3;
+        b.mdpart = c.mdpart.as<float>();
         b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
         for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, L, k, iter, s));
         c.stats.lk_launches[L] = iter;

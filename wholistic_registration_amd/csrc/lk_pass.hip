@@ -1,0 +1,447 @@
+// The fused per-voxel pass of one LK iteration (k_lk_pass) on gfx950.
+//
+// Per voxel of a pyramid level (calFlow3d_Wei_v1.py:442-588): dense motion
+// (the CP bilinear interpolant, :265-292,627-640, or the level's initial
+// field on its first iteration), trilinear warp of the moving level image at
+// the deformed coordinate (:37-66), the mask warp (:473-474), the residual
+// with its 3x3x1 mean and mask (:477-485), the six boundary-clipped
+// central-difference samples (:134-206), and the nine per-patch
+// structure-tensor sums (:546-588) plus the residual energy (:101-131).
+//
+// Tiles: a workgroup owns `cols` = np_x*(2r+1) columns and `rows` = 2r+1
+// rows of one z slice, i.e. whole patches, so each per-patch sum is finished
+// inside one workgroup and written once (no atomics, a deterministic order).
+// Threads own one column each and walk the rows; the 3x3 residual mean uses
+// a register ring for the vertical sum and one LDS exchange per row for the
+// horizontal one.  Two extra lanes and two extra rows recompute the halo.
+// Workgroups are assigned XCD-major (each XCD gets a contiguous range of
+// tiles) with z fastest, so z-neighbour tiles, which share moving-image
+// slices, run on the same XCD and hit its L2.
+//
+// The kernel is VALU-bound (a wave64 FP32 instruction per 4 clocks per SIMD;
+// packed FP32 is not used), so the per-voxel instruction count is what is
+// minimised: the moving image rows are padded by kPad replicated voxels at
+// both x ends so every clamped tap row is one plain 16- or 8-byte load; row
+// addresses are 32-bit byte offsets from a workgroup-uniform base; the CP
+// interpolation weights of the tile's rows are formed once in LDS; the six
+// gradient samples take a branch-free fast path away from the faces.
+
+#include <cfloat>
+#include <cmath>
+
+#include "lk_common.h"
+#include "wr_internal.h"
+
+namespace wr {
+
+namespace {
+
+constexpr int kMaxThreads = 256;
+
+// Coordinate clamped to [0, hi] (map_coordinates mode='nearest') split into
+// floor and fraction.
+struct Samp {
+    int f;
+    float t;
+};
+__device__ __forceinline__ Samp split(float c, float hi) {
+    const float u = __builtin_amdgcn_fmed3f(c, 0.0f, hi);
+    const float fl = floorf(u);
+    Samp s;
+    s.f = static_cast<int>(fl);
+    s.t = u - fl;
+    return s;
+}
+
+// A level image with padded rows: voxel (x, y, z) is the float at byte
+// offset z*planeB + y*pitchB + (x + kPad)*4 of `buf`, for x in
+// [-kPad, nx - 1 + kPad]; the pads hold the edge voxel of the row.
+struct PVol {
+    const unsigned char* buf;
+    unsigned pitchB, planeB;
+    int nx, ny, nz;
+    float hx, hy, hz;      // n - 1
+};
+
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+
+template <class V>
+__device__ __forceinline__ V ld(const PVol& v, unsigned off) {
+    return *reinterpret_cast<const V*>(v.buf + off);
+}
+
+// Byte offsets of the clamped line points f-1, f, f+1, f+2 along an axis of
+// n points with stride `st`.
+struct Line4 {
+    unsigned o0, o1, o2, o3;
+};
+__device__ __forceinline__ Line4 line4(int f, int n, unsigned st) {
+    Line4 l;
+    l.o1 = static_cast<unsigned>(f) * st;
+    l.o0 = f > 0 ? l.o1 - st : l.o1;
+    l.o2 = f < n - 1 ? l.o1 + st : l.o1;
+    l.o3 = f < n - 2 ? l.o2 + st : l.o2;
+    return l;
+}
+
+// Sample of the 4-point line l[0..3] = f(clamp(c0 - 1 + q)) at the clipped
+// coordinate `c` (inside [0, n-1]), c0 = floor of the clamped centre.
+// Whenever the fraction is non-zero the two taps lie inside the line; with a
+// zero fraction the second tap has weight 0.
+__device__ __forceinline__ float line_sample(float l0, float l1, float l2, float l3, int c0, float c) {
+    const int f = static_cast<int>(floorf(c));
+    const float t = c - static_cast<float>(f);
+    const int o = clampi(f - c0 + 1, 0, 3);
+    const int o1 = min(o + 1, 3);
+    return lerpf(pick4(l0, l1, l2, l3, o), pick4(l0, l1, l2, l3, o1), t);
+}
+
+// T(clip(c + 1)) - T(clip(c - 1)) along one axis of the line.  Away from the
+// faces (1 <= c <= n - 2) neither sample is clipped: c - 1 is exact in
+// float32 and floors to c0 - 1 with the centre's fraction t0, and c + 1
+// floors to c0 + 1 or, rounded up to an integer, to c0 + 2 with a zero
+// fraction -- the same taps and weights the general form picks.
+__device__ __forceinline__ float line_diff(float l0, float l1, float l2, float l3, int c0, float t0, float c,
+                                           float hi) {
+    if (c >= 1.0f && c <= hi - 1.0f) {
+        const float cp = c + 1.0f;
+        const float fp = floorf(cp);
+        const float up = fp == static_cast<float>(c0 + 1) ? lerpf(l2, l3, cp - fp) : l3;
+        return up - lerpf(l0, l1, t0);
+    }
+    return line_sample(l0, l1, l2, l3, c0, __builtin_amdgcn_fmed3f(c + 1.0f, 0.0f, hi)) -
+           line_sample(l0, l1, l2, l3, c0, __builtin_amdgcn_fmed3f(c - 1.0f, 0.0f, hi));
+}
+
+struct Grad {
+    float w, gx, gy, gz;
+};
+
+// Warp value and central-difference gradients at the deformed coordinate
+// (cx, cy, cz) (calFlow3d:134-206): I_a = (T(c + e_a) - T(c - e_a)) / 2 where
+// only the perturbed coordinate is clipped to [0, n_a - 1] and the other two
+// follow map_coordinates' nearest mode.  With c0 = floor(clamp(c)) per axis,
+// every sample's taps lie on the 4-point lines c0-1..c0+2 (clamped), so the
+// 32 distinct voxels are the 2x2x2 cube around the centre plus, per axis,
+// the two planes one step further out: 4 x 16-byte rows holding X0..X3 at
+// (Y1|Y2, Z1|Z2) and 8 x 8-byte rows holding X1, X2 for the y and z lines.
+__device__ __forceinline__ Grad tri_grad(const PVol& v, float cx, float cy, float cz) {
+    const Samp sx = split(cx, v.hx), sy = split(cy, v.hy), sz = split(cz, v.hz);
+    const Line4 y = line4(sy.f, v.ny, v.pitchB);
+    const Line4 z = line4(sz.f, v.nz, v.planeB);
+    const unsigned xo = static_cast<unsigned>(sx.f + (kPad - 1)) * 4u;   // X0 = fx - 1
+    const unsigned b1 = z.o1 + xo, b2 = z.o2 + xo, b0 = z.o0 + xo + 4u, b3 = z.o3 + xo + 4u;
+    const f4u q11 = ld<f4u>(v, b1 + y.o1), q21 = ld<f4u>(v, b1 + y.o2);
+    const f4u q12 = ld<f4u>(v, b2 + y.o1), q22 = ld<f4u>(v, b2 + y.o2);
+    const f2u e01 = ld<f2u>(v, b1 + y.o0 + 4u), e02 = ld<f2u>(v, b2 + y.o0 + 4u);
+    const f2u e31 = ld<f2u>(v, b1 + y.o3 + 4u), e32 = ld<f2u>(v, b2 + y.o3 + 4u);
+    const f2u f10 = ld<f2u>(v, b0 + y.o1), f20 = ld<f2u>(v, b0 + y.o2);
+    const f2u f13 = ld<f2u>(v, b3 + y.o1), f23 = ld<f2u>(v, b3 + y.o2);
+    // central cube c[z][y][x]
+    const float c111 = q11.y, c112 = q11.z, c121 = q21.y, c122 = q21.z;
+    const float c211 = q12.y, c212 = q12.z, c221 = q22.y, c222 = q22.z;
+    // x line: bilinear in (y, z) of columns X0..X3
+    const float g1 = lerpf(lerpf(c111, c121, sy.t), lerpf(c211, c221, sy.t), sz.t);
+    const float g2 = lerpf(lerpf(c112, c122, sy.t), lerpf(c212, c222, sy.t), sz.t);
+    const float g0 = lerpf(lerpf(q11.x, q21.x, sy.t), lerpf(q12.x, q22.x, sy.t), sz.t);
+    const float g3 = lerpf(lerpf(q11.w, q21.w, sy.t), lerpf(q12.w, q22.w, sy.t), sz.t);
+    // x-interpolated central rows, shared by the y and z lines
+    const float x11 = lerpf(c111, c112, sx.t), x21 = lerpf(c121, c122, sx.t);
+    const float x12 = lerpf(c211, c212, sx.t), x22 = lerpf(c221, c222, sx.t);
+    // y line: bilinear in (x, z) of rows Y0..Y3
+    const float h1 = lerpf(x11, x12, sz.t);
+    const float h2 = lerpf(x21, x22, sz.t);
+    const float h0 = lerpf(lerpf(e01.x, e01.y, sx.t), lerpf(e02.x, e02.y, sx.t), sz.t);
+    const float h3 = lerpf(lerpf(e31.x, e31.y, sx.t), lerpf(e32.x, e32.y, sx.t), sz.t);
+    // z line: bilinear in (x, y) of slices Z0..Z3
+    const float k1 = lerpf(x11, x21, sy.t);
+    const float k2 = lerpf(x12, x22, sy.t);
+    const float k0 = lerpf(lerpf(f10.x, f10.y, sx.t), lerpf(f20.x, f20.y, sx.t), sy.t);
+    const float k3 = lerpf(lerpf(f13.x, f13.y, sx.t), lerpf(f23.x, f23.y, sx.t), sy.t);
+
+    Grad o;
+    o.w = lerpf(k1, k2, sz.t);
+    o.gx = line_diff(g0, g1, g2, g3, sx.f, sx.t, cx, v.hx) * 0.5f;
+    o.gy = line_diff(h0, h1, h2, h3, sy.f, sy.t, cy, v.hy) * 0.5f;
+    o.gz = line_diff(k0, k1, k2, k3, sz.f, sz.t, cz, v.hz) * 0.5f;
+    return o;
+}
+
+// Trilinear sample only (4 x 8-byte rows).
+__device__ __forceinline__ float tri(const PVol& v, float cx, float cy, float cz) {
+    const Samp sx = split(cx, v.hx), sy = split(cy, v.hy), sz = split(cz, v.hz);
+    const unsigned y1 = static_cast<unsigned>(sy.f) * v.pitchB;
+    const unsigned y2 = sy.f < v.ny - 1 ? y1 + v.pitchB : y1;
+    const unsigned xo = static_cast<unsigned>(sx.f + kPad) * 4u;
+    const unsigned z1 = static_cast<unsigned>(sz.f) * v.planeB + xo;
+    const unsigned z2 = (sz.f < v.nz - 1 ? z1 + v.planeB : z1);
+    const f2u r00 = ld<f2u>(v, z1 + y1), r01 = ld<f2u>(v, z1 + y2);
+    const f2u r10 = ld<f2u>(v, z2 + y1), r11 = ld<f2u>(v, z2 + y2);
+    const float a = lerpf(lerpf(r00.x, r00.y, sx.t), lerpf(r01.x, r01.y, sx.t), sy.t);
+    const float c = lerpf(lerpf(r10.x, r10.y, sx.t), lerpf(r11.x, r11.y, sx.t), sy.t);
+    return lerpf(a, c, sz.t);
+}
+
+struct PassArgs {
+    PVol mov;
+    PVol mmov;
+    const float* ref;
+    const uint8_t* mref;
+    const float* m0;
+    const float* cp;
+    float* sums;
+    double* errpart;
+    const Ctl* ctl;
+    int nx, ny, nz, r, p, ncx, ncy, extra_x, extra_y;
+    int cols, rows, nbx, nby, np_x;
+    long long blocks;
+    long long ncp;
+    float z_ratio;
+};
+
+// Decode the XCD-major, z-fastest tile order.
+__device__ __forceinline__ void tile_of(long long nb, int nz, int nbx, int& bx, int& by, int& z,
+                                        long long& logical) {
+    const long long bid = blockIdx.x;
+    const int xcd = static_cast<int>(bid & 7);
+    const long long slot = bid >> 3;
+    const long long per = nb >> 3;
+    const int rem = static_cast<int>(nb & 7);
+    logical = xcd * per + min(xcd, rem) + slot;
+    z = static_cast<int>(logical % nz);
+    const long long t2 = logical / nz;
+    bx = static_cast<int>(t2 % nbx);
+    by = static_cast<int>(t2 / nbx);
+}
+
+template <bool DENSE, bool MREF, bool MMOV>
+__global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
+    if (a.ctl->done) return;
+    __shared__ float s_v[2][kMaxThreads];
+    __shared__ float s_red[9][kMaxThreads];
+    __shared__ double s_err[kMaxThreads / 64];
+    __shared__ float s_rt[kMaxThreads];     // CP y-weight of each walked row
+    __shared__ int s_rj[kMaxThreads];       // its CP rows, relative to ja
+
+    int bx, by, z;
+    long long logical;
+    tile_of(a.blocks, a.nz, a.nbx, bx, by, z, logical);
+    const int tid = threadIdx.x;
+    const int x0 = bx * a.cols, y0 = by * a.rows;
+    const int col = x0 - 1 + tid;
+    // every lane samples a real (clamped) voxel; lanes outside the tile and
+    // its halo carry no weight
+    const int xc = clampi(col, 0, a.nx - 1);
+    const float xf = static_cast<float>(xc);
+    const float zf = static_cast<float>(z);
+    const bool owned = tid >= 1 && tid <= a.cols && col < a.nx;
+    const int cover_x = a.ncx * a.p, cover_y = a.ncy * a.p;
+    const int wx = (owned && col < cover_x) ? 1 + (col == a.nx - 1 ? a.extra_x : 0) : 0;
+    const float wxf = static_cast<float>(wx);
+    const int tl = max(tid - 1, 0), tr = min(tid + 1, static_cast<int>(blockDim.x) - 1);
+    const int plane = a.nx * a.ny;
+    const int nvox = plane * a.nz;
+    const int yend = min(y0 + a.rows, a.ny);
+    const int nsteps = (yend - y0) + 2;
+
+    // CP field: the walked rows interpolate between at most three CP rows
+    // (ja .. ja+2); the x-interpolated values of those rows are formed once
+    // per lane, the y weights of every row once per workgroup.
+    float vcx[3] = {0.0f, 0.0f, 0.0f}, vcy[3] = {0.0f, 0.0f, 0.0f}, vcz[3] = {0.0f, 0.0f, 0.0f};
+    if (!DENSE) {
+        const float* cpx = a.cp + static_cast<size_t>(z) * a.ncy * a.ncx;
+        const float* cpy = cpx + a.ncp;
+        const float* cpz = cpy + a.ncp;
+        const Axis1 axx = cp_axis(xc, a.r, a.p, a.ncx);
+        const int ja = cp_axis(clampi(y0 - 1, 0, a.ny - 1), a.r, a.p, a.ncy).i0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int jr = min(ja + q, a.ncy - 1) * a.ncx;
+            vcx[q] = lerpf(cpx[jr + axx.i0], cpx[jr + axx.i1], axx.t);
+            vcy[q] = lerpf(cpy[jr + axx.i0], cpy[jr + axx.i1], axx.t);
+            vcz[q] = lerpf(cpz[jr + axx.i0], cpz[jr + axx.i1], axx.t);
+        }
+        if (tid < nsteps) {
+            const Axis1 ay = cp_axis(clampi(y0 - 1 + tid, 0, a.ny - 1), a.r, a.p, a.ncy);
+            s_rt[tid] = ay.t;
+            s_rj[tid] = (ay.i0 - ja) | ((ay.i1 - ja) << 8);
+        }
+        __syncthreads();
+    }
+
+    float d0 = 0.0f, d1 = 0.0f;        // residual of the two previous rows
+    float pgx = 0.0f, pgy = 0.0f, pgz = 0.0f;
+    bool pmask = false;
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f, acc4 = 0.0f, acc5 = 0.0f, acc6 = 0.0f,
+          acc7 = 0.0f, acc8 = 0.0f;
+    double err = 0.0;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, bxv = 0.0f, byv = 0.0f, bzv = 0.0f;
+    int cur = -1;
+
+    for (int s = 0; s < nsteps; ++s) {
+        const int yy = y0 - 1 + s;
+        const int yc = clampi(yy, 0, a.ny - 1);
+        const bool grad_row = yy >= y0 && yy < yend && yy < cover_y;   // uniform
+        const int v = z * plane + yc * a.nx + xc;
+        float mx, my, mz;
+        if (DENSE) {
+            mx = a.m0[v];
+            my = a.m0[v + nvox];
+            mz = a.m0[v + 2 * nvox];
+        } else {
+            const int pr = __builtin_amdgcn_readfirstlane(s_rj[s]);
+            if (pr != cur) {   // uniform: the row crossed into the next CP interval
+                cur = pr;
+                const int j0 = pr & 255, j1 = pr >> 8;
+                ax = pick4(vcx[0], vcx[1], vcx[2], vcx[2], j0);
+                ay = pick4(vcy[0], vcy[1], vcy[2], vcy[2], j0);
+                az = pick4(vcz[0], vcz[1], vcz[2], vcz[2], j0);
+                bxv = pick4(vcx[0], vcx[1], vcx[2], vcx[2], j1);
+                byv = pick4(vcy[0], vcy[1], vcy[2], vcy[2], j1);
+                bzv = pick4(vcz[0], vcz[1], vcz[2], vcz[2], j1);
+            }
+            const float t = s_rt[s];
+            mx = lerpf(ax, bxv, t);
+            my = lerpf(ay, byv, t);
+            mz = lerpf(az, bzv, t);
+        }
+        const float cx = xf + mx;
+        const float cy = static_cast<float>(yc) + my;
+        const float cz = zf + mz;
+        float w, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        if (grad_row) {
+            const Grad g = tri_grad(a.mov, cx, cy, cz);
+            w = g.w;
+            gx = g.gx;
+            gy = g.gy;
+            gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
+        } else {
+            w = tri(a.mov, cx, cy, cz);
+        }
+        const float d = a.ref[v] - w;
+        bool mk = false;
+        if (MREF) mk = a.mref[v] != 0;
+        if (MMOV) mk = mk || tri(a.mmov, cx, cy, cz) > 0.0f;
+        if (s >= 2) {
+            const float vsum = d0 + d1 + d;
+            float* buf = s_v[s & 1];
+            buf[tid] = vsum;
+            __syncthreads();
+            const int yo = yy - 1;
+            float it = (buf[tl] + vsum + buf[tr]) * (1.0f / 9.0f);
+            if (pmask) it = 0.0f;
+            const float ie = owned ? it : 0.0f;
+            err += static_cast<double>(ie) * static_cast<double>(ie);
+            const float wyf = yo < cover_y ? static_cast<float>(1 + (yo == a.ny - 1 ? a.extra_y : 0)) : 0.0f;
+            const float m = pmask ? 0.0f : wxf * wyf;
+            const float mgx = m * pgx, mgy = m * pgy, mgz = m * pgz;
+            acc0 += mgx * pgx;
+            acc1 += mgx * pgy;
+            acc2 += mgx * pgz;
+            acc3 += mgy * pgy;
+            acc4 += mgy * pgz;
+            acc5 += mgz * pgz;
+            acc6 += mgx * it;
+            acc7 += mgy * it;
+            acc8 += mgz * it;
+        }
+        d0 = d1;
+        d1 = d;
+        pgx = gx;
+        pgy = gy;
+        pgz = gz;
+        pmask = mk;
+    }
+
+    // residual energy of the tile
+    for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
+    if ((tid & 63) == 0) s_err[tid >> 6] = err;
+    // per-column partial sums -> per-patch sums
+    s_red[0][tid] = acc0;
+    s_red[1][tid] = acc1;
+    s_red[2][tid] = acc2;
+    s_red[3][tid] = acc3;
+    s_red[4][tid] = acc4;
+    s_red[5][tid] = acc5;
+    s_red[6][tid] = acc6;
+    s_red[7][tid] = acc7;
+    s_red[8][tid] = acc8;
+    __syncthreads();
+    if (tid == 0) {
+        double e = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) e += s_err[w];
+        a.errpart[logical] = e;
+    }
+    if (by < a.ncy) {
+        const int ntask = a.np_x * 9;
+        for (int task = tid; task < ntask; task += blockDim.x) {
+            const int pi = task / 9, k = task - 9 * (task / 9);
+            const int gi = bx * a.np_x + pi;
+            if (gi >= a.ncx) continue;
+            double sacc = 0.0;
+            const int u0 = 1 + pi * a.p;
+            for (int u = 0; u < a.p; ++u) sacc += static_cast<double>(s_red[k][u0 + u]);
+            a.sums[static_cast<size_t>(k) * a.ncp + (static_cast<size_t>(z) * a.ncy + by) * a.ncx + gi] =
+                static_cast<float>(sacc);
+        }
+    }
+}
+
+PVol make_pvol(const float* base_x0, int pitch, int nx, int ny, int nz) {
+    PVol v;
+    v.buf = reinterpret_cast<const unsigned char*>(base_x0 - kPad);
+    v.pitchB = static_cast<unsigned>(pitch) * 4u;
+    v.planeB = static_cast<unsigned>(pitch) * static_cast<unsigned>(ny) * 4u;
+    v.nx = nx;
+    v.ny = ny;
+    v.nz = nz;
+    v.hx = static_cast<float>(nx - 1);
+    v.hy = static_cast<float>(ny - 1);
+    v.hz = static_cast<float>(nz - 1);
+    return v;
+}
+
+}  // namespace
+
+void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s) {
+    PassArgs a;
+    a.mov = make_pvol(b.mov, b.pitch, g.nx, g.ny, g.nz);
+    a.mmov = make_pvol(b.mmov ? b.mmov : b.mov, b.pitch, g.nx, g.ny, g.nz);
+    a.ref = b.ref;
+    a.mref = b.mref;
+    a.m0 = b.m0;
+    a.cp = b.cp;
+    a.sums = b.sums;
+    a.errpart = b.errpart;
+    a.ctl = b.ctl;
+    a.nx = g.nx;
+    a.ny = g.ny;
+    a.nz = g.nz;
+    a.r = g.r;
+    a.p = g.p;
+    a.ncx = g.ncx;
+    a.ncy = g.ncy;
+    a.extra_x = g.extra_x;
+    a.extra_y = g.extra_y;
+    a.cols = t.cols;
+    a.rows = t.rows;
+    a.nbx = t.nbx;
+    a.nby = t.nby;
+    a.np_x = t.np_x;
+    a.blocks = t.blocks;
+    a.ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
+    a.z_ratio = g.z_ratio;
+    const int threads = std::min(kMaxThreads, ((t.cols + 2 + 63) / 64) * 64);
+    const dim3 grid(static_cast<unsigned>(t.blocks));
+    const bool mr = b.mref != nullptr, mm = b.mmov != nullptr;
+    using Fn = void (*)(PassArgs);
+    static const Fn table[8] = {k_lk_pass<false, false, false>, k_lk_pass<false, false, true>,
+                                k_lk_pass<false, true, false>,  k_lk_pass<false, true, true>,
+                                k_lk_pass<true, false, false>,  k_lk_pass<true, false, true>,
+                                k_lk_pass<true, true, false>,   k_lk_pass<true, true, true>};
+    const int sel = (k == 0 ? 4 : 0) | (mr ? 2 : 0) | (mm ? 1 : 0);
+    hipLaunchKernelGGL(table[sel], grid, dim3(threads), 0, s, a);
+}
+
+}  // namespace wr

@@ -44,6 +44,10 @@ struct Table {
     size_t w_off = 0;     // doubles
     size_t i_off = 0;     // ints
     bool identity = false;
+    // x-pass blocking: outputs in groups of xo_blk columns; for group g the
+    // source taps lie in [span[2g], span[2g+1]] (ints at s_off in the arena)
+    int xo_blk = 1, n_cb = 0, span_max = 0;
+    size_t s_off = 0;
 };
 
 enum Kernel { KERNEL_CUBIC = 0, KERNEL_TENT = 1 };
@@ -54,21 +58,38 @@ Table build_table(int n_in, int n_out, Kernel kern, std::vector<double>& w_arena
                   std::vector<int>& i_arena);
 
 enum Elem { E_F32 = 0, E_U16 = 1, E_F64 = 2 };
+
+// Longest x row accepted by the resampling and LK kernels.
+constexpr int kMaxRow = 8192;
+// Source span (inputs) of one x-pass column group staged per row in LDS.
+constexpr int kXSpan = 191;
 enum OutMode { OUT_F64 = 0, OUT_F32 = 1, OUT_MASK = 2 };
+
+// Replicated voxels at each x end of a padded row (the moving level image).
+constexpr int kPad = 4;
+
+// Output placement of a resample: row `r`, column `x` goes to element
+// r * pitch + pad + x (pitch = row length, pad = 0 for a dense array).
+struct OutLayout {
+    int pitch;
+    int pad;
+};
 
 // Resample along x: in (nz, ny, t.n_in) with element stride `in_stride`
 // (3 for one component of an interleaved field) and component offset
 // `in_comp`; out (nz, ny, t.n_out).
 void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                        OutMode om, double divisor, const Table& t, const double* w,
-                       const int* idx, int ny, int nz, hipStream_t s);
+                       const int* idx, int ny, int nz, OutLayout ol, hipStream_t s);
 // Resample along y: in (nz, t.n_in, nx) -> out (nz, t.n_out, nx).
 void launch_resample_y(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                        OutMode om, double divisor, const Table& t, const double* w,
-                       const int* idx, int nx, int nz, hipStream_t s);
-// Elementwise conversion (identity resample): out = convert(in).
+                       const int* idx, int nx, int nz, OutLayout ol, hipStream_t s);
+// Elementwise conversion (identity resample) of nrows rows of nx: out = convert(in).
 void launch_convert(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
-                    OutMode om, double divisor, size_t n, hipStream_t s);
+                    OutMode om, double divisor, int nx, long long nrows, OutLayout ol, hipStream_t s);
+// Fill the kPad replicated voxels at both ends of every padded float row.
+void launch_pad_rows(float* buf, long long nrows, int nx, int pitch, hipStream_t s);
 
 // ---------------------------------------------------------------- LK solver
 
@@ -110,10 +131,11 @@ struct LkTile {
 LkTile plan_tile(const LevelGeom& g, int np_x, int nb_y);
 
 struct LkBuffers {
-    const float* mov;        // level moving image (nz, ny, nx)
+    int pitch;               // row pitch of the padded mov / mmov images
+    const float* mov;        // level moving image, padded rows, x = 0 of row 0
     const float* ref;        // level template
     const uint8_t* mref;     // level mask_ref (> 0 after resampling) or null
-    const float* mmov;       // level mask_mov (continuous) or null
+    const float* mmov;       // level mask_mov (continuous, padded like mov) or null
     const float* m0;         // dense field at level start, 3 planes, or null
     float* cp;               // control-point field, 3 planes of ncx*ncy*nz
     float* dnei;             // neighbour differences, 3 planes
@@ -124,7 +146,14 @@ struct LkBuffers {
     double* log;             // [level][iter][4]: diff, pen, cur, maxdiff
     int* counts;             // [level][3]: rec, upd, nan
     hipEvent_t* events;      // null, or 2*iter events of this level bracketing the LK passes
+    float* mdpart;           // per-workgroup max |delta motion| partials
 };
+
+// Partial-maximum slots needed by one iteration with `ncp` control points.
+size_t mdpart_slots(long long ncp);
+
+// The fused per-voxel pass of iteration k (lk_pass.hip).
+void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s);
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
                       double inv_n, hipStream_t s);
@@ -144,7 +173,7 @@ void launch_warp(const void* in, Elem in_t, int nx, int ny, int nz, const float*
 struct Context {
     int device = 0;
     DevBuf mov_l, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;
-    DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts;
+    DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts, mdpart;
     DevBuf tab_w, tab_i;
     std::vector<hipEvent_t> events;
     wr_stats stats{};
