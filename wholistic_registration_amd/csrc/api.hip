@@ -249,6 +249,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     WR_ARG(layer >= 0 && layer < WR_MAX_LEVELS, "wr_get_motion: layer out of range");
     WR_ARG(iter >= 1, "wr_get_motion: iter must be at least 1");
     WR_ARG(r >= 0, "wr_get_motion: r must be non-negative");
+    WR_ARG(2 * r + 1 <= kMaxTileCols, "wr_get_motion: r above 123 (patch wider than one workgroup tile)");
     WR_ARG(prm->zRatio != 0.0 && std::isfinite(prm->zRatio), "wr_get_motion: zRatio must be finite, non-zero");
     WR_ARG(prm->movRange > 0.0, "wr_get_motion: movRange must be positive");
     const int p = 2 * r + 1;

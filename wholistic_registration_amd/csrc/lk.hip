@@ -35,7 +35,6 @@ namespace wr {
 
 namespace {
 
-constexpr int kMaxThreads = 256;
 
 // ---------------------------------------------------------------- CP kernels
 
@@ -334,7 +333,8 @@ size_t mdpart_slots(long long ncp) { return static_cast<size_t>((ncp + 255) / 25
 
 LkTile plan_tile(const LevelGeom& g, int np_x, int nb_y) {
     LkTile t{};
-    const int maxp = std::max(1, (kMaxThreads - 2) / g.p);
+    // a workgroup has at most 4 waves of 62 owned columns each
+    const int maxp = std::max(1, kMaxTileCols / g.p);
     int npx;
     if (np_x > 0) {
         npx = std::min(np_x, maxp);

@@ -37,6 +37,7 @@ namespace wr {
 namespace {
 
 constexpr int kMaxThreads = 256;
+constexpr int kWaveCols = 62;   // tile columns owned by one wave (64 lanes - 2 halo)
 
 // Coordinate clamped to [0, hi] (map_coordinates mode='nearest') split into
 // floor and fraction.
@@ -118,53 +119,89 @@ struct Grad {
     float w, gx, gy, gz;
 };
 
-// Warp value and central-difference gradients at the deformed coordinate
-// (cx, cy, cz) (calFlow3d:134-206): I_a = (T(c + e_a) - T(c - e_a)) / 2 where
-// only the perturbed coordinate is clipped to [0, n_a - 1] and the other two
-// follow map_coordinates' nearest mode.  With c0 = floor(clamp(c)) per axis,
-// every sample's taps lie on the 4-point lines c0-1..c0+2 (clamped), so the
-// 32 distinct voxels are the 2x2x2 cube around the centre plus, per axis,
-// the two planes one step further out: 4 x 16-byte rows holding X0..X3 at
-// (Y1|Y2, Z1|Z2) and 8 x 8-byte rows holding X1, X2 for the y and z lines.
-__device__ __forceinline__ Grad tri_grad(const PVol& v, float cx, float cy, float cz) {
-    const Samp sx = split(cx, v.hx), sy = split(cy, v.hy), sz = split(cz, v.hz);
-    const Line4 y = line4(sy.f, v.ny, v.pitchB);
-    const Line4 z = line4(sz.f, v.nz, v.planeB);
-    const unsigned xo = static_cast<unsigned>(sx.f + (kPad - 1)) * 4u;   // X0 = fx - 1
+// The 32 voxels around one deformed coordinate, as 4 x 16-byte rows holding
+// X0..X3 at (Y1|Y2, Z1|Z2) and 8 x 8-byte rows holding X1, X2 for the y and
+// z lines (see tri_grad).
+struct Taps {
+    f4u q11, q21, q12, q22;
+    f2u e01, e02, e31, e32, f10, f20, f13, f23;
+};
+
+struct Coord {
+    float cx, cy, cz;
+    Samp sx, sy, sz;
+};
+
+__device__ __forceinline__ Coord make_coord(const PVol& v, float cx, float cy, float cz) {
+    Coord c;
+    c.cx = cx;
+    c.cy = cy;
+    c.cz = cz;
+    c.sx = split(cx, v.hx);
+    c.sy = split(cy, v.hy);
+    c.sz = split(cz, v.hz);
+    return c;
+}
+
+__device__ __forceinline__ Taps gather(const PVol& v, const Coord& c) {
+    const Line4 y = line4(c.sy.f, v.ny, v.pitchB);
+    const Line4 z = line4(c.sz.f, v.nz, v.planeB);
+    const unsigned xo = static_cast<unsigned>(c.sx.f + (kPad - 1)) * 4u;   // X0 = fx - 1
     const unsigned b1 = z.o1 + xo, b2 = z.o2 + xo, b0 = z.o0 + xo + 4u, b3 = z.o3 + xo + 4u;
-    const f4u q11 = ld<f4u>(v, b1 + y.o1), q21 = ld<f4u>(v, b1 + y.o2);
-    const f4u q12 = ld<f4u>(v, b2 + y.o1), q22 = ld<f4u>(v, b2 + y.o2);
-    const f2u e01 = ld<f2u>(v, b1 + y.o0 + 4u), e02 = ld<f2u>(v, b2 + y.o0 + 4u);
-    const f2u e31 = ld<f2u>(v, b1 + y.o3 + 4u), e32 = ld<f2u>(v, b2 + y.o3 + 4u);
-    const f2u f10 = ld<f2u>(v, b0 + y.o1), f20 = ld<f2u>(v, b0 + y.o2);
-    const f2u f13 = ld<f2u>(v, b3 + y.o1), f23 = ld<f2u>(v, b3 + y.o2);
+    Taps t;
+    t.q11 = ld<f4u>(v, b1 + y.o1);
+    t.q21 = ld<f4u>(v, b1 + y.o2);
+    t.q12 = ld<f4u>(v, b2 + y.o1);
+    t.q22 = ld<f4u>(v, b2 + y.o2);
+    t.e01 = ld<f2u>(v, b1 + y.o0 + 4u);
+    t.e02 = ld<f2u>(v, b2 + y.o0 + 4u);
+    t.e31 = ld<f2u>(v, b1 + y.o3 + 4u);
+    t.e32 = ld<f2u>(v, b2 + y.o3 + 4u);
+    t.f10 = ld<f2u>(v, b0 + y.o1);
+    t.f20 = ld<f2u>(v, b0 + y.o2);
+    t.f13 = ld<f2u>(v, b3 + y.o1);
+    t.f23 = ld<f2u>(v, b3 + y.o2);
+    return t;
+}
+
+// Warp value and central-difference gradients at the deformed coordinate
+// (calFlow3d:134-206): I_a = (T(c + e_a) - T(c - e_a)) / 2 where only the
+// perturbed coordinate is clipped to [0, n_a - 1] and the other two follow
+// map_coordinates' nearest mode.  With c0 = floor(clamp(c)) per axis, every
+// sample's taps lie on the 4-point lines c0-1..c0+2 (clamped), so the 32
+// distinct voxels are the 2x2x2 cube around the centre plus, per axis, the
+// two planes one step further out.
+__device__ __forceinline__ Grad eval(const PVol& v, const Coord& c, const Taps& q) {
+    const Samp& sx = c.sx;
+    const Samp& sy = c.sy;
+    const Samp& sz = c.sz;
     // central cube c[z][y][x]
-    const float c111 = q11.y, c112 = q11.z, c121 = q21.y, c122 = q21.z;
-    const float c211 = q12.y, c212 = q12.z, c221 = q22.y, c222 = q22.z;
+    const float c111 = q.q11.y, c112 = q.q11.z, c121 = q.q21.y, c122 = q.q21.z;
+    const float c211 = q.q12.y, c212 = q.q12.z, c221 = q.q22.y, c222 = q.q22.z;
     // x line: bilinear in (y, z) of columns X0..X3
     const float g1 = lerpf(lerpf(c111, c121, sy.t), lerpf(c211, c221, sy.t), sz.t);
     const float g2 = lerpf(lerpf(c112, c122, sy.t), lerpf(c212, c222, sy.t), sz.t);
-    const float g0 = lerpf(lerpf(q11.x, q21.x, sy.t), lerpf(q12.x, q22.x, sy.t), sz.t);
-    const float g3 = lerpf(lerpf(q11.w, q21.w, sy.t), lerpf(q12.w, q22.w, sy.t), sz.t);
+    const float g0 = lerpf(lerpf(q.q11.x, q.q21.x, sy.t), lerpf(q.q12.x, q.q22.x, sy.t), sz.t);
+    const float g3 = lerpf(lerpf(q.q11.w, q.q21.w, sy.t), lerpf(q.q12.w, q.q22.w, sy.t), sz.t);
     // x-interpolated central rows, shared by the y and z lines
     const float x11 = lerpf(c111, c112, sx.t), x21 = lerpf(c121, c122, sx.t);
     const float x12 = lerpf(c211, c212, sx.t), x22 = lerpf(c221, c222, sx.t);
     // y line: bilinear in (x, z) of rows Y0..Y3
     const float h1 = lerpf(x11, x12, sz.t);
     const float h2 = lerpf(x21, x22, sz.t);
-    const float h0 = lerpf(lerpf(e01.x, e01.y, sx.t), lerpf(e02.x, e02.y, sx.t), sz.t);
-    const float h3 = lerpf(lerpf(e31.x, e31.y, sx.t), lerpf(e32.x, e32.y, sx.t), sz.t);
+    const float h0 = lerpf(lerpf(q.e01.x, q.e01.y, sx.t), lerpf(q.e02.x, q.e02.y, sx.t), sz.t);
+    const float h3 = lerpf(lerpf(q.e31.x, q.e31.y, sx.t), lerpf(q.e32.x, q.e32.y, sx.t), sz.t);
     // z line: bilinear in (x, y) of slices Z0..Z3
     const float k1 = lerpf(x11, x21, sy.t);
     const float k2 = lerpf(x12, x22, sy.t);
-    const float k0 = lerpf(lerpf(f10.x, f10.y, sx.t), lerpf(f20.x, f20.y, sx.t), sy.t);
-    const float k3 = lerpf(lerpf(f13.x, f13.y, sx.t), lerpf(f23.x, f23.y, sx.t), sy.t);
+    const float k0 = lerpf(lerpf(q.f10.x, q.f10.y, sx.t), lerpf(q.f20.x, q.f20.y, sx.t), sy.t);
+    const float k3 = lerpf(lerpf(q.f13.x, q.f13.y, sx.t), lerpf(q.f23.x, q.f23.y, sx.t), sy.t);
 
     Grad o;
     o.w = lerpf(k1, k2, sz.t);
-    o.gx = line_diff(g0, g1, g2, g3, sx.f, sx.t, cx, v.hx) * 0.5f;
-    o.gy = line_diff(h0, h1, h2, h3, sy.f, sy.t, cy, v.hy) * 0.5f;
-    o.gz = line_diff(k0, k1, k2, k3, sz.f, sz.t, cz, v.hz) * 0.5f;
+    o.gx = line_diff(g0, g1, g2, g3, sx.f, sx.t, c.cx, v.hx) * 0.5f;
+    o.gy = line_diff(h0, h1, h2, h3, sy.f, sy.t, c.cy, v.hy) * 0.5f;
+    o.gz = line_diff(k0, k1, k2, k3, sz.f, sz.t, c.cz, v.hz) * 0.5f;
     return o;
 }
 
@@ -215,10 +252,13 @@ __device__ __forceinline__ void tile_of(long long nb, int nz, int nbx, int& bx, 
     by = static_cast<int>(t2 / nbx);
 }
 
+// Each wave owns kWaveCols consecutive columns of the tile plus one halo
+// lane on each side; waves walk the rows independently (the 3x3 mean's
+// horizontal neighbours come from cross-lane shuffles), and the next row's
+// gathers are issued before the current row is evaluated.
 template <bool DENSE, bool MREF, bool MMOV>
 __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     if (a.ctl->done) return;
-    __shared__ float s_v[2][kMaxThreads];
     __shared__ float s_red[9][kMaxThreads];
     __shared__ double s_err[kMaxThreads / 64];
     __shared__ float s_rt[kMaxThreads];     // CP y-weight of each walked row
@@ -228,18 +268,19 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     long long logical;
     tile_of(a.blocks, a.nz, a.nbx, bx, by, z, logical);
     const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
     const int x0 = bx * a.cols, y0 = by * a.rows;
-    const int col = x0 - 1 + tid;
-    // every lane samples a real (clamped) voxel; lanes outside the tile and
-    // its halo carry no weight
+    const int lcol = wave * kWaveCols + lane - 1;        // column within the tile
+    const int col = x0 + lcol;
+    // every lane samples a real (clamped) voxel; lanes outside the tile carry
+    // no weight
     const int xc = clampi(col, 0, a.nx - 1);
     const float xf = static_cast<float>(xc);
     const float zf = static_cast<float>(z);
-    const bool owned = tid >= 1 && tid <= a.cols && col < a.nx;
+    const bool owned = lane >= 1 && lane <= kWaveCols && lcol < a.cols && col < a.nx;
     const int cover_x = a.ncx * a.p, cover_y = a.ncy * a.p;
     const int wx = (owned && col < cover_x) ? 1 + (col == a.nx - 1 ? a.extra_x : 0) : 0;
     const float wxf = static_cast<float>(wx);
-    const int tl = max(tid - 1, 0), tr = min(tid + 1, static_cast<int>(blockDim.x) - 1);
     const int plane = a.nx * a.ny;
     const int nvox = plane * a.nz;
     const int yend = min(y0 + a.rows, a.ny);
@@ -262,30 +303,22 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
             vcy[q] = lerpf(cpy[jr + axx.i0], cpy[jr + axx.i1], axx.t);
             vcz[q] = lerpf(cpz[jr + axx.i0], cpz[jr + axx.i1], axx.t);
         }
-        if (tid < nsteps) {
-            const Axis1 ay = cp_axis(clampi(y0 - 1 + tid, 0, a.ny - 1), a.r, a.p, a.ncy);
-            s_rt[tid] = ay.t;
-            s_rj[tid] = (ay.i0 - ja) | ((ay.i1 - ja) << 8);
+        for (int i = tid; i < nsteps; i += blockDim.x) {
+            const Axis1 ay = cp_axis(clampi(y0 - 1 + i, 0, a.ny - 1), a.r, a.p, a.ncy);
+            s_rt[i] = ay.t;
+            s_rj[i] = (ay.i0 - ja) | ((ay.i1 - ja) << 8);
         }
         __syncthreads();
     }
 
-    float d0 = 0.0f, d1 = 0.0f;        // residual of the two previous rows
-    float pgx = 0.0f, pgy = 0.0f, pgz = 0.0f;
-    bool pmask = false;
-    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f, acc4 = 0.0f, acc5 = 0.0f, acc6 = 0.0f,
-          acc7 = 0.0f, acc8 = 0.0f;
-    double err = 0.0;
     float ax = 0.0f, ay = 0.0f, az = 0.0f, bxv = 0.0f, byv = 0.0f, bzv = 0.0f;
     int cur = -1;
-
-    for (int s = 0; s < nsteps; ++s) {
-        const int yy = y0 - 1 + s;
-        const int yc = clampi(yy, 0, a.ny - 1);
-        const bool grad_row = yy >= y0 && yy < yend && yy < cover_y;   // uniform
-        const int v = z * plane + yc * a.nx + xc;
+    // deformed coordinate of row s of this lane
+    auto coord_of = [&](int s) {
+        const int yc = clampi(y0 - 1 + s, 0, a.ny - 1);
         float mx, my, mz;
         if (DENSE) {
+            const int v = z * plane + yc * a.nx + xc;
             mx = a.m0[v];
             my = a.m0[v + nvox];
             mz = a.m0[v + 2 * nvox];
@@ -306,67 +339,78 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
             my = lerpf(ay, byv, t);
             mz = lerpf(az, bzv, t);
         }
-        const float cx = xf + mx;
-        const float cy = static_cast<float>(yc) + my;
-        const float cz = zf + mz;
-        float w, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        if (grad_row) {
-            const Grad g = tri_grad(a.mov, cx, cy, cz);
-            w = g.w;
-            gx = g.gx;
-            gy = g.gy;
-            gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
-        } else {
-            w = tri(a.mov, cx, cy, cz);
+        return make_coord(a.mov, xf + mx, static_cast<float>(yc) + my, zf + mz);
+    };
+    auto voxel_of = [&](int s) { return z * plane + clampi(y0 - 1 + s, 0, a.ny - 1) * a.nx + xc; };
+
+    float d0 = 0.0f, d1 = 0.0f;        // residual of the two previous rows
+    float pgx = 0.0f, pgy = 0.0f, pgz = 0.0f;
+    bool pmask = false;
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, acc3 = 0.0f, acc4 = 0.0f, acc5 = 0.0f, acc6 = 0.0f,
+          acc7 = 0.0f, acc8 = 0.0f;
+    double err = 0.0;
+
+    if (wave * kWaveCols < a.cols) {   // uniform: the wave has tile columns
+        for (int s = 0; s < nsteps; ++s) {
+            const Coord cc = coord_of(s);
+            const Taps tc = gather(a.mov, cc);
+            const int vc = voxel_of(s);
+            const float rc = a.ref[vc];
+            const int yy = y0 - 1 + s;
+            const Grad g = eval(a.mov, cc, tc);
+            const float gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
+            const float d = rc - g.w;
+            bool mk = false;
+            if (MREF) mk = a.mref[vc] != 0;
+            if (MMOV) mk = mk || tri(a.mmov, cc.cx, cc.cy, cc.cz) > 0.0f;
+            if (s >= 2) {
+                const float vsum = d0 + d1 + d;
+                const float left = __shfl_up(vsum, 1, 64);
+                const float right = __shfl_down(vsum, 1, 64);
+                const int yo = yy - 1;
+                float it = (left + vsum + right) * (1.0f / 9.0f);
+                if (pmask) it = 0.0f;
+                const float ie = owned ? it : 0.0f;
+                err += static_cast<double>(ie) * static_cast<double>(ie);
+                const float wyf = yo < cover_y ? static_cast<float>(1 + (yo == a.ny - 1 ? a.extra_y : 0)) : 0.0f;
+                const float m = pmask ? 0.0f : wxf * wyf;
+                const float mgx = m * pgx, mgy = m * pgy, mgz = m * pgz;
+                acc0 += mgx * pgx;
+                acc1 += mgx * pgy;
+                acc2 += mgx * pgz;
+                acc3 += mgy * pgy;
+                acc4 += mgy * pgz;
+                acc5 += mgz * pgz;
+                acc6 += mgx * it;
+                acc7 += mgy * it;
+                acc8 += mgz * it;
+            }
+            d0 = d1;
+            d1 = d;
+            // rows outside [y0, yend) and the uncovered tail carry no gradient
+            const bool grad_row = yy >= y0 && yy < yend && yy < cover_y;
+            pgx = grad_row ? g.gx : 0.0f;
+            pgy = grad_row ? g.gy : 0.0f;
+            pgz = grad_row ? gz : 0.0f;
+            pmask = mk;
         }
-        const float d = a.ref[v] - w;
-        bool mk = false;
-        if (MREF) mk = a.mref[v] != 0;
-        if (MMOV) mk = mk || tri(a.mmov, cx, cy, cz) > 0.0f;
-        if (s >= 2) {
-            const float vsum = d0 + d1 + d;
-            float* buf = s_v[s & 1];
-            buf[tid] = vsum;
-            __syncthreads();
-            const int yo = yy - 1;
-            float it = (buf[tl] + vsum + buf[tr]) * (1.0f / 9.0f);
-            if (pmask) it = 0.0f;
-            const float ie = owned ? it : 0.0f;
-            err += static_cast<double>(ie) * static_cast<double>(ie);
-            const float wyf = yo < cover_y ? static_cast<float>(1 + (yo == a.ny - 1 ? a.extra_y : 0)) : 0.0f;
-            const float m = pmask ? 0.0f : wxf * wyf;
-            const float mgx = m * pgx, mgy = m * pgy, mgz = m * pgz;
-            acc0 += mgx * pgx;
-            acc1 += mgx * pgy;
-            acc2 += mgx * pgz;
-            acc3 += mgy * pgy;
-            acc4 += mgy * pgz;
-            acc5 += mgz * pgz;
-            acc6 += mgx * it;
-            acc7 += mgy * it;
-            acc8 += mgz * it;
-        }
-        d0 = d1;
-        d1 = d;
-        pgx = gx;
-        pgy = gy;
-        pgz = gz;
-        pmask = mk;
     }
 
     // residual energy of the tile
     for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
-    if ((tid & 63) == 0) s_err[tid >> 6] = err;
+    if (lane == 0) s_err[wave] = err;
     // per-column partial sums -> per-patch sums
-    s_red[0][tid] = acc0;
-    s_red[1][tid] = acc1;
-    s_red[2][tid] = acc2;
-    s_red[3][tid] = acc3;
-    s_red[4][tid] = acc4;
-    s_red[5][tid] = acc5;
-    s_red[6][tid] = acc6;
-    s_red[7][tid] = acc7;
-    s_red[8][tid] = acc8;
+    if (owned) {
+        s_red[0][lcol] = acc0;
+        s_red[1][lcol] = acc1;
+        s_red[2][lcol] = acc2;
+        s_red[3][lcol] = acc3;
+        s_red[4][lcol] = acc4;
+        s_red[5][lcol] = acc5;
+        s_red[6][lcol] = acc6;
+        s_red[7][lcol] = acc7;
+        s_red[8][lcol] = acc8;
+    }
     __syncthreads();
     if (tid == 0) {
         double e = 0.0;
@@ -375,13 +419,15 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     }
     if (by < a.ncy) {
         const int ntask = a.np_x * 9;
+        const int lastc = min(a.cols, a.nx - x0);   // tile columns inside the volume
         for (int task = tid; task < ntask; task += blockDim.x) {
             const int pi = task / 9, k = task - 9 * (task / 9);
             const int gi = bx * a.np_x + pi;
             if (gi >= a.ncx) continue;
             double sacc = 0.0;
-            const int u0 = 1 + pi * a.p;
-            for (int u = 0; u < a.p; ++u) sacc += static_cast<double>(s_red[k][u0 + u]);
+            const int u0 = pi * a.p;
+            const int u1 = min(u0 + a.p, lastc);
+            for (int u = u0; u < u1; ++u) sacc += static_cast<double>(s_red[k][u]);
             a.sums[static_cast<size_t>(k) * a.ncp + (static_cast<size_t>(z) * a.ncy + by) * a.ncx + gi] =
                 static_cast<float>(sacc);
         }
@@ -432,7 +478,7 @@ void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int
     a.blocks = t.blocks;
     a.ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
     a.z_ratio = g.z_ratio;
-    const int threads = std::min(kMaxThreads, ((t.cols + 2 + 63) / 64) * 64);
+    const int threads = 64 * ((t.cols + kWaveCols - 1) / kWaveCols);
     const dim3 grid(static_cast<unsigned>(t.blocks));
     const bool mr = b.mref != nullptr, mm = b.mmov != nullptr;
     using Fn = void (*)(PassArgs);

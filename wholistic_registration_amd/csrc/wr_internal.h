@@ -119,6 +119,9 @@ struct LevelGeom {
     double mov_range;
 };
 
+// Widest LK tile: 4 waves, each owning 62 columns plus a halo lane per side.
+constexpr int kMaxTileCols = 248;
+
 struct LkTile {
     int np_x;            // patches per workgroup along x
     int nb_y;            // patch rows per workgroup
