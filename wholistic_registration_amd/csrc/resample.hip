@@ -14,8 +14,9 @@
 // x pass (the fast axis): a workgroup owns a group of output columns and up
 // to 64 rows; it stages the group's table slice and each row's span of
 // source columns in LDS, one row per lane.
-// y pass (a slow axis): one thread per output voxel along x; every tap is a
-// coalesced row read and the tap table is uniform per workgroup.
+// y pass (a slow axis): a thread owns one or two voxels along x of eight
+// output rows; every tap is a coalesced row read and the tap table is
+// uniform per workgroup.
 
 #include <algorithm>
 #include <cmath>
@@ -160,9 +161,11 @@ Table build_table(int n_in, int n_out, Kernel kern, std::vector<double>& w_arena
     return t;
 }
 
-#pragma clang fp contract(on)
-
 // ------------------------------------------------------------------ kernels
+//
+// Contraction stays off for the kernels as well: every tap is a rounded
+// float64 product followed by a rounded add (no FMA), as in the reference's
+// weights * values then sum.
 
 template <int IN_T>
 __device__ __forceinline__ double load_d(const void* p, size_t i) {
@@ -184,24 +187,26 @@ struct XPass {
     int n_in, n_out, k, xo_blk, n_cb;
     int sp;             // odd LDS row pitch (>= every group's span)
     int rt;             // rows per workgroup (<= 64, one per lane)
+    int paired;         // dense 2- or 4-byte input aligned to two elements
     long long nrows;
+    size_t n_total;     // elements of the input (even for paired staging)
     OutLayout ol;
 };
 
-// x pass: a workgroup stages the table slice of one column group and the
-// source span of `rt` rows in LDS; lane = row, wave = column (round robin),
-// so every table read is a wave-uniform broadcast and the per-row source
-// reads hit distinct banks (odd row pitch).  Each output is stored directly;
-// a row's columns are written by consecutive iterations and merge in L2.
+// Output columns of a column group handled by one of the 4 waves (xo_blk <= 64).
+constexpr int kXColsPerWave = 16;
+
+// x pass: a workgroup stages the source span of `rt` rows of one column
+// group in LDS; lane = row, wave = column (round robin), so the table row of
+// a column is read with wave-uniform scalar loads and the per-row source
+// reads hit distinct banks (odd row pitch).  The results are transposed
+// through LDS and stored with lanes along the row.
 template <int IN_T>
 __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in, int in_stride, int in_comp,
                                                      void* __restrict__ out, int om, double divisor, XPass p) {
     using T = typename std::conditional<IN_T == E_F64, double, float>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int tabn = p.xo_blk * p.k;
-    double* sw = reinterpret_cast<double*>(smem_raw);                 // [c][t]
-    int* si = reinterpret_cast<int*>(sw + tabn);                        // [c][t]
-    T* sin = reinterpret_cast<T*>(smem_raw + ((static_cast<size_t>(tabn) * 12 + 15) & ~size_t(15)));
+    T* sin = reinterpret_cast<T*>(smem_raw);
     const int cb = static_cast<int>(blockIdx.x % p.n_cb);
     const long long r0 = static_cast<long long>(blockIdx.x / p.n_cb) * p.rt;
     const int nr = static_cast<int>(min(static_cast<long long>(p.rt), p.nrows - r0));
@@ -210,22 +215,55 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
     const int lo = p.span[2 * cb];
     const int span = p.span[2 * cb + 1] - lo + 1;
     const size_t tab0 = static_cast<size_t>(xo0) * p.k;
-    for (int i = threadIdx.x; i < nxo * p.k; i += 256) {
-        sw[i] = p.w[tab0 + i];
-        si[i] = p.idx[tab0 + i] - lo;
-    }
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
     // wave w stages rows w, w+4, ..., lanes along x; the (up to) 16 rows of a
     // lane are loaded before any is stored, so the loads overlap
-    for (int xx = lane; xx < span; xx += 64) {
+    // All 16 loads of a lane are issued unconditionally (row index clamped to
+    // the block's last row, so every address is inside the input) before any
+    // result is used; rows beyond the block are dropped at the LDS store.
+    if constexpr (IN_T != E_F64) {
+        if (p.paired) {
+            // dense rows of 2- or 4-byte elements (even element count): each
+            // lane loads two consecutive elements from an even element index
+            // (4- or 8-byte aligned)
+            using In = typename std::conditional<IN_T == E_U16, uint16_t, float>::type;
+            typedef In In2 __attribute__((ext_vector_type(2)));
+            const In* src = static_cast<const In*>(in);
+            const size_t last_pair = p.n_total - 2;
+            for (int d = lane; d < ((span + 2) >> 1); d += 64) {
+                In2 v[16];
+                int off[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int rr = min(wave + 4 * j, nr - 1);
+                    const size_t e0 = static_cast<size_t>(r0 + rr) * p.n_in + lo;
+                    const size_t a0 = e0 & ~static_cast<size_t>(1);
+                    off[j] = static_cast<int>(e0 - a0);
+                    // a pair that holds a needed element lies at or before the
+                    // last pair; one that does not is clamped and dropped
+                    const size_t e = min(a0 + 2 * static_cast<size_t>(d), last_pair);
+                    v[j] = *reinterpret_cast<const In2*>(src + e);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int rr = wave + 4 * j;
+                    if (rr < nr) {
+                        const int x0 = 2 * d - off[j];
+                        if (x0 >= 0 && x0 < span) sin[rr * p.sp + x0] = static_cast<T>(v[j].x);
+                        if (x0 + 1 < span) sin[rr * p.sp + x0 + 1] = static_cast<T>(v[j].y);
+                    }
+                }
+            }
+        }
+    }
+    for (int xx = lane; xx < (p.paired ? 0 : span); xx += 64) {
         T vals[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const int rr = wave + 4 * j;
-            vals[j] = rr < nr ? static_cast<T>(load_d<IN_T>(
-                                    in, ((static_cast<size_t>(r0 + rr) * p.n_in + lo + xx) * in_stride) + in_comp))
-                              : T(0);
+            const int rr = min(wave + 4 * j, nr - 1);
+            vals[j] = static_cast<T>(
+                load_d<IN_T>(in, ((static_cast<size_t>(r0 + rr) * p.n_in + lo + xx) * in_stride) + in_comp));
         }
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
@@ -234,49 +272,142 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
         }
     }
     __syncthreads();
+    // the outputs of a lane (its row, columns wave, wave + 4, ...) stay in
+    // registers until every wave has finished reading the staged rows
+    double res[kXColsPerWave];
     if (lane < nr) {
         const T* row = sin + lane * p.sp;
-        const size_t obase = static_cast<size_t>(r0 + lane) * p.ol.pitch + p.ol.pad + xo0;
-        for (int c = wave; c < nxo; c += 4) {
-            const double* wc = sw + c * p.k;
-            const int* ic = si + c * p.k;
-            double acc = __dmul_rn(wc[0], static_cast<double>(row[ic[0]]));
-            for (int t = 1; t < p.k; ++t) acc = __dadd_rn(acc, __dmul_rn(wc[t], static_cast<double>(row[ic[t]])));
-            store_out(out, obase + c, acc, om, divisor);
+#pragma unroll
+        for (int q = 0; q < kXColsPerWave; ++q) {
+            const int c = wave + 4 * q;
+            if (c < nxo) {
+                // wave-uniform table row: scalar loads, independent LDS reads
+                const double* wc = p.w + tab0 + static_cast<size_t>(c) * p.k;
+                const int* ic = p.idx + tab0 + static_cast<size_t>(c) * p.k;
+                const T* rl = row - lo;
+                double acc = wc[0] * static_cast<double>(rl[ic[0]]);
+#pragma unroll 8
+                for (int t = 1; t < p.k; ++t)
+                    acc = acc + wc[t] * static_cast<double>(rl[ic[t]]);
+                res[q] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    // transpose through LDS (odd pitch po, in doubles, over the staged rows)
+    // so that the stores run along the rows
+    double* so = reinterpret_cast<double*>(sin);
+    const int po = p.xo_blk | 1;
+    if (lane < nr) {
+#pragma unroll
+        for (int q = 0; q < kXColsPerWave; ++q) {
+            const int c = wave + 4 * q;
+            if (c < nxo) so[lane * po + c] = res[q];
+        }
+    }
+    __syncthreads();
+    const int lg = __builtin_ctz(p.xo_blk);
+    for (int i = threadIdx.x; i < (nr << lg); i += 256) {
+        const int rr = i >> lg, c = i & (p.xo_blk - 1);
+        if (c < nxo)
+            store_out(out, static_cast<size_t>(r0 + rr) * p.ol.pitch + p.ol.pad + xo0 + c, so[rr * po + c], om,
+                      divisor);
+    }
+}
+
+// V consecutive elements of a dense row (V > 1: aligned to V elements).
+template <int IN_T, int V>
+__device__ __forceinline__ void load_v(const void* p, size_t i, int stride, int comp, double (&v)[V]) {
+    if constexpr (V == 1) {
+        v[0] = load_d<IN_T>(p, i * stride + comp);
+    } else {
+        using E = typename std::conditional<IN_T == E_U16, uint16_t,
+                                            typename std::conditional<IN_T == E_F32, float, double>::type>::type;
+        typedef E EV __attribute__((ext_vector_type(V)));
+        const EV e = *reinterpret_cast<const EV*>(static_cast<const E*>(p) + i);
+#pragma unroll
+        for (int q = 0; q < V; ++q) v[q] = static_cast<double>(e[q]);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void store_v(void* out, size_t i, const double (&v)[V], int om, double divisor) {
+    if constexpr (V == 1) {
+        store_out(out, i, v[0], om, divisor);
+    } else {
+        if (om == OUT_F64) {
+            typedef double DV __attribute__((ext_vector_type(V)));
+            DV o;
+#pragma unroll
+            for (int q = 0; q < V; ++q) o[q] = v[q];
+            *reinterpret_cast<DV*>(static_cast<double*>(out) + i) = o;
+        } else if (om == OUT_F32) {
+            typedef float FV __attribute__((ext_vector_type(V)));
+            FV o;
+#pragma unroll
+            for (int q = 0; q < V; ++q) o[q] = static_cast<float>(v[q] / divisor);
+            *reinterpret_cast<FV*>(static_cast<float*>(out) + i) = o;
+        } else {
+            typedef uint8_t BV __attribute__((ext_vector_type(V)));
+            BV o;
+#pragma unroll
+            for (int q = 0; q < V; ++q) o[q] = v[q] > 0.0 ? 1 : 0;
+            *reinterpret_cast<BV*>(static_cast<uint8_t*>(out) + i) = o;
         }
     }
 }
 
-template <int IN_T>
+// Output rows of one y-pass workgroup.
+constexpr int kYRows = 8;
+
+// y pass: a thread owns V consecutive x of kYRows consecutive output rows of
+// one z slice; the tap table of a row is wave-uniform.
+template <int IN_T, int V>
 __global__ __launch_bounds__(256) void k_resample_y(const void* __restrict__ in, int in_stride,
                                                      int in_comp, void* __restrict__ out, int om,
                                                      double divisor, const double* __restrict__ w,
                                                      const int* __restrict__ idx, int n_in,
-                                                     int n_out, int k, int nx, int xblocks, OutLayout ol) {
-    const long long b = blockIdx.x;
-    const int x = static_cast<int>(b % xblocks) * 256 + threadIdx.x;
-    const long long zy = b / xblocks;          // z * n_out + yo
+                                                     int n_out, int k, int nx, int xblocks, int yblocks,
+                                                     OutLayout ol) {
+    const unsigned b = blockIdx.x;
+    const int xb = static_cast<int>(b % static_cast<unsigned>(xblocks));
+    const unsigned rest = b / static_cast<unsigned>(xblocks);
+    const int yb = static_cast<int>(rest % static_cast<unsigned>(yblocks));
+    const int z = static_cast<int>(rest / static_cast<unsigned>(yblocks));
+    const int x = (xb * 256 + static_cast<int>(threadIdx.x)) * V;
     if (x >= nx) return;
-    const int yo = static_cast<int>(zy % n_out);
-    const long long z = zy / n_out;
-    const double* wr = w + static_cast<size_t>(yo) * k;
-    const int* ir = idx + static_cast<size_t>(yo) * k;
     const size_t base = static_cast<size_t>(z) * n_in;
-    double acc = __dmul_rn(wr[0], load_d<IN_T>(in, ((base + ir[0]) * nx + x) * in_stride + in_comp));
-#pragma unroll 8
-    for (int t = 1; t < k; ++t)
-        acc = __dadd_rn(acc, __dmul_rn(wr[t], load_d<IN_T>(in, ((base + ir[t]) * nx + x) * in_stride + in_comp)));
-    store_out(out, static_cast<size_t>(zy) * ol.pitch + ol.pad + x, acc, om, divisor);
+    const int y0 = yb * kYRows;
+    const int y1 = min(y0 + kYRows, n_out);
+    for (int yo = y0; yo < y1; ++yo) {
+        const double* wr = w + static_cast<size_t>(yo) * k;
+        const int* ir = idx + static_cast<size_t>(yo) * k;
+        double acc[V], v[V];
+        load_v<IN_T, V>(in, (base + ir[0]) * nx + x, in_stride, in_comp, v);
+#pragma unroll
+        for (int q = 0; q < V; ++q) acc[q] = wr[0] * v[q];
+#pragma unroll 4
+        for (int t = 1; t < k; ++t) {
+            load_v<IN_T, V>(in, (base + ir[t]) * nx + x, in_stride, in_comp, v);
+#pragma unroll
+            for (int q = 0; q < V; ++q) acc[q] = acc[q] + wr[t] * v[q];
+        }
+        store_v<V>(out, (static_cast<size_t>(z) * n_out + yo) * ol.pitch + ol.pad + x, acc, om, divisor);
+    }
 }
 
-template <int IN_T>
+// Elementwise conversion: a thread owns V consecutive x of one row.
+template <int IN_T, int V>
 __global__ __launch_bounds__(256) void k_convert(const void* __restrict__ in, int in_stride,
                                                   int in_comp, void* __restrict__ out, int om,
-                                                  double divisor, int nx, size_t n, OutLayout ol) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n) return;
-    const size_t row = i / nx, x = i - row * nx;
-    store_out(out, row * ol.pitch + ol.pad + x, load_d<IN_T>(in, i * in_stride + in_comp), om, divisor);
+                                                  double divisor, int nx, int xblocks, OutLayout ol) {
+    const unsigned b = blockIdx.x;
+    const unsigned row = b / static_cast<unsigned>(xblocks);
+    const int x = (static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x)) * V;
+    if (x >= nx) return;
+    double v[V];
+    load_v<IN_T, V>(in, static_cast<size_t>(row) * nx + x, in_stride, in_comp, v);
+    store_v<V>(out, static_cast<size_t>(row) * ol.pitch + ol.pad + x, v, om, divisor);
 }
 
 __global__ __launch_bounds__(256) void k_pad_rows(float* __restrict__ buf, long long nrows, int nx, int pitch) {
@@ -306,11 +437,18 @@ void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, vo
     p.sp = t.span_max | 1;
     p.nrows = static_cast<long long>(ny) * nz;
     p.ol = ol;
+    p.n_total = static_cast<size_t>(p.nrows) * t.n_in;
+    {
+        const size_t in_esz = in_t == E_U16 ? 2 : (in_t == E_F32 ? 4 : 8);
+        p.paired = in_t != E_F64 && in_stride == 1 && in_comp == 0 && p.n_total % 2 == 0 &&
+                   reinterpret_cast<uintptr_t>(in) % (2 * in_esz) == 0;
+    }
     const size_t esz = in_t == E_F64 ? sizeof(double) : sizeof(float);
-    const size_t tab = (static_cast<size_t>(t.xo_blk) * t.k * 12 + 15) & ~size_t(15);
-    // at most 64 KiB of LDS: the table slice plus one source span per row
-    p.rt = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, (65536 - tab) / (esz * p.sp))));
-    const size_t lds = tab + esz * static_cast<size_t>(p.sp) * p.rt;
+    // at most 64 KiB of LDS: per row, the larger of its source span and its
+    // transposed output columns
+    const size_t per_row = std::max<size_t>(esz * p.sp, sizeof(double) * (t.xo_blk | 1));
+    p.rt = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, 65536 / per_row)));
+    const size_t lds = per_row * p.rt;
     const long long groups = (p.nrows + p.rt - 1) / p.rt;
     const unsigned blocks = static_cast<unsigned>(groups * t.n_cb);
     switch (in_t) {
@@ -325,42 +463,77 @@ void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, vo
     }
 }
 
+namespace {
+
+size_t elem_bytes(Elem e) { return e == E_U16 ? 2 : (e == E_F32 ? 4 : 8); }
+size_t out_bytes(OutMode om) { return om == OUT_F64 ? 8 : (om == OUT_F32 ? 4 : 1); }
+
+// V-element vector access is possible: dense input, rows and output rows that
+// start on multiples of V elements, and V-aligned base pointers.
+bool vec_ok(int v, const void* in, Elem in_t, int in_stride, int in_comp, const void* out, OutMode om, int nx,
+            OutLayout ol) {
+    return in_stride == 1 && in_comp == 0 && nx % v == 0 && ol.pitch % v == 0 && ol.pad % v == 0 &&
+           reinterpret_cast<uintptr_t>(in) % (v * elem_bytes(in_t)) == 0 &&
+           reinterpret_cast<uintptr_t>(out) % (v * out_bytes(om)) == 0;
+}
+
+template <int V>
+void launch_y(const void* in, Elem in_t, int in_stride, int in_comp, void* out, OutMode om, double divisor,
+              const Table& t, const double* wp, const int* ip, int nx, int nz, OutLayout ol, hipStream_t s) {
+    const int xblocks = (nx / V + 255) / 256;
+    const int yblocks = (t.n_out + kYRows - 1) / kYRows;
+    const unsigned blocks = static_cast<unsigned>(static_cast<long long>(xblocks) * yblocks * nz);
+    switch (in_t) {
+        case E_F32:
+            k_resample_y<E_F32, V><<<blocks, 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, wp, ip, t.n_in,
+                                                          t.n_out, t.k, nx, xblocks, yblocks, ol);
+            break;
+        case E_U16:
+            k_resample_y<E_U16, V><<<blocks, 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, wp, ip, t.n_in,
+                                                          t.n_out, t.k, nx, xblocks, yblocks, ol);
+            break;
+        default:
+            k_resample_y<E_F64, V><<<blocks, 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, wp, ip, t.n_in,
+                                                          t.n_out, t.k, nx, xblocks, yblocks, ol);
+    }
+}
+
+template <int V>
+void launch_conv(const void* in, Elem in_t, int in_stride, int in_comp, void* out, OutMode om, double divisor,
+                 int nx, long long nrows, OutLayout ol, hipStream_t s) {
+    const int xblocks = (nx / V + 255) / 256;
+    const unsigned blocks = static_cast<unsigned>(xblocks * nrows);
+    switch (in_t) {
+        case E_F32:
+            k_convert<E_F32, V><<<blocks, 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, nx, xblocks, ol);
+            break;
+        case E_U16:
+            k_convert<E_U16, V><<<blocks, 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, nx, xblocks, ol);
+            break;
+        default:
+            k_convert<E_F64, V><<<blocks, 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, nx, xblocks, ol);
+    }
+}
+
+}  // namespace
+
 void launch_resample_y(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                        OutMode om, double divisor, const Table& t, const double* w,
                        const int* idx, int nx, int nz, OutLayout ol, hipStream_t s) {
-    const int xblocks = (nx + 255) / 256;
-    const long long blocks = static_cast<long long>(xblocks) * t.n_out * nz;
     const double* wp = w + t.w_off;
     const int* ip = idx + t.i_off;
-    switch (in_t) {
-        case E_F32:
-            k_resample_y<E_F32><<<dim3(blocks), 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor,
-                                                            wp, ip, t.n_in, t.n_out, t.k, nx, xblocks, ol);
-            break;
-        case E_U16:
-            k_resample_y<E_U16><<<dim3(blocks), 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor,
-                                                            wp, ip, t.n_in, t.n_out, t.k, nx, xblocks, ol);
-            break;
-        default:
-            k_resample_y<E_F64><<<dim3(blocks), 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor,
-                                                            wp, ip, t.n_in, t.n_out, t.k, nx, xblocks, ol);
-    }
+    if (vec_ok(2, in, in_t, in_stride, in_comp, out, om, nx, ol))
+        launch_y<2>(in, in_t, in_stride, in_comp, out, om, divisor, t, wp, ip, nx, nz, ol, s);
+    else
+        launch_y<1>(in, in_t, in_stride, in_comp, out, om, divisor, t, wp, ip, nx, nz, ol, s);
 }
 
 void launch_convert(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                     OutMode om, double divisor, int nx, long long nrows, OutLayout ol, hipStream_t s) {
-    const size_t n = static_cast<size_t>(nx) * nrows;
-    const long long blocks = static_cast<long long>((n + 255) / 256);
-    switch (in_t) {
-        case E_F32:
-            k_convert<E_F32><<<dim3(blocks), 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, nx, n, ol);
-            break;
-        case E_U16:
-            k_convert<E_U16><<<dim3(blocks), 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, nx, n, ol);
-            break;
-        default:
-            k_convert<E_F64><<<dim3(blocks), 256, 0, s>>>(in, in_stride, in_comp, out, om, divisor, nx, n, ol);
-    }
+    if (vec_ok(4, in, in_t, in_stride, in_comp, out, om, nx, ol))
+        launch_conv<4>(in, in_t, in_stride, in_comp, out, om, divisor, nx, nrows, ol, s);
+    else
+        launch_conv<1>(in, in_t, in_stride, in_comp, out, om, divisor, nx, nrows, ol, s);
 }
 
 void launch_pad_rows(float* buf, long long nrows, int nx, int pitch, hipStream_t s) {
