@@ -89,6 +89,18 @@ def smooth_penalty(tmpl, mask):
     return float((mx + my) / 2)
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the level-0 LK pass from the committed counter
+    runs (FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size), or None
+    when the workload has no such record."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
 def algorithmic_bytes(w, records, s_in, s_p=4, s_m=1):
     """Bytes per registration: LK passes, pyramid, level-0 conversion, final
     warp with motion write."""
@@ -265,10 +277,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_lk_pass (level 0)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,
-                     "bytes_per_launch": bytes_per_launch},
+                     "bytes_per_launch": bytes_per_launch,
+                     "bound_note": "the pass is VALU-issue bound (about 250 FP32 VALU instructions "
+                                   "per wave-row); HBM is the roofline it is priced against"},
         "pipeline_gbs": alg / (total / args.steps) / 1e9,
         "algorithmic_bytes_per_step": alg,
     }
+    traffic = measured_traffic(args.workload)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     out["epe_vs_truth"] = truth_epe(motion, field, mask)
     if not (args.no_cpu_baseline and args.no_epe_check):
         crop = tuple(int(v) for v in args.cpu_crop.split(","))
