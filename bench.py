@@ -77,16 +77,11 @@ def make_pair_device(w, dev):
 
 
 def smooth_penalty(tmpl, mask):
-    """getSmPnltNormFctr (preprocess.py:42-66) on the float32 template, in
-    float64 (the harness converts uint16 to float32 first)."""
-    import torch
-    ref = tmpl.float()
-    m = mask > 0 if mask is not None else torch.zeros_like(ref, dtype=torch.bool)
-    gx = ((ref[..., 2:] - ref[..., :-2]) / 2).double()
-    gy = ((ref[:, 2:, :] - ref[:, :-2, :]) / 2).double()
-    mx = (gx ** 2)[~m[..., 1:-1]].mean()
-    my = (gy ** 2)[~m[:, 1:-1, :]].mean()
-    return float((mx + my) / 2)
+    """getSmPnltNormFctr (preprocess.py:42-66) of the template on the device
+    (wr_smooth_penalty_factor), after the harness's uint16 -> float32
+    conversion."""
+    from wholistic_registration_amd import device as D
+    return D.smooth_penalty_factor(tmpl.float().contiguous(), mask)
 
 
 def measured_traffic(workload):

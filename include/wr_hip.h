@@ -114,6 +114,19 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype,
 int wr_correct_motion(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz,
                       const float* motion, float* out, void* stream);
 
+/* Replaces getSmPnltNormFctr(dat_ref, option)
+ * (src/wholistic_registration/utils/preprocess.py:42-66), called by the
+ * harness before getMotion (registration.py:295):
+ *   factor = (mean(((ref[x+1]-ref[x-1])/2)^2) + mean(((ref[y+1]-ref[y-1])/2)^2)) / 2
+ * over the voxels whose mask_ref is 0 (interior x, resp. y, only), in the
+ * input dtype's arithmetic (uint16 differences wrap modulo 2^16).
+ *   ref       (Z,Y,X) volume, dtype WR_F32, WR_U16 or WR_F64
+ *   mask_ref  (Z,Y,X) float32, nonzero = excluded, or NULL
+ *   factor    host double; NaN when a direction has no unmasked voxel.
+ * Synchronises the stream. */
+int wr_smooth_penalty_factor(wr_ctx* ctx, const void* ref, int dtype, int nx, int ny, int nz,
+                             const float* mask_ref, double* factor, void* stream);
+
 /* Pyramid level image as the reference builds it: imresize(vol,
  * output_shape=(int(nx/2^level), int(ny/2^level), nz)) (bicubic,
  * antialiased, imresize.py:148-191; calFlow3d_Wei_v1.py:341-351), stored as

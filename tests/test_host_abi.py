@@ -92,3 +92,10 @@ def test_product_path_has_no_cpu_fallback():
             assert "from oracle" not in src and "import oracle" not in src, fn
             if fn not in ("synth.py",):
                 assert "map_coordinates" not in src, fn
+
+
+def test_smooth_penalty_null_arguments_are_rejected():
+    lib = _lib.load()
+    out = ctypes.c_double()
+    rc = lib.wr_smooth_penalty_factor(None, None, 0, 4, 4, 4, None, ctypes.byref(out), None)
+    assert rc == _lib.WR_E_ARG

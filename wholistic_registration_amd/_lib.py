@@ -73,6 +73,7 @@ _SIGNATURES = {
     "wr_pyramid_level": ([_VP, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_imresize": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_get_stats": ([_VP, ctypes.POINTER(WrStats)], _I),
+    "wr_smooth_penalty_factor": ([_VP, _VP, _I, _I, _I, _I, _VP, ctypes.POINTER(ctypes.c_double), _VP], _I),
 }
 
 _lib = None

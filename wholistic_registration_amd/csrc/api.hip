@@ -167,7 +167,7 @@ void wr_destroy(wr_ctx* ctx) {
     Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
-                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i})
+                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     delete ctx;
@@ -222,6 +222,29 @@ int wr_pyramid_level(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int
     resample2(in, elem_of(dtype), 1, 0, out, OUT_F32, 1.0, pl, c.tab_w.as<double>(), c.tab_i.as<int>(), nz,
               c.tmp.as<double>(), dense(lx), s);
     WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_smooth_penalty_factor(wr_ctx* ctx, const void* ref, int dtype, int nx, int ny, int nz,
+                             const float* mask_ref, double* factor, void* stream) {
+    WR_ARG(ctx && ref && factor, "wr_smooth_penalty_factor: null argument");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_smooth_penalty_factor: empty volume");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16 || dtype == WR_F64,
+           "wr_smooth_penalty_factor: dtype must be WR_F32, WR_U16 or WR_F64");
+    WR_ARG(static_cast<long long>(ny) * nz < (1LL << 24), "wr_smooth_penalty_factor: more than 2^24 rows");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t rows = static_cast<size_t>(ny) * nz;
+    WR_TRY(grow(c.pen, (4 * rows + 3) * sizeof(double)));
+    double* part = c.pen.as<double>();
+    double* out = part + 4 * rows;
+    launch_penalty(ref, elem_of(dtype), mask_ref, nx, ny, nz, part, out, s);
+    WR_HIP_CHECK(hipGetLastError());
+    double h[3];
+    WR_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipStreamSynchronize(s));
+    *factor = h[2];
     return WR_OK;
 }
 

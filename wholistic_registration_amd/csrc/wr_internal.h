@@ -171,13 +171,20 @@ void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int ite
 void launch_warp(const void* in, Elem in_t, int nx, int ny, int nz, const float* motion,
                  float* out, hipStream_t s);
 
+// ---------------------------------------------------------------- penalty
+
+// getSmPnltNormFctr on a (nz, ny, nx) volume: `part` holds 4 doubles per row
+// (ny * nz rows), `out` receives {mean_x, mean_y, factor} on the device.
+void launch_penalty(const void* ref, Elem t, const float* mask, int nx, int ny, int nz, double* part,
+                    double* out, hipStream_t s);
+
 // ---------------------------------------------------------------- context
 
 struct Context {
     int device = 0;
     DevBuf mov_l, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;
     DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts, mdpart;
-    DevBuf tab_w, tab_i;
+    DevBuf tab_w, tab_i, pen;
     std::vector<hipEvent_t> events;
     wr_stats stats{};
 };

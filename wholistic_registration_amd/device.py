@@ -206,3 +206,27 @@ def imresize_xy(vol, nx_out, ny_out, method="bicubic"):
                          0 if method == "bicubic" else 1, _ptr(out), _stream())
     _lib.check(rc, "wr_imresize")
     return out
+
+
+def smooth_penalty_factor(ref, mask_ref=None):
+    """``getSmPnltNormFctr`` (preprocess.py:42-66) of a (Z, Y, X) device
+    volume: half the sum of the mean squared central differences along x and
+    y over the voxels whose ``mask_ref`` is zero, in the arithmetic of the
+    volume's dtype (uint16 differences wrap).  ``mask_ref``: (Z, Y, X) float32
+    or bool device tensor (nonzero = excluded) or None."""
+    torch = _torch()
+    _check_vol(ref, "ref")
+    if mask_ref is not None:
+        _check_vol(mask_ref, "mask_ref")
+        if tuple(mask_ref.shape) != tuple(ref.shape):
+            raise ValueError("mask_ref must match the volume's shape")
+        if mask_ref.dtype != torch.float32:
+            mask_ref = mask_ref.to(torch.float32)
+    nz, ny, nx = ref.shape
+    out = ctypes.c_double()
+    lib = _lib.load()
+    ctx = _lib.context(ref.device.index or 0)
+    rc = lib.wr_smooth_penalty_factor(ctx, _ptr(ref), dtype_code(ref), nx, ny, nz, _ptr(mask_ref),
+                                      ctypes.byref(out), _stream())
+    _lib.check(rc, "wr_smooth_penalty_factor")
+    return float(out.value)
