@@ -188,7 +188,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-epe-check", action="store_true",
                     help="skip the float64-oracle endpoint-error check on the crop")
-    ap.add_argument("--cpu-crop", default="4,512,1024", help="z,y,x of the CPU-baseline crop")
+    ap.add_argument("--cpu-crop", default="8,512,1024", help="z,y,x of the CPU-baseline crop")
     ap.add_argument("--tile-patches-x", type=int, default=0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration)")
