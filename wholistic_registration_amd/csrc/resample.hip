@@ -123,9 +123,10 @@ Table build_table(int n_in, int n_out, Kernel kern, std::vector<double>& w_arena
             w_arena.push_back(w[static_cast<size_t>(j) * ntap + k]);
             i_arena.push_back(ind[static_cast<size_t>(j) * ntap + k]);
         }
-    // x-pass column groups: the largest power of two <= 64 whose source span
-    // fits kXSpan inputs (and at most 2048 table entries); the span of each
-    // group
+    // x-pass column groups: the largest power of two <= kXMaxCols whose
+    // source span fits kXSpan inputs (and at most 2048 table entries); the
+    // span of each group, then whether every output of the group takes its
+    // taps from consecutive inputs (no mirrored index)
     auto span_of = [&](int xo_try) {
         int worst = 0;
         for (int g = 0; g * xo_try < n_out; ++g) {
@@ -141,7 +142,7 @@ Table build_table(int n_in, int n_out, Kernel kern, std::vector<double>& w_arena
         return worst;
     };
     int xo = 1;
-    while (xo * 2 <= 64 && xo * 2 * t.k <= 2048 && span_of(xo * 2) <= kXSpan) xo *= 2;
+    while (xo * 2 <= kXMaxCols && xo * 2 * t.k <= 2048 && span_of(xo * 2) <= kXSpan) xo *= 2;
     t.xo_blk = xo;
     t.n_cb = (n_out + xo - 1) / xo;
     t.s_off = i_arena.size();
@@ -157,6 +158,15 @@ Table build_table(int n_in, int n_out, Kernel kern, std::vector<double>& w_arena
         i_arena.push_back(lo);
         i_arena.push_back(hi);
         t.span_max = std::max(t.span_max, hi - lo + 1);
+    }
+    t.a_off = i_arena.size();
+    for (int g = 0; g < t.n_cb; ++g) {
+        bool aff = true;
+        for (int j = g * xo; j < std::min(n_out, (g + 1) * xo) && aff; ++j) {
+            const int* row = &i_arena[t.i_off + static_cast<size_t>(j) * t.k];
+            for (int k = 1; k < t.k; ++k) aff = aff && row[k] == row[0] + k;
+        }
+        i_arena.push_back(aff ? 1 : 0);
     }
     return t;
 }
@@ -184,8 +194,9 @@ struct XPass {
     const double* w;
     const int* idx;
     const int* span;
+    const int* aff;
     int n_in, n_out, k, xo_blk, n_cb;
-    int sp;             // odd LDS row pitch (>= every group's span)
+    int sp;             // LDS row pitch in staged elements (see launch_resample_x)
     int rt;             // rows per workgroup (<= 64, one per lane)
     int paired;         // dense 2- or 4-byte input aligned to two elements
     long long nrows;
@@ -193,20 +204,30 @@ struct XPass {
     OutLayout ol;
 };
 
-// Output columns of a column group handled by one of the 4 waves (xo_blk <= 64).
-constexpr int kXColsPerWave = 16;
+// Output columns of a column group handled by one of the 4 waves.
+constexpr int kXColsPerWave = kXMaxCols / 4;
 
-// x pass: a workgroup stages the source span of `rt` rows of one column
-// group in LDS; lane = row, wave = column (round robin), so the table row of
-// a column is read with wave-uniform scalar loads and the per-row source
-// reads hit distinct banks (odd row pitch).  The results are transposed
-// through LDS and stored with lanes along the row.
+// x pass: a workgroup owns `rt` rows (one per lane) and one column group.  It
+// stages the group's source span of its rows in LDS in the input's own type
+// (uint16, float32 or float64), then computes with lane = row and wave =
+// column (round robin): the table row of a column is wave-uniform (scalar
+// loads), the per-row reads hit distinct banks (odd row pitch in dwords),
+// and a group whose outputs read consecutive inputs uses immediate LDS
+// offsets.  The results are transposed through LDS and stored along the rows.
+//
+// Paired staging (dense 2- or 4-byte input): each lane loads two elements
+// from an even element index, so a row's span starts at LDS element o (0 or
+// 1, the parity of its first element) and the pair d lands at elements 2d,
+// 2d + 1.  The loads carry no branch (row index clamped to the block's last
+// row, pair index to the input's last pair), so all of them are in flight
+// together.
 template <int IN_T>
 __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in, int in_stride, int in_comp,
                                                      void* __restrict__ out, int om, double divisor, XPass p) {
-    using T = typename std::conditional<IN_T == E_F64, double, float>::type;
+    using S = typename std::conditional<IN_T == E_U16, uint16_t,
+                                        typename std::conditional<IN_T == E_F32, float, double>::type>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* sin = reinterpret_cast<T*>(smem_raw);
+    S* sin = reinterpret_cast<S*>(smem_raw);
     const int cb = static_cast<int>(blockIdx.x % p.n_cb);
     const long long r0 = static_cast<long long>(blockIdx.x / p.n_cb) * p.rt;
     const int nr = static_cast<int>(min(static_cast<long long>(p.rt), p.nrows - r0));
@@ -217,58 +238,57 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
     const size_t tab0 = static_cast<size_t>(xo0) * p.k;
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
-    // wave w stages rows w, w+4, ..., lanes along x; the (up to) 16 rows of a
-    // lane are loaded before any is stored, so the loads overlap
-    // All 16 loads of a lane are issued unconditionally (row index clamped to
-    // the block's last row, so every address is inside the input) before any
-    // result is used; rows beyond the block are dropped at the LDS store.
+    const bool paired = IN_T != E_F64 && p.paired;
+
     if constexpr (IN_T != E_F64) {
-        if (p.paired) {
-            // dense rows of 2- or 4-byte elements (even element count): each
-            // lane loads two consecutive elements from an even element index
-            // (4- or 8-byte aligned)
-            using In = typename std::conditional<IN_T == E_U16, uint16_t, float>::type;
-            typedef In In2 __attribute__((ext_vector_type(2)));
-            const In* src = static_cast<const In*>(in);
+        if (paired) {
+            typedef S S2 __attribute__((ext_vector_type(2)));
+            const S* src = static_cast<const S*>(in);
             const size_t last_pair = p.n_total - 2;
-            for (int d = lane; d < ((span + 2) >> 1); d += 64) {
-                In2 v[16];
-                int off[16];
+            const int npair = (span + 2) >> 1;
+            for (int q = 0; 64 * q < npair; ++q) {
+                const int d = lane + 64 * q;
+                S2 v[16];
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     const int rr = min(wave + 4 * j, nr - 1);
                     const size_t e0 = static_cast<size_t>(r0 + rr) * p.n_in + lo;
-                    const size_t a0 = e0 & ~static_cast<size_t>(1);
-                    off[j] = static_cast<int>(e0 - a0);
-                    // a pair that holds a needed element lies at or before the
-                    // last pair; one that does not is clamped and dropped
-                    const size_t e = min(a0 + 2 * static_cast<size_t>(d), last_pair);
-                    v[j] = *reinterpret_cast<const In2*>(src + e);
+                    const size_t e = min((e0 & ~static_cast<size_t>(1)) + 2 * static_cast<size_t>(d), last_pair);
+                    v[j] = *reinterpret_cast<const S2*>(src + e);
                 }
+                if (d < npair) {
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    const int rr = wave + 4 * j;
-                    if (rr < nr) {
-                        const int x0 = 2 * d - off[j];
-                        if (x0 >= 0 && x0 < span) sin[rr * p.sp + x0] = static_cast<T>(v[j].x);
-                        if (x0 + 1 < span) sin[rr * p.sp + x0 + 1] = static_cast<T>(v[j].y);
+                    for (int j = 0; j < 16; ++j) {
+                        const int rr = wave + 4 * j;
+                        if (rr < nr) {
+                            S* dst = sin + rr * p.sp + 2 * d;
+                            if constexpr (IN_T == E_U16) {
+                                *reinterpret_cast<S2*>(dst) = v[j];   // 4-byte aligned: sp is even
+                            } else {
+                                dst[0] = v[j].x;
+                                dst[1] = v[j].y;
+                            }
+                        }
                     }
                 }
             }
         }
     }
-    for (int xx = lane; xx < (p.paired ? 0 : span); xx += 64) {
-        T vals[16];
+    if (!paired) {
+        // strided or float64 input: element loads, 16 rows of a lane in flight
+        for (int xx = lane; xx < span; xx += 64) {
+            S vals[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int rr = min(wave + 4 * j, nr - 1);
-            vals[j] = static_cast<T>(
-                load_d<IN_T>(in, ((static_cast<size_t>(r0 + rr) * p.n_in + lo + xx) * in_stride) + in_comp));
-        }
+            for (int j = 0; j < 16; ++j) {
+                const int rr = min(wave + 4 * j, nr - 1);
+                vals[j] = static_cast<S>(
+                    load_d<IN_T>(in, ((static_cast<size_t>(r0 + rr) * p.n_in + lo + xx) * in_stride) + in_comp));
+            }
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int rr = wave + 4 * j;
-            if (rr < nr) sin[rr * p.sp + xx] = vals[j];
+            for (int j = 0; j < 16; ++j) {
+                const int rr = wave + 4 * j;
+                if (rr < nr) sin[rr * p.sp + xx] = vals[j];
+            }
         }
     }
     __syncthreads();
@@ -276,19 +296,31 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
     // registers until every wave has finished reading the staged rows
     double res[kXColsPerWave];
     if (lane < nr) {
-        const T* row = sin + lane * p.sp;
+        const int o = paired ? static_cast<int>((static_cast<size_t>(r0 + lane) * p.n_in + lo) & 1) : 0;
+        const S* rl = sin + lane * p.sp + o - lo;     // rl[i] = input column i of this row
+        const bool aff = p.aff[cb] != 0;
 #pragma unroll
         for (int q = 0; q < kXColsPerWave; ++q) {
             const int c = wave + 4 * q;
             if (c < nxo) {
-                // wave-uniform table row: scalar loads, independent LDS reads
                 const double* wc = p.w + tab0 + static_cast<size_t>(c) * p.k;
                 const int* ic = p.idx + tab0 + static_cast<size_t>(c) * p.k;
-                const T* rl = row - lo;
-                double acc = wc[0] * static_cast<double>(rl[ic[0]]);
+                double acc;
+                if (aff) {
+                    // taps ic[0] + t: one base address, immediate offsets
+                    const S* b = rl + ic[0];
+                    acc = wc[0] * static_cast<double>(b[0]);
+                    int t = 1;
+                    for (; t + 8 <= p.k; t += 8) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc = acc + wc[t + u] * static_cast<double>(b[t + u]);
+                    }
+                    for (; t < p.k; ++t) acc = acc + wc[t] * static_cast<double>(b[t]);
+                } else {
+                    acc = wc[0] * static_cast<double>(rl[ic[0]]);
 #pragma unroll 8
-                for (int t = 1; t < p.k; ++t)
-                    acc = acc + wc[t] * static_cast<double>(rl[ic[t]]);
+                    for (int t = 1; t < p.k; ++t) acc = acc + wc[t] * static_cast<double>(rl[ic[t]]);
+                }
                 res[q] = acc;
             }
         }
@@ -296,7 +328,7 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
     __syncthreads();
     // transpose through LDS (odd pitch po, in doubles, over the staged rows)
     // so that the stores run along the rows
-    double* so = reinterpret_cast<double*>(sin);
+    double* so = reinterpret_cast<double*>(smem_raw);
     const int po = p.xo_blk | 1;
     if (lane < nr) {
 #pragma unroll
@@ -429,22 +461,25 @@ void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, vo
     p.w = w + t.w_off;
     p.idx = idx + t.i_off;
     p.span = idx + t.s_off;
+    p.aff = idx + t.a_off;
     p.n_in = t.n_in;
     p.n_out = t.n_out;
     p.k = t.k;
     p.xo_blk = t.xo_blk;
     p.n_cb = t.n_cb;
-    p.sp = t.span_max | 1;
     p.nrows = static_cast<long long>(ny) * nz;
     p.ol = ol;
     p.n_total = static_cast<size_t>(p.nrows) * t.n_in;
-    {
-        const size_t in_esz = in_t == E_U16 ? 2 : (in_t == E_F32 ? 4 : 8);
-        p.paired = in_t != E_F64 && in_stride == 1 && in_comp == 0 && p.n_total % 2 == 0 &&
-                   reinterpret_cast<uintptr_t>(in) % (2 * in_esz) == 0;
-    }
-    const size_t esz = in_t == E_F64 ? sizeof(double) : sizeof(float);
-    // at most 64 KiB of LDS: per row, the larger of its source span and its
+    const size_t esz = in_t == E_U16 ? 2 : (in_t == E_F32 ? 4 : 8);
+    p.paired = in_t != E_F64 && in_stride == 1 && in_comp == 0 && p.n_total % 2 == 0 &&
+               reinterpret_cast<uintptr_t>(in) % (2 * esz) == 0;
+    // row pitch: room for the span shifted by one element plus the second
+    // element of the last pair, and an odd number of dwords (conflict-free
+    // reads with one row per lane); uint16 rows hold an even element count
+    if (in_t == E_U16) p.sp = 2 * (((t.span_max + 3) / 2) | 1);
+    else if (in_t == E_F32) p.sp = (t.span_max + 2) | 1;
+    else p.sp = t.span_max | 1;
+    // at most 64 KiB of LDS: per row, the larger of its staged span and its
     // transposed output columns
     const size_t per_row = std::max<size_t>(esz * p.sp, sizeof(double) * (t.xo_blk | 1));
     p.rt = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, 65536 / per_row)));

@@ -45,9 +45,11 @@ struct Table {
     size_t i_off = 0;     // ints
     bool identity = false;
     // x-pass blocking: outputs in groups of xo_blk columns; for group g the
-    // source taps lie in [span[2g], span[2g+1]] (ints at s_off in the arena)
+    // source taps lie in [span[2g], span[2g+1]] (ints at s_off in the arena),
+    // and aff[g] (ints at a_off) is 1 when every output of the group reads
+    // consecutive inputs
     int xo_blk = 1, n_cb = 0, span_max = 0;
-    size_t s_off = 0;
+    size_t s_off = 0, a_off = 0;
 };
 
 enum Kernel { KERNEL_CUBIC = 0, KERNEL_TENT = 1 };
@@ -61,8 +63,10 @@ enum Elem { E_F32 = 0, E_U16 = 1, E_F64 = 2 };
 
 // Longest x row accepted by the resampling and LK kernels.
 constexpr int kMaxRow = 8192;
-// Source span (inputs) of one x-pass column group staged per row in LDS.
+// Source span (inputs) of one x-pass column group staged per row in LDS, and
+// the widest column group.
 constexpr int kXSpan = 191;
+constexpr int kXMaxCols = 32;
 enum OutMode { OUT_F64 = 0, OUT_F32 = 1, OUT_MASK = 2 };
 
 // Replicated voxels at each x end of a padded row (the moving level image).
