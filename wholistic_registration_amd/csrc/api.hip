@@ -38,6 +38,7 @@ int n_cp(int n, int r) {
 struct Plan2 {
     Table tx, ty;
     bool x_first;
+    FusedY fy;           // fused x-then-y blocking (rows = 0: two passes)
 };
 
 // Axis order of imresize: ascending scale, stable (imresize.py:170-171,185-187).
@@ -59,6 +60,8 @@ void resample2(const void* in, Elem in_t, int stride, int comp, void* out, OutMo
         launch_resample_x(in, in_t, stride, comp, out, om, div, tx, w, idx, ty.n_in, nz, ol, s);
     } else if (tx.identity) {
         launch_resample_y(in, in_t, stride, comp, out, om, div, ty, w, idx, tx.n_in, nz, ol, s);
+    } else if (pl.x_first && stride == 1 && comp == 0 && fused_ok(tx, pl.fy, in_t)) {
+        launch_resample_xy(in, in_t, out, om, div, tx, ty, pl.fy, w, idx, nz, ol, s);
     } else if (pl.x_first) {
         launch_resample_x(in, in_t, stride, comp, tmp, OUT_F64, 1.0, tx, w, idx, ty.n_in, nz, OutLayout{tx.n_out, 0}, s);
         launch_resample_y(tmp, E_F64, 1, 0, out, om, div, ty, w, idx, tx.n_out, nz, ol, s);
@@ -83,6 +86,7 @@ Plan2 make_plan(int nx_in, int nx_out, int ny_in, int ny_out, Kernel k, std::vec
     p.tx = build_table(nx_in, nx_out, k, wa, ia);
     p.ty = build_table(ny_in, ny_out, k, wa, ia);
     p.x_first = x_before_y(nx_in, nx_out, ny_in, ny_out);
+    if (p.x_first) p.fy = plan_fused_y(p.ty, ia);
     return p;
 }
 

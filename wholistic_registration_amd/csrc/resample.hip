@@ -207,13 +207,13 @@ struct XPass {
 // Output columns of a column group handled by one of the 4 waves.
 constexpr int kXColsPerWave = kXMaxCols / 4;
 
-// x pass: a workgroup owns `rt` rows (one per lane) and one column group.  It
-// stages the group's source span of its rows in LDS in the input's own type
-// (uint16, float32 or float64), then computes with lane = row and wave =
-// column (round robin): the table row of a column is wave-uniform (scalar
-// loads), the per-row reads hit distinct banks (odd row pitch in dwords),
-// and a group whose outputs read consecutive inputs uses immediate LDS
-// offsets.  The results are transposed through LDS and stored along the rows.
+// Staging and x compute of one column group, shared by the x pass and the
+// fused x-then-y pass.  The group's source span of each row is staged in LDS
+// in the input's own type (uint16, float32 or float64); the compute runs with
+// lane = row and wave = column (round robin): the table row of a column is
+// wave-uniform (scalar loads), the per-row reads hit distinct banks (odd row
+// pitch in dwords), and a group whose outputs read consecutive inputs uses
+// immediate LDS offsets.
 //
 // Paired staging (dense 2- or 4-byte input): each lane loads two elements
 // from an even element index, so a row's span starts at LDS element o (0 or
@@ -221,23 +221,22 @@ constexpr int kXColsPerWave = kXMaxCols / 4;
 // 2d + 1.  The loads carry no branch (row index clamped to the block's last
 // row, pair index to the input's last pair), so all of them are in flight
 // together.
+//
+// Stage the span of column group cb of the rows r0 .. r0 + nr - 1 (nr <= 64)
+// and compute the group's outputs of row r0 + lane into res (lane < nr):
+// columns wave, wave + 4, ...  Ends after every wave has read the staged rows.
 template <int IN_T>
-__global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in, int in_stride, int in_comp,
-                                                     void* __restrict__ out, int om, double divisor, XPass p) {
+__device__ __forceinline__ void x_group(const void* __restrict__ in, int in_stride, int in_comp, const XPass& p,
+                                        long long r0, int nr, int cb, unsigned char* smem_raw, int wave, int lane,
+                                        double (&res)[kXColsPerWave]) {
     using S = typename std::conditional<IN_T == E_U16, uint16_t,
                                         typename std::conditional<IN_T == E_F32, float, double>::type>::type;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     S* sin = reinterpret_cast<S*>(smem_raw);
-    const int cb = static_cast<int>(blockIdx.x % p.n_cb);
-    const long long r0 = static_cast<long long>(blockIdx.x / p.n_cb) * p.rt;
-    const int nr = static_cast<int>(min(static_cast<long long>(p.rt), p.nrows - r0));
     const int xo0 = cb * p.xo_blk;
     const int nxo = min(p.xo_blk, p.n_out - xo0);
     const int lo = p.span[2 * cb];
     const int span = p.span[2 * cb + 1] - lo + 1;
     const size_t tab0 = static_cast<size_t>(xo0) * p.k;
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    const int lane = threadIdx.x & 63;
     const bool paired = IN_T != E_F64 && p.paired;
 
     if constexpr (IN_T != E_F64) {
@@ -294,7 +293,6 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
     __syncthreads();
     // the outputs of a lane (its row, columns wave, wave + 4, ...) stay in
     // registers until every wave has finished reading the staged rows
-    double res[kXColsPerWave];
     if (lane < nr) {
         const int o = paired ? static_cast<int>((static_cast<size_t>(r0 + lane) * p.n_in + lo) & 1) : 0;
         const S* rl = sin + lane * p.sp + o - lo;     // rl[i] = input column i of this row
@@ -326,6 +324,23 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
         }
     }
     __syncthreads();
+}
+
+// x pass: a workgroup owns `rt` rows (one per lane) and one column group;
+// the results are transposed through LDS and stored along the rows.
+template <int IN_T>
+__global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in, int in_stride, int in_comp,
+                                                     void* __restrict__ out, int om, double divisor, XPass p) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int cb = static_cast<int>(blockIdx.x % p.n_cb);
+    const long long r0 = static_cast<long long>(blockIdx.x / p.n_cb) * p.rt;
+    const int nr = static_cast<int>(min(static_cast<long long>(p.rt), p.nrows - r0));
+    const int xo0 = cb * p.xo_blk;
+    const int nxo = min(p.xo_blk, p.n_out - xo0);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    double res[kXColsPerWave];
+    x_group<IN_T>(in, in_stride, in_comp, p, r0, nr, cb, smem_raw, wave, lane, res);
     // transpose through LDS (odd pitch po, in doubles, over the staged rows)
     // so that the stores run along the rows
     double* so = reinterpret_cast<double*>(smem_raw);
@@ -344,6 +359,62 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
         if (c < nxo)
             store_out(out, static_cast<size_t>(r0 + rr) * p.ol.pitch + p.ol.pad + xo0 + c, so[rr * po + c], om,
                       divisor);
+    }
+}
+
+// Fused x-then-y pass (both axes resampled, x first): a workgroup owns one
+// column group of the x pass and `ty_rows` output rows of one z slice.  It
+// runs the x pass on the source rows those outputs read (at most 64, one per
+// lane), keeps the float64 results in LDS, and applies the y taps from there,
+// so the float64 intermediate never goes to HBM.  Same arithmetic, in the
+// same order, as the two separate passes.
+struct YFused {
+    const double* w;
+    const int* idx;
+    const int* span;    // per block of output rows: first and last source row
+    int n_in, n_out, k, rows, nblk;
+};
+
+template <int IN_T>
+__global__ __launch_bounds__(256) void k_resample_xy(const void* __restrict__ in, void* __restrict__ out, int om,
+                                                      double divisor, XPass p, YFused q, OutLayout ol) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const unsigned b = blockIdx.x;
+    const int cb = static_cast<int>(b % static_cast<unsigned>(p.n_cb));
+    const unsigned rest = b / static_cast<unsigned>(p.n_cb);
+    const int yb = static_cast<int>(rest % static_cast<unsigned>(q.nblk));
+    const int z = static_cast<int>(rest / static_cast<unsigned>(q.nblk));
+    const int ylo = q.span[2 * yb];
+    const int nr = q.span[2 * yb + 1] - ylo + 1;
+    const long long r0 = static_cast<long long>(z) * q.n_in + ylo;
+    const int xo0 = cb * p.xo_blk;
+    const int nxo = min(p.xo_blk, p.n_out - xo0);
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    double res[kXColsPerWave];
+    x_group<IN_T>(in, 1, 0, p, r0, nr, cb, smem_raw, wave, lane, res);
+    // x results of the block's source rows: xr[row][column], odd pitch
+    double* xr = reinterpret_cast<double*>(smem_raw);
+    const int po = p.xo_blk | 1;
+    if (lane < nr) {
+#pragma unroll
+        for (int u = 0; u < kXColsPerWave; ++u) {
+            const int c = wave + 4 * u;
+            if (c < nxo) xr[lane * po + c] = res[u];
+        }
+    }
+    __syncthreads();
+    // y taps: lanes along the columns, 8 output rows at a time
+    const int c = threadIdx.x & (kXMaxCols - 1);
+    if (c >= nxo) return;
+    const int yo0 = yb * q.rows;
+    const int yo1 = min(yo0 + q.rows, q.n_out);
+    for (int yo = yo0 + static_cast<int>(threadIdx.x) / kXMaxCols; yo < yo1; yo += 256 / kXMaxCols) {
+        const double* wy = q.w + static_cast<size_t>(yo) * q.k;
+        const int* iy = q.idx + static_cast<size_t>(yo) * q.k;
+        double acc = wy[0] * xr[(iy[0] - ylo) * po + c];
+        for (int t = 1; t < q.k; ++t) acc = acc + wy[t] * xr[(iy[t] - ylo) * po + c];
+        store_out(out, (static_cast<size_t>(z) * q.n_out + yo) * ol.pitch + ol.pad + xo0 + c, acc, om, divisor);
     }
 }
 
@@ -454,9 +525,26 @@ __global__ __launch_bounds__(256) void k_pad_rows(float* __restrict__ buf, long 
 
 // ------------------------------------------------------------------ launchers
 
-void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
-                       OutMode om, double divisor, const Table& t, const double* w,
-                       const int* idx, int ny, int nz, OutLayout ol, hipStream_t s) {
+namespace {
+
+// x-pass LDS row pitch (staged elements): room for the span shifted by one
+// element plus the second element of the last pair, and an odd number of
+// dwords (conflict-free reads with one row per lane); uint16 rows hold an even
+// element count.
+int x_pitch(Elem in_t, int span_max) {
+    if (in_t == E_U16) return 2 * (((span_max + 3) / 2) | 1);
+    if (in_t == E_F32) return (span_max + 2) | 1;
+    return span_max | 1;
+}
+
+// LDS bytes per staged row: the larger of its span and its float64 outputs.
+size_t x_lds_per_row(Elem in_t, const Table& t) {
+    const size_t esz = in_t == E_U16 ? 2 : (in_t == E_F32 ? 4 : 8);
+    return std::max<size_t>(esz * x_pitch(in_t, t.span_max), sizeof(double) * (t.xo_blk | 1));
+}
+
+XPass make_xpass(const void* in, Elem in_t, int in_stride, int in_comp, const Table& t, const double* w,
+                 const int* idx, long long nrows, OutLayout ol, size_t* lds_per_row) {
     XPass p;
     p.w = w + t.w_off;
     p.idx = idx + t.i_off;
@@ -467,22 +555,26 @@ void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, vo
     p.k = t.k;
     p.xo_blk = t.xo_blk;
     p.n_cb = t.n_cb;
-    p.nrows = static_cast<long long>(ny) * nz;
+    p.nrows = nrows;
     p.ol = ol;
     p.n_total = static_cast<size_t>(p.nrows) * t.n_in;
     const size_t esz = in_t == E_U16 ? 2 : (in_t == E_F32 ? 4 : 8);
     p.paired = in_t != E_F64 && in_stride == 1 && in_comp == 0 && p.n_total % 2 == 0 &&
                reinterpret_cast<uintptr_t>(in) % (2 * esz) == 0;
-    // row pitch: room for the span shifted by one element plus the second
-    // element of the last pair, and an odd number of dwords (conflict-free
-    // reads with one row per lane); uint16 rows hold an even element count
-    if (in_t == E_U16) p.sp = 2 * (((t.span_max + 3) / 2) | 1);
-    else if (in_t == E_F32) p.sp = (t.span_max + 2) | 1;
-    else p.sp = t.span_max | 1;
-    // at most 64 KiB of LDS: per row, the larger of its staged span and its
-    // transposed output columns
-    const size_t per_row = std::max<size_t>(esz * p.sp, sizeof(double) * (t.xo_blk | 1));
-    p.rt = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, 65536 / per_row)));
+    p.sp = x_pitch(in_t, t.span_max);
+    *lds_per_row = x_lds_per_row(in_t, t);
+    p.rt = static_cast<int>(std::max<size_t>(1, std::min<size_t>(64, 65536 / *lds_per_row)));
+    return p;
+}
+
+}  // namespace
+
+void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
+                       OutMode om, double divisor, const Table& t, const double* w,
+                       const int* idx, int ny, int nz, OutLayout ol, hipStream_t s) {
+    size_t per_row = 0;
+    const XPass p = make_xpass(in, in_t, in_stride, in_comp, t, w, idx, static_cast<long long>(ny) * nz, ol,
+                               &per_row);
     const size_t lds = per_row * p.rt;
     const long long groups = (p.nrows + p.rt - 1) / p.rt;
     const unsigned blocks = static_cast<unsigned>(groups * t.n_cb);
@@ -495,6 +587,76 @@ void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, vo
             break;
         default:
             k_resample_x<E_F64><<<blocks, 256, lds, s>>>(in, in_stride, in_comp, out, om, divisor, p);
+    }
+}
+
+FusedY plan_fused_y(const Table& ty, std::vector<int>& i_arena) {
+    FusedY f;
+    if (ty.identity) return f;
+    const int* idx = &i_arena[ty.i_off];
+    auto worst_span = [&](int rows) {
+        int worst = 0;
+        for (int b = 0; b * rows < ty.n_out; ++b) {
+            int lo = ty.n_in, hi = 0;
+            for (int j = b * rows; j < std::min(ty.n_out, (b + 1) * rows); ++j)
+                for (int k = 0; k < ty.k; ++k) {
+                    lo = std::min(lo, idx[static_cast<size_t>(j) * ty.k + k]);
+                    hi = std::max(hi, idx[static_cast<size_t>(j) * ty.k + k]);
+                }
+            worst = std::max(worst, hi - lo + 1);
+        }
+        return worst;
+    };
+    int rows = std::min(ty.n_out, 64);
+    while (rows >= kFusedMinRows && worst_span(rows) > 64) --rows;
+    if (rows < kFusedMinRows) return f;
+    f.rows = rows;
+    f.nblk = (ty.n_out + rows - 1) / rows;
+    f.s_off = i_arena.size();
+    for (int b = 0; b < f.nblk; ++b) {
+        int lo = ty.n_in, hi = 0;
+        for (int j = b * rows; j < std::min(ty.n_out, (b + 1) * rows); ++j)
+            for (int k = 0; k < ty.k; ++k) {
+                lo = std::min(lo, i_arena[ty.i_off + static_cast<size_t>(j) * ty.k + k]);
+                hi = std::max(hi, i_arena[ty.i_off + static_cast<size_t>(j) * ty.k + k]);
+            }
+        i_arena.push_back(lo);
+        i_arena.push_back(hi);
+    }
+    return f;
+}
+
+
+bool fused_ok(const Table& tx, const FusedY& fy, Elem in_t) {
+    // 64 staged source rows must fit the 64 KiB of LDS
+    return fy.rows > 0 && !tx.identity && x_lds_per_row(in_t, tx) * 64 <= 65536;
+}
+
+void launch_resample_xy(const void* in, Elem in_t, void* out, OutMode om, double divisor, const Table& tx,
+                        const Table& ty, const FusedY& fy, const double* w, const int* idx, int nz, OutLayout ol,
+                        hipStream_t s) {
+    size_t per_row = 0;
+    const XPass p = make_xpass(in, in_t, 1, 0, tx, w, idx, static_cast<long long>(ty.n_in) * nz, ol, &per_row);
+    YFused q;
+    q.w = w + ty.w_off;
+    q.idx = idx + ty.i_off;
+    q.span = idx + fy.s_off;
+    q.n_in = ty.n_in;
+    q.n_out = ty.n_out;
+    q.k = ty.k;
+    q.rows = fy.rows;
+    q.nblk = fy.nblk;
+    const size_t lds = per_row * 64;
+    const unsigned blocks = static_cast<unsigned>(static_cast<long long>(tx.n_cb) * fy.nblk * nz);
+    switch (in_t) {
+        case E_F32:
+            k_resample_xy<E_F32><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol);
+            break;
+        case E_U16:
+            k_resample_xy<E_U16><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol);
+            break;
+        default:
+            k_resample_xy<E_F64><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol);
     }
 }
 

@@ -85,6 +85,21 @@ struct OutLayout {
 void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                        OutMode om, double divisor, const Table& t, const double* w,
                        const int* idx, int ny, int nz, OutLayout ol, hipStream_t s);
+// Fused x-then-y resample: output rows in blocks of `rows` (0 = not
+// possible), block b reading source rows [span[2b], span[2b+1]] (ints at
+// s_off in the arena), at most 64 of them.
+constexpr int kFusedMinRows = 8;
+struct FusedY {
+    int rows = 0, nblk = 0;
+    size_t s_off = 0;
+};
+FusedY plan_fused_y(const Table& ty, std::vector<int>& i_arena);
+// The fused pass applies: both tables non-identity, x first, dense input.
+bool fused_ok(const Table& tx, const FusedY& fy, Elem in_t);
+// in (nz, ty.n_in, tx.n_in) dense -> out (nz, ty.n_out, tx.n_out) with layout ol.
+void launch_resample_xy(const void* in, Elem in_t, void* out, OutMode om, double divisor, const Table& tx,
+                        const Table& ty, const FusedY& fy, const double* w, const int* idx, int nz, OutLayout ol,
+                        hipStream_t s);
 // Resample along y: in (nz, t.n_in, nx) -> out (nz, t.n_out, nx).
 void launch_resample_y(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                        OutMode om, double divisor, const Table& t, const double* w,
