@@ -321,7 +321,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         const size_t ncp = static_cast<size_t>(g.ncx) * g.ncy * nz;
         ncp_max = std::max(ncp_max, ncp);
         nblk_max = std::max(nblk_max, static_cast<size_t>(tiles[L].blocks));
-        ncpb_max = std::max(ncpb_max, (ncp + 255) / 256);
+        ncpb_max = std::max(ncpb_max, (ncp + 63) / 64);   // neighbour-difference workgroups of >= 64 threads
     }
     if (prm->forced_records || prm->forced_updates)
         WR_ARG(prm->forced_records && prm->forced_updates, "wr_get_motion: forced schedule needs both arrays");

@@ -29,6 +29,7 @@
 #include <cmath>
 
 #include "lk_common.h"
+#include "lk_ctl.h"
 #include "wr_internal.h"
 
 namespace wr {
@@ -68,107 +69,6 @@ __global__ __launch_bounds__(256) void k_cp_init(const float* __restrict__ m0, f
     cp[i] = m0[v];
     cp[i + ncp] = m0[v + nvox];
     cp[i + 2 * ncp] = m0[v + 2 * nvox];
-}
-
-// 3x3 neighbour difference on the CP grid, replicated edges, float64
-// accumulation in the filter's C order then rounded to float32 (scipy
-// correlate on a float32 array), z component times zRatio_L in float32.
-__global__ __launch_bounds__(256) void k_neidiff(const Ctl* __restrict__ ctl, const float* __restrict__ cp,
-                                                  float* __restrict__ dn, double* __restrict__ penpart,
-                                                  int ncx, int ncy, int nz, float z_ratio) {
-    if (ctl->done) return;
-    __shared__ double s_w[4];
-    const long long ncp = static_cast<long long>(ncx) * ncy * nz;
-    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-    double q = 0.0;
-    if (i < ncp) {
-        const int ix = static_cast<int>(i % ncx);
-        const int iy = static_cast<int>((i / ncx) % ncy);
-        const long long zb = i - (static_cast<long long>(iy) * ncx + ix);
-        float dd[3];
-        for (int c = 0; c < 3; ++c) {
-            const float* f = cp + c * ncp + zb;
-            double t = 0.0;
-            for (int a = -1; a <= 1; ++a) {
-                const int xi = clampi(ix + a, 0, ncx - 1);
-                for (int b = -1; b <= 1; ++b) {
-                    const int yi = clampi(iy + b, 0, ncy - 1);
-                    const double wgt = (a == 0 && b == 0) ? -1.0 : 0.125;
-                    t = __dadd_rn(t, __dmul_rn(wgt, static_cast<double>(f[yi * ncx + xi])));
-                }
-            }
-            dd[c] = static_cast<float>(t);
-        }
-        dd[2] = __fmul_rn(dd[2], z_ratio);
-        dn[i] = dd[0];
-        dn[i + ncp] = dd[1];
-        dn[i + 2 * ncp] = dd[2];
-        const float qq = __fadd_rn(__fadd_rn(__fmul_rn(dd[0], dd[0]), __fmul_rn(dd[1], dd[1])), __fmul_rn(dd[2], dd[2]));
-        q = static_cast<double>(qq);
-    }
-    for (int off = 32; off > 0; off >>= 1) q += __shfl_down(q, off, 64);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = q;
-    __syncthreads();
-    if (threadIdx.x == 0) penpart[blockIdx.x] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
-}
-
-// Error terms, log and the error-based stop rule of iteration k.
-__global__ __launch_bounds__(1024) void k_err_finish(Ctl* __restrict__ c, const double* __restrict__ errpart,
-                                                      long long nerr, const double* __restrict__ penpart,
-                                                      long long npen, double* __restrict__ log,
-                                                      int* __restrict__ counts) {
-    if (c->done) return;
-    __shared__ double s_e[16], s_p[16];
-    double e = 0.0, pp = 0.0;
-    for (long long i = threadIdx.x; i < nerr; i += blockDim.x) e += errpart[i];
-    for (long long i = threadIdx.x; i < npen; i += blockDim.x) pp += penpart[i];
-    for (int off = 32; off > 0; off >>= 1) {
-        e += __shfl_down(e, off, 64);
-        pp += __shfl_down(pp, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        s_e[threadIdx.x >> 6] = e;
-        s_p[threadIdx.x >> 6] = pp;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double esum = 0.0, psum = 0.0;
-    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
-        esum += s_e[w];
-        psum += s_p[w];
-    }
-    const double nvox = c->inv_n;
-    const double diff = esum / nvox;
-    const double pen = (psum * c->s_pen) / nvox;
-    const double cur = diff + pen;
-    const int k = c->rec;
-    double* row = log + static_cast<size_t>(k) * 4;
-    row[0] = diff;
-    row[1] = pen;
-    row[2] = cur;
-    c->rec = k + 1;
-    counts[0] = k + 1;
-    c->last_err = cur;
-    bool stop;
-    if (c->frec >= 0) {
-        stop = (k == c->frec - 1) && (c->fupd == c->frec - 1);
-    } else if (k == c->iter - 1) {
-        stop = true;
-    } else {
-        const int worse = (c->old_err[0] <= cur) + (c->old_err[1] <= cur) + (c->old_err[2] <= cur);
-        if (worse > 1) {
-            stop = true;
-        } else if (fabs(c->old_err[2] - cur) < c->tol) {
-            stop = true;
-        } else {
-            stop = false;
-            c->old_err[0] = c->old_err[1];
-            c->old_err[1] = c->old_err[2];
-            c->old_err[2] = cur;
-        }
-    }
-    c->maxdiff = 0u;
-    if (stop) c->done = 1;
 }
 
 __device__ __forceinline__ void block_max_to(float md, float* __restrict__ dst) {
@@ -259,31 +159,22 @@ __global__ __launch_bounds__(256) void k_dense_maxdiff(const Ctl* __restrict__ c
     block_max_to(md, mdpart + blockIdx.x);
 }
 
+// Error terms, log and the error-based stop rule of iteration k.
+__global__ __launch_bounds__(1024) void k_err_finish(Ctl* __restrict__ c, const double* __restrict__ errpart,
+                                                      long long nerr, const double* __restrict__ penpart,
+                                                      long long npen, double* __restrict__ log,
+                                                      int* __restrict__ counts) {
+    if (c->done) return;
+    err_finish_block(c, errpart, nerr, penpart, npen, log, counts);
+}
+
 // The |delta motion| < 1e-3 stop test of the update just made; `nparts`
 // per-workgroup maxima from k_solve (and k_dense_maxdiff on a level's first
 // update) are reduced here.
 __global__ __launch_bounds__(1024) void k_check(Ctl* __restrict__ c, const float* __restrict__ mdpart, int nparts,
                                                  double* __restrict__ log, int* __restrict__ counts) {
     if (c->done) return;
-    __shared__ float s_m[16];
-    float md = 0.0f;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) md = fmaxf(md, mdpart[i]);
-    for (int off = 32; off > 0; off >>= 1) md = fmaxf(md, __shfl_down(md, off, 64));
-    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = md;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) md = fmaxf(md, s_m[w]);
-    const int k = c->rec - 1;
-    c->maxdiff = __float_as_uint(md);
-    log[static_cast<size_t>(k) * 4 + 3] = static_cast<double>(md);
-    c->upd += 1;
-    counts[1] = c->upd;
-    counts[2] = c->nan;
-    if (c->frec >= 0) {
-        if (k == c->frec - 1) c->done = 1;
-    } else if (md < 1e-3f) {
-        c->done = 1;
-    }
+    check_finish_block(c, mdpart, nparts, log, counts);
 }
 
 // Dense field at the end of a level: the CP interpolant when the level made
@@ -378,12 +269,12 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     (void)iter;
     const long long ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
     const unsigned cpb = blocks_for(ncp, 256);
-    k_neidiff<<<cpb, 256, 0, s>>>(b.ctl, b.cp, b.dnei, b.penpart, g.ncx, g.ncy, g.nz, g.z_ratio);
-
+    // the LK launch also computes the neighbour differences (appended
+    // workgroups, one per `threads` CPs)
     if (b.events) (void)hipEventRecord(b.events[2 * k], s);
-    launch_lk_pass(g, t, b, k, s);
+    const long long npen = launch_lk_pass(g, t, b, k, s);
     if (b.events) (void)hipEventRecord(b.events[2 * k + 1], s);
-    k_err_finish<<<1, 1024, 0, s>>>(b.ctl, b.errpart, t.blocks, b.penpart, cpb, b.log, b.counts);
+    k_err_finish<<<1, 1024, 0, s>>>(b.ctl, b.errpart, t.blocks, b.penpart, npen, b.log, b.counts);
     k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d, b.mdpart);
     int nparts = static_cast<int>(cpb);
     if (k == 0) {

@@ -11,9 +11,13 @@
 // Tiles: a workgroup owns `cols` = np_x*(2r+1) columns and `rows` = 2r+1
 // rows of one z slice, i.e. whole patches, so each per-patch sum is finished
 // inside one workgroup and written once (no atomics, a deterministic order).
-// Threads own one column each and walk the rows; the 3x3 residual mean uses
-// a register ring for the vertical sum and one LDS exchange per row for the
-// horizontal one.  Two extra lanes and two extra rows recompute the halo.
+// A wave owns 62 columns plus one halo lane on each side and walks the rows
+// on its own; the 3x3 residual mean uses a register ring for the vertical sum
+// and cross-lane shuffles for the horizontal one.  Two extra rows recompute
+// the halo.
+//
+// The launch also carries, after the tiles, the workgroups of the iteration's
+// neighbour differences on the CP grid (they read only the CP field).
 // Workgroups are assigned XCD-major (each XCD gets a contiguous range of
 // tiles) with z fastest, so z-neighbour tiles, which share moving-image
 // slices, run on the same XCD and hit its L2.
@@ -30,6 +34,7 @@
 #include <cmath>
 
 #include "lk_common.h"
+#include "lk_ctl.h"
 #include "wr_internal.h"
 
 namespace wr {
@@ -121,7 +126,7 @@ struct Grad {
 
 // The 32 voxels around one deformed coordinate, as 4 x 16-byte rows holding
 // X0..X3 at (Y1|Y2, Z1|Z2) and 8 x 8-byte rows holding X1, X2 for the y and
-// z lines (see tri_grad).
+// z lines (see eval).
 struct Taps {
     f4u q11, q21, q12, q22;
     f2u e01, e02, e31, e32, f10, f20, f13, f23;
@@ -230,6 +235,9 @@ struct PassArgs {
     float* sums;
     double* errpart;
     const Ctl* ctl;
+    float* dnei;           // neighbour differences (written by the appended workgroups)
+    double* penpart;       // their per-workgroup sums of |D|^2
+    long long npen;        // appended neighbour-difference workgroups
     int nx, ny, nz, r, p, ncx, ncy, extra_x, extra_y;
     int cols, rows, nbx, nby, np_x;
     long long blocks;
@@ -259,6 +267,17 @@ __device__ __forceinline__ void tile_of(long long nb, int nz, int nbx, int& bx, 
 template <bool DENSE, bool MREF, bool MMOV>
 __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     if (a.ctl->done) return;
+    if (static_cast<long long>(blockIdx.x) >= a.blocks) {
+        // appended workgroups: neighbour differences of the CP field
+        // (calFlow3d_Wei_v1.py:69-98,488-498) and their |D|^2 partials
+        const long long pb = static_cast<long long>(blockIdx.x) - a.blocks;
+        const long long i = pb * blockDim.x + threadIdx.x;
+        double q = 0.0, unused = 0.0;
+        if (i < a.ncp) q = neidiff_one(a.cp, a.dnei, i, a.ncp, a.ncx, a.ncy, a.z_ratio);
+        block_sum2(q, unused);
+        if (threadIdx.x == 0) a.penpart[pb] = q;
+        return;
+    }
     __shared__ float s_red[9][kMaxThreads];
     __shared__ double s_err[kMaxThreads / 64];
     __shared__ float s_rt[kMaxThreads];     // CP y-weight of each walked row
@@ -450,7 +469,7 @@ PVol make_pvol(const float* base_x0, int pitch, int nx, int ny, int nz) {
 
 }  // namespace
 
-void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s) {
+long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s) {
     PassArgs a;
     a.mov = make_pvol(b.mov, b.pitch, g.nx, g.ny, g.nz);
     a.mmov = make_pvol(b.mmov ? b.mmov : b.mov, b.pitch, g.nx, g.ny, g.nz);
@@ -461,6 +480,8 @@ void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int
     a.sums = b.sums;
     a.errpart = b.errpart;
     a.ctl = b.ctl;
+    a.dnei = b.dnei;
+    a.penpart = b.penpart;
     a.nx = g.nx;
     a.ny = g.ny;
     a.nz = g.nz;
@@ -479,7 +500,8 @@ void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int
     a.ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
     a.z_ratio = g.z_ratio;
     const int threads = 64 * ((t.cols + kWaveCols - 1) / kWaveCols);
-    const dim3 grid(static_cast<unsigned>(t.blocks));
+    a.npen = (a.ncp + threads - 1) / threads;
+    const dim3 grid(static_cast<unsigned>(t.blocks + a.npen));
     const bool mr = b.mref != nullptr, mm = b.mmov != nullptr;
     using Fn = void (*)(PassArgs);
     static const Fn table[8] = {k_lk_pass<false, false, false>, k_lk_pass<false, false, true>,
@@ -488,6 +510,7 @@ void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int
                                 k_lk_pass<true, true, false>,   k_lk_pass<true, true, true>};
     const int sel = (k == 0 ? 4 : 0) | (mr ? 2 : 0) | (mm ? 1 : 0);
     hipLaunchKernelGGL(table[sel], grid, dim3(threads), 0, s, a);
+    return a.npen;
 }
 
 }  // namespace wr

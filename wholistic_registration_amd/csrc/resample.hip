@@ -592,7 +592,10 @@ void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, vo
 
 FusedY plan_fused_y(const Table& ty, std::vector<int>& i_arena) {
     FusedY f;
-    if (ty.identity) return f;
+    // downsampling only: an upsample's y pass writes more than it reads, and
+    // the separate y pass (two x per thread, eight rows per workgroup) is the
+    // faster way to stream it
+    if (ty.identity || ty.n_out >= ty.n_in) return f;
     const int* idx = &i_arena[ty.i_off];
     auto worst_span = [&](int rows) {
         int worst = 0;

@@ -124,7 +124,7 @@ struct Ctl {
     double old_err[3];   // the last three accepted errors (calFlow3d:426,528-530)
     double tol;
     double s_pen;        // smoothPenalty * (2r+1)^2
-    double inv_n;        // 1 / (x*y*z)
+    double inv_n;        // x*y*z of the level (the error terms divide by it)
     double last_err;
 };
 
@@ -163,7 +163,7 @@ struct LkBuffers {
     float* dnei;             // neighbour differences, 3 planes
     float* sums;             // 9 planes of per-patch sums
     double* errpart;         // per-workgroup sum of It^2
-    double* penpart;         // per-workgroup sum of |D|^2
+    double* penpart;         // per-workgroup sum of |D|^2 (one per 64 CPs at most)
     Ctl* ctl;
     double* log;             // [level][iter][4]: diff, pen, cur, maxdiff
     int* counts;             // [level][3]: rec, upd, nan
@@ -174,8 +174,9 @@ struct LkBuffers {
 // Partial-maximum slots needed by one iteration with `ncp` control points.
 size_t mdpart_slots(long long ncp);
 
-// The fused per-voxel pass of iteration k (lk_pass.hip).
-void launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s);
+// The fused per-voxel pass of iteration k plus the neighbour differences of
+// the CP field (lk_pass.hip); returns the number of |D|^2 partials written.
+long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s);
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
                       double inv_n, hipStream_t s);
