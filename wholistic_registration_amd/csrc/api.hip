@@ -38,7 +38,8 @@ int n_cp(int n, int r) {
 struct Plan2 {
     Table tx, ty;
     bool x_first;
-    FusedY fy;           // fused x-then-y blocking (rows = 0: two passes)
+    FusedY fy;           // fused x-then-y blocking for downsampling (rows = 0: none)
+    FusedY fu;           // direct x-then-y blocking for upsampling (rows = 0: none)
 };
 
 // Axis order of imresize: ascending scale, stable (imresize.py:170-171,185-187).
@@ -60,6 +61,8 @@ void resample2(const void* in, Elem in_t, int stride, int comp, void* out, OutMo
         launch_resample_x(in, in_t, stride, comp, out, om, div, tx, w, idx, ty.n_in, nz, ol, s);
     } else if (tx.identity) {
         launch_resample_y(in, in_t, stride, comp, out, om, div, ty, w, idx, tx.n_in, nz, ol, s);
+    } else if (pl.x_first && stride == 1 && comp == 0 && pl.fu.rows > 0) {
+        launch_resample_up(in, in_t, out, om, div, tx, ty, pl.fu, w, idx, nz, ol, s);
     } else if (pl.x_first && stride == 1 && comp == 0 && fused_ok(tx, pl.fy, in_t)) {
         launch_resample_xy(in, in_t, out, om, div, tx, ty, pl.fy, w, idx, nz, ol, s);
     } else if (pl.x_first) {
@@ -86,7 +89,10 @@ Plan2 make_plan(int nx_in, int nx_out, int ny_in, int ny_out, Kernel k, std::vec
     p.tx = build_table(nx_in, nx_out, k, wa, ia);
     p.ty = build_table(ny_in, ny_out, k, wa, ia);
     p.x_first = x_before_y(nx_in, nx_out, ny_in, ny_out);
-    if (p.x_first) p.fy = plan_fused_y(p.ty, ia);
+    if (p.x_first) {
+        p.fy = plan_fused_y(p.ty, ia);
+        p.fu = plan_up_y(p.tx, p.ty, ia);
+    }
     return p;
 }
 
@@ -257,6 +263,7 @@ int wr_correct_motion(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, in
     WR_ARG(ctx && in && motion && out, "wr_correct_motion: null argument");
     WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_correct_motion: empty volume");
     WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_correct_motion: dtype must be WR_F32 or WR_U16");
+    WR_ARG(256.0 * ((nx + 255) / 256) * ny * nz < 4294967296.0, "wr_correct_motion: volume too large for one launch");
     WR_HIP_CHECK(hipSetDevice(ctx->c.device));
     launch_warp(in, elem_of(dtype), nx, ny, nz, motion, out, static_cast<hipStream_t>(stream));
     WR_HIP_CHECK(hipGetLastError());
@@ -272,6 +279,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     // the LK kernels index the three field planes with 32-bit offsets
     WR_ARG(3.0 * nx * ny * nz < 2147483647.0, "wr_get_motion: volume exceeds 715M voxels");
     WR_ARG(nx <= kMaxRow, "wr_get_motion: x extent above 8192 voxels");
+    // row-structured launches: 256 threads per (started) 256 voxels of a row
+    WR_ARG(256.0 * ((nx + 255) / 256) * ny * nz < 4294967296.0, "wr_get_motion: too many rows for one launch");
     const int layer = prm->layer, iter = prm->iter, r = prm->r;
     WR_ARG(layer >= 0 && layer < WR_MAX_LEVELS, "wr_get_motion: layer out of range");
     WR_ARG(iter >= 1, "wr_get_motion: iter must be at least 1");
@@ -330,11 +339,13 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     const size_t n0 = static_cast<size_t>(nx) * ny * nz;
     WR_TRY(upload_tables(c, wa, ia));
     WR_TRY(grow(c.tmp, tmp_need));
-    // the moving image (and mask_mov) of every level live in rows padded by
-    // kPad replicated voxels on both x ends; the template is dense
-    const int pitch0 = nx + 2 * kPad;
-    const size_t padded0 = static_cast<size_t>(pitch0) * ny * nz;
-    WR_ARG(3.0 * padded0 < 2147483647.0, "wr_get_motion: volume exceeds 715M voxels");
+    // the moving image (and mask_mov) of every level is stored padded in x, y
+    // and z (padded_layout); the template is dense.  The LK gathers address
+    // it with 32-bit byte offsets and 24-bit row products.
+    const size_t padded0 = padded_floats(nx, ny, nz);
+    WR_ARG(4.0 * padded0 < 4294967295.0, "wr_get_motion: padded moving volume above 4 GiB");
+    WR_ARG(static_cast<double>(nz + kPadLo + kPadHi) * (ny + kPadLo + kPadHi) < 16777216.0,
+           "wr_get_motion: more than 2^24 padded rows");
     WR_TRY(grow(c.mov_l, padded0 * sizeof(float)));
     if (layer > 0 || dtype != WR_F32) WR_TRY(grow(c.ref_l, n0 * sizeof(float)));
     if (mask_ref) WR_TRY(grow(c.mref_l, n0));
@@ -375,11 +386,9 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         const size_t nv = static_cast<size_t>(g.nx) * g.ny * nz;
 
         // level images (float32) and masks
-        const int pitch = g.nx + 2 * kPad;
-        const long long rows = static_cast<long long>(g.ny) * nz;
-        const OutLayout padded{pitch, kPad};
+        const OutLayout padded = padded_layout(g.nx, g.ny);
         resample2(mov, et, 1, 0, c.mov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, s);
-        launch_pad_rows(c.mov_l.as<float>(), rows, g.nx, pitch, s);
+        launch_pad_volume(c.mov_l.as<float>(), g.nx, g.ny, nz, s);
         const float* ref_l;
         if (L == 0 && dtype == WR_F32) {
             ref_l = static_cast<const float*>(ref);
@@ -391,7 +400,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             resample2(mask_ref, E_F32, 1, 0, c.mref_l.p, OUT_MASK, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), s);
         if (mask_mov) {
             resample2(mask_mov, E_F32, 1, 0, c.mmov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, s);
-            launch_pad_rows(c.mmov_l.as<float>(), rows, g.nx, pitch, s);
+            launch_pad_volume(c.mmov_l.as<float>(), g.nx, g.ny, nz, s);
         }
 
         // initial dense field of the level, 3 planes
@@ -417,11 +426,10 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
                          static_cast<double>(nv), s);
 
         LkBuffers b;
-        b.pitch = pitch;
-        b.mov = c.mov_l.as<float>() + kPad;
+        b.mov = c.mov_l.as<float>();
         b.ref = ref_l;
         b.mref = mask_ref ? c.mref_l.as<uint8_t>() : nullptr;
-        b.mmov = mask_mov ? c.mmov_l.as<float>() + kPad : nullptr;
+        b.mmov = mask_mov ? c.mmov_l.as<float>() : nullptr;
         b.m0 = m0;
         b.cp = c.cp.as<float>();
         b.dnei = c.dnei.as<float>();

@@ -141,19 +141,25 @@ __global__ __launch_bounds__(256) void k_dense_maxdiff(const Ctl* __restrict__ c
                                                         const float* __restrict__ cp, int nx, int ny, int nz,
                                                         int r, int p, int ncx, int ncy, float* __restrict__ mdpart) {
     if (c->done) return;
-    const int plane = nx * ny;
-    const int nvox = plane * nz;
-    const int ncp = ncx * ncy * nz;
+    const size_t nvox = static_cast<size_t>(nx) * ny * nz;
+    const size_t ncp = static_cast<size_t>(ncx) * ncy * nz;
+    const unsigned nrows = static_cast<unsigned>(ny) * static_cast<unsigned>(nz);
     float md = 0.0f;
-    for (int v = blockIdx.x * 256 + threadIdx.x; v < nvox; v += kDenseBlocks * 256) {
-        const int x = v % nx;
-        const int y = (v / nx) % ny;
-        const int z = v / plane;
-        const Axis1 ax = cp_axis(x, r, p, ncx), ay = cp_axis(y, r, p, ncy);
-        const float* base = cp + z * ncy * ncx;
-        for (int q = 0; q < 3; ++q) {
-            const float m1 = cp_bilinear(base + q * ncp, ncx, ax, ay);
-            md = fmaxf(md, fabsf(m1 - m0[v + q * nvox]));
+    // rows (y, z) round robin over the workgroups; the row's CP weights are
+    // uniform, the column's per thread
+    for (unsigned row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const int y = static_cast<int>(row % static_cast<unsigned>(ny));
+        const int z = static_cast<int>(row / static_cast<unsigned>(ny));
+        const Axis1 ay = cp_axis(y, r, p, ncy);
+        const float* base = cp + static_cast<size_t>(z) * ncy * ncx;
+        const float* mrow = m0 + static_cast<size_t>(row) * nx;
+        for (int x = threadIdx.x; x < nx; x += 256) {
+            const Axis1 ax = cp_axis(x, r, p, ncx);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float m1 = cp_bilinear(base + q * ncp, ncx, ax, ay);
+                md = fmaxf(md, fabsf(m1 - mrow[x + q * nvox]));
+            }
         }
     }
     block_max_to(md, mdpart + blockIdx.x);
@@ -181,20 +187,23 @@ __global__ __launch_bounds__(1024) void k_check(Ctl* __restrict__ c, const float
 // at least one update, else its initial field.  Writes the 3-plane layout
 // (for the next level's upsample) and/or the interleaved (Z,Y,X,3) layout
 // (the returned motion), and optionally motion + grid.
+typedef float f3u __attribute__((ext_vector_type(3), aligned(4)));
+
 __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, const float* __restrict__ m0,
                                                     const float* __restrict__ cp, int nx, int ny, int nz, int r,
-                                                    int p, int ncx, int ncy, float* __restrict__ soa,
+                                                    int p, int ncx, int ncy, int xblocks, float* __restrict__ soa,
                                                     float* __restrict__ aos, float* __restrict__ coords,
                                                     int* __restrict__ counts) {
-    const size_t plane = static_cast<size_t>(nx) * ny;
-    const size_t nvox = plane * nz;
+    const unsigned b = blockIdx.x;
+    const unsigned row = b / static_cast<unsigned>(xblocks);
+    const int x = static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x);
+    if (b == 0 && threadIdx.x == 0) counts[2] = c->nan;
+    if (x >= nx) return;
+    const int y = static_cast<int>(row % static_cast<unsigned>(ny));
+    const int z = static_cast<int>(row / static_cast<unsigned>(ny));
+    const size_t nvox = static_cast<size_t>(nx) * ny * nz;
     const size_t ncp = static_cast<size_t>(ncx) * ncy * nz;
-    const size_t v = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    if (v == 0) counts[2] = c->nan;
-    if (v >= nvox) return;
-    const int x = static_cast<int>(v % nx);
-    const int y = static_cast<int>((v / nx) % ny);
-    const int z = static_cast<int>(v / plane);
+    const size_t v = static_cast<size_t>(row) * nx + x;
     float m[3];
     if (c->upd > 0) {
         const Axis1 ax = cp_axis(x, r, p, ncx), ay = cp_axis(y, r, p, ncy);
@@ -205,12 +214,19 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
     }
     if (soa)
         for (int q = 0; q < 3; ++q) soa[v + q * nvox] = m[q];
-    if (aos)
-        for (int q = 0; q < 3; ++q) aos[3 * v + q] = m[q];
+    if (aos) {
+        f3u o;
+        o.x = m[0];
+        o.y = m[1];
+        o.z = m[2];
+        *reinterpret_cast<f3u*>(aos + 3 * v) = o;
+    }
     if (coords) {
-        coords[3 * v + 0] = static_cast<float>(x) + m[0];
-        coords[3 * v + 1] = static_cast<float>(y) + m[1];
-        coords[3 * v + 2] = static_cast<float>(z) + m[2];
+        f3u o;
+        o.x = static_cast<float>(x) + m[0];
+        o.y = static_cast<float>(y) + m[1];
+        o.z = static_cast<float>(z) + m[2];
+        *reinterpret_cast<f3u*>(coords + 3 * v) = o;
     }
 }
 
@@ -290,9 +306,10 @@ void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int ite
                       float* motion_aos, float* coords_aos, hipStream_t s) {
     (void)level;
     (void)iter;
-    const long long nvox = static_cast<long long>(g.nx) * g.ny * g.nz;
-    k_level_end<<<blocks_for(nvox, 256), 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx,
-                                                      g.ncy, dense_soa, motion_aos, coords_aos, b.counts);
+    const int xblocks = (g.nx + 255) / 256;
+    const unsigned blocks = static_cast<unsigned>(static_cast<long long>(xblocks) * g.ny * g.nz);
+    k_level_end<<<blocks, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, xblocks,
+                                       dense_soa, motion_aos, coords_aos, b.counts);
 }
 
 }  // namespace wr

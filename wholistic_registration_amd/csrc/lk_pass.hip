@@ -59,36 +59,37 @@ __device__ __forceinline__ Samp split(float c, float hi) {
     return s;
 }
 
-// A level image with padded rows: voxel (x, y, z) is the float at byte
-// offset z*planeB + y*pitchB + (x + kPad)*4 of `buf`, for x in
-// [-kPad, nx - 1 + kPad]; the pads hold the edge voxel of the row.
+// A level image stored padded (padded_layout): voxel (x, y, z), for x in
+// [-kPad, nx - 1 + kPad], y in [-kPadLo, ny - 1 + kPadHi] and z likewise,
+// is the float at byte offset origin + z*planeB + y*pitchB + x*4 of the
+// allocation; the pads hold the nearest voxel of the volume, so every tap of
+// a clamped gather is an unconditional load: one per-lane 32-bit offset from
+// the allocation start plus the tap row's fixed offset.
 struct PVol {
-    const unsigned char* buf;
-    unsigned pitchB, planeB;
+    const float* base;       // allocation start
+    unsigned pitchB, planeB; // row and plane strides in bytes
+    unsigned plane_rows;     // stored rows per plane
+    unsigned origin;         // byte offset of voxel (0, 0, 0)
     int nx, ny, nz;
-    float hx, hy, hz;      // n - 1
+    float hx, hy, hz;        // n - 1
 };
 
-typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f4v __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f2v __attribute__((ext_vector_type(2), aligned(4)));
 
-template <class V>
-__device__ __forceinline__ V ld(const PVol& v, unsigned off) {
-    return *reinterpret_cast<const V*>(v.buf + off);
+// 16 or 8 bytes at byte offset off of the allocation (4-byte aligned).
+__device__ __forceinline__ f4v ld4(const PVol& v, unsigned off) {
+    return *reinterpret_cast<const f4v*>(reinterpret_cast<const unsigned char*>(v.base) + off);
+}
+__device__ __forceinline__ f2v ld2(const PVol& v, unsigned off) {
+    return *reinterpret_cast<const f2v*>(reinterpret_cast<const unsigned char*>(v.base) + off);
 }
 
-// Byte offsets of the clamped line points f-1, f, f+1, f+2 along an axis of
-// n points with stride `st`.
-struct Line4 {
-    unsigned o0, o1, o2, o3;
-};
-__device__ __forceinline__ Line4 line4(int f, int n, unsigned st) {
-    Line4 l;
-    l.o1 = static_cast<unsigned>(f) * st;
-    l.o0 = f > 0 ? l.o1 - st : l.o1;
-    l.o2 = f < n - 1 ? l.o1 + st : l.o1;
-    l.o3 = f < n - 2 ? l.o2 + st : l.o2;
-    return l;
+// Byte offset of voxel (fx, fy, fz) from the origin: (fz * plane_rows + fy)
+// rows (fewer than 2^24) of pitchB (< 2^24) bytes, 24-bit products.
+__device__ __forceinline__ unsigned voxel_off(const PVol& v, int fx, int fy, int fz) {
+    const unsigned row = __umul24(static_cast<unsigned>(fz), v.plane_rows) + static_cast<unsigned>(fy);
+    return __umul24(row, v.pitchB) + static_cast<unsigned>(fx) * 4u;
 }
 
 // Sample of the 4-point line l[0..3] = f(clamp(c0 - 1 + q)) at the clipped
@@ -128,8 +129,8 @@ struct Grad {
 // X0..X3 at (Y1|Y2, Z1|Z2) and 8 x 8-byte rows holding X1, X2 for the y and
 // z lines (see eval).
 struct Taps {
-    f4u q11, q21, q12, q22;
-    f2u e01, e02, e31, e32, f10, f20, f13, f23;
+    f4v q11, q21, q12, q22;
+    f2v e01, e02, e31, e32, f10, f20, f13, f23;
 };
 
 struct Coord {
@@ -148,24 +149,26 @@ __device__ __forceinline__ Coord make_coord(const PVol& v, float cx, float cy, f
     return c;
 }
 
+// The 12 tap rows of the gather at c, as offsets from that of voxel
+// (X0, Y0, Z0) = (fx - 1, fy - 1, fz - 1), which the pads keep inside the
+// allocation.  Where the clamped reference would repeat an edge row or
+// plane, the pads hold it.
 __device__ __forceinline__ Taps gather(const PVol& v, const Coord& c) {
-    const Line4 y = line4(c.sy.f, v.ny, v.pitchB);
-    const Line4 z = line4(c.sz.f, v.nz, v.planeB);
-    const unsigned xo = static_cast<unsigned>(c.sx.f + (kPad - 1)) * 4u;   // X0 = fx - 1
-    const unsigned b1 = z.o1 + xo, b2 = z.o2 + xo, b0 = z.o0 + xo + 4u, b3 = z.o3 + xo + 4u;
+    const unsigned py = v.pitchB, pz = v.planeB;
+    const unsigned a = v.origin + voxel_off(v, c.sx.f - 1, c.sy.f, c.sz.f) - pz - py;
     Taps t;
-    t.q11 = ld<f4u>(v, b1 + y.o1);
-    t.q21 = ld<f4u>(v, b1 + y.o2);
-    t.q12 = ld<f4u>(v, b2 + y.o1);
-    t.q22 = ld<f4u>(v, b2 + y.o2);
-    t.e01 = ld<f2u>(v, b1 + y.o0 + 4u);
-    t.e02 = ld<f2u>(v, b2 + y.o0 + 4u);
-    t.e31 = ld<f2u>(v, b1 + y.o3 + 4u);
-    t.e32 = ld<f2u>(v, b2 + y.o3 + 4u);
-    t.f10 = ld<f2u>(v, b0 + y.o1);
-    t.f20 = ld<f2u>(v, b0 + y.o2);
-    t.f13 = ld<f2u>(v, b3 + y.o1);
-    t.f23 = ld<f2u>(v, b3 + y.o2);
+    t.q11 = ld4(v, a + pz + py);
+    t.q21 = ld4(v, a + pz + 2u * py);
+    t.q12 = ld4(v, a + 2u * pz + py);
+    t.q22 = ld4(v, a + 2u * pz + 2u * py);
+    t.e01 = ld2(v, a + pz + 4u);
+    t.e02 = ld2(v, a + 2u * pz + 4u);
+    t.e31 = ld2(v, a + pz + 3u * py + 4u);
+    t.e32 = ld2(v, a + 2u * pz + 3u * py + 4u);
+    t.f10 = ld2(v, a + py + 4u);
+    t.f20 = ld2(v, a + 2u * py + 4u);
+    t.f13 = ld2(v, a + 3u * pz + py + 4u);
+    t.f23 = ld2(v, a + 3u * pz + 2u * py + 4u);
     return t;
 }
 
@@ -213,16 +216,13 @@ __device__ __forceinline__ Grad eval(const PVol& v, const Coord& c, const Taps& 
 // Trilinear sample only (4 x 8-byte rows).
 __device__ __forceinline__ float tri(const PVol& v, float cx, float cy, float cz) {
     const Samp sx = split(cx, v.hx), sy = split(cy, v.hy), sz = split(cz, v.hz);
-    const unsigned y1 = static_cast<unsigned>(sy.f) * v.pitchB;
-    const unsigned y2 = sy.f < v.ny - 1 ? y1 + v.pitchB : y1;
-    const unsigned xo = static_cast<unsigned>(sx.f + kPad) * 4u;
-    const unsigned z1 = static_cast<unsigned>(sz.f) * v.planeB + xo;
-    const unsigned z2 = (sz.f < v.nz - 1 ? z1 + v.planeB : z1);
-    const f2u r00 = ld<f2u>(v, z1 + y1), r01 = ld<f2u>(v, z1 + y2);
-    const f2u r10 = ld<f2u>(v, z2 + y1), r11 = ld<f2u>(v, z2 + y2);
-    const float a = lerpf(lerpf(r00.x, r00.y, sx.t), lerpf(r01.x, r01.y, sx.t), sy.t);
-    const float c = lerpf(lerpf(r10.x, r10.y, sx.t), lerpf(r11.x, r11.y, sx.t), sy.t);
-    return lerpf(a, c, sz.t);
+    const unsigned a = v.origin + voxel_off(v, sx.f, sy.f, sz.f);
+    const unsigned py = v.pitchB, pz = v.planeB;
+    const f2v r00 = ld2(v, a), r01 = ld2(v, a + py);
+    const f2v r10 = ld2(v, a + pz), r11 = ld2(v, a + pz + py);
+    const float p = lerpf(lerpf(r00.x, r00.y, sx.t), lerpf(r01.x, r01.y, sx.t), sy.t);
+    const float q = lerpf(lerpf(r10.x, r10.y, sx.t), lerpf(r11.x, r11.y, sx.t), sy.t);
+    return lerpf(p, q, sz.t);
 }
 
 struct PassArgs {
@@ -360,7 +360,6 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
         }
         return make_coord(a.mov, xf + mx, static_cast<float>(yc) + my, zf + mz);
     };
-    auto voxel_of = [&](int s) { return z * plane + clampi(y0 - 1 + s, 0, a.ny - 1) * a.nx + xc; };
 
     float d0 = 0.0f, d1 = 0.0f;        // residual of the two previous rows
     float pgx = 0.0f, pgy = 0.0f, pgz = 0.0f;
@@ -373,14 +372,15 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
         for (int s = 0; s < nsteps; ++s) {
             const Coord cc = coord_of(s);
             const Taps tc = gather(a.mov, cc);
-            const int vc = voxel_of(s);
-            const float rc = a.ref[vc];
+            // template / mask row of this step: uniform base, per-lane column
+            const int vrow = z * plane + clampi(y0 - 1 + s, 0, a.ny - 1) * a.nx;
+            const float rc = (a.ref + vrow)[xc];
             const int yy = y0 - 1 + s;
             const Grad g = eval(a.mov, cc, tc);
             const float gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
             const float d = rc - g.w;
             bool mk = false;
-            if (MREF) mk = a.mref[vc] != 0;
+            if (MREF) mk = (a.mref + vrow)[xc] != 0;
             if (MMOV) mk = mk || tri(a.mmov, cc.cx, cc.cy, cc.cz) > 0.0f;
             if (s >= 2) {
                 const float vsum = d0 + d1 + d;
@@ -453,11 +453,14 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     }
 }
 
-PVol make_pvol(const float* base_x0, int pitch, int nx, int ny, int nz) {
+PVol make_pvol(const float* base, int nx, int ny, int nz) {
+    const OutLayout ol = padded_layout(nx, ny);
     PVol v;
-    v.buf = reinterpret_cast<const unsigned char*>(base_x0 - kPad);
-    v.pitchB = static_cast<unsigned>(pitch) * 4u;
-    v.planeB = static_cast<unsigned>(pitch) * static_cast<unsigned>(ny) * 4u;
+    v.base = base;
+    v.pitchB = static_cast<unsigned>(ol.pitch) * 4u;
+    v.plane_rows = static_cast<unsigned>(ol.plane_rows);
+    v.planeB = v.pitchB * v.plane_rows;
+    v.origin = static_cast<unsigned>((ol.lead * ol.pitch + kPad) * 4);
     v.nx = nx;
     v.ny = ny;
     v.nz = nz;
@@ -471,8 +474,8 @@ PVol make_pvol(const float* base_x0, int pitch, int nx, int ny, int nz) {
 
 long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s) {
     PassArgs a;
-    a.mov = make_pvol(b.mov, b.pitch, g.nx, g.ny, g.nz);
-    a.mmov = make_pvol(b.mmov ? b.mmov : b.mov, b.pitch, g.nx, g.ny, g.nz);
+    a.mov = make_pvol(b.mov, g.nx, g.ny, g.nz);
+    a.mmov = make_pvol(b.mmov ? b.mmov : b.mov, g.nx, g.ny, g.nz);
     a.ref = b.ref;
     a.mref = b.mref;
     a.m0 = b.m0;

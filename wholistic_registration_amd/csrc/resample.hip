@@ -184,9 +184,24 @@ __device__ __forceinline__ double load_d(const void* p, size_t i) {
     else return static_cast<const double*>(p)[i];
 }
 
+// Stored row of logical row r (see OutLayout).
+__device__ __forceinline__ size_t out_row(const OutLayout& ol, long long r) {
+    if (ol.plane_rows == 0) return static_cast<size_t>(r);
+    const long long z = r / ol.ny;
+    return static_cast<size_t>(z * ol.plane_rows + (r - z * ol.ny) + ol.lead);
+}
+
+// v / divisor: a multiply when the divisor is a power of two (its reciprocal
+// is exact, so the product equals the quotient), else the division.
+__device__ __forceinline__ double scaled(double v, double divisor) {
+    if (divisor == 1.0) return v;
+    if (__builtin_amdgcn_frexp_mant(divisor) == 0.5) return v * (1.0 / divisor);
+    return v / divisor;
+}
+
 __device__ __forceinline__ void store_out(void* out, size_t i, double v, int om, double divisor) {
     if (om == OUT_F64) static_cast<double*>(out)[i] = v;
-    else if (om == OUT_F32) static_cast<float*>(out)[i] = static_cast<float>(v / divisor);
+    else if (om == OUT_F32) static_cast<float*>(out)[i] = static_cast<float>(scaled(v, divisor));
     else static_cast<uint8_t*>(out)[i] = v > 0.0 ? 1 : 0;
 }
 
@@ -357,7 +372,7 @@ __global__ __launch_bounds__(256) void k_resample_x(const void* __restrict__ in,
     for (int i = threadIdx.x; i < (nr << lg); i += 256) {
         const int rr = i >> lg, c = i & (p.xo_blk - 1);
         if (c < nxo)
-            store_out(out, static_cast<size_t>(r0 + rr) * p.ol.pitch + p.ol.pad + xo0 + c, so[rr * po + c], om,
+            store_out(out, out_row(p.ol, r0 + rr) * p.ol.pitch + p.ol.pad + xo0 + c, so[rr * po + c], om,
                       divisor);
     }
 }
@@ -414,7 +429,78 @@ __global__ __launch_bounds__(256) void k_resample_xy(const void* __restrict__ in
         const int* iy = q.idx + static_cast<size_t>(yo) * q.k;
         double acc = wy[0] * xr[(iy[0] - ylo) * po + c];
         for (int t = 1; t < q.k; ++t) acc = acc + wy[t] * xr[(iy[t] - ylo) * po + c];
-        store_out(out, (static_cast<size_t>(z) * q.n_out + yo) * ol.pitch + ol.pad + xo0 + c, acc, om, divisor);
+        store_out(out, out_row(ol, static_cast<long long>(z) * q.n_out + yo) * ol.pitch + ol.pad + xo0 + c, acc, om,
+                  divisor);
+    }
+}
+
+// Upsample (both axes, x first, few taps per output): a thread owns one
+// output column x of a block of output rows of one z slice.  It forms the x
+// pass of each source row the block reads (at most kUpRows) into its own LDS
+// column, then applies the y taps of every output row from there.  Same
+// float64 arithmetic and order as the two separate passes; no intermediate in
+// HBM and no barrier (a thread reads back only what it wrote).
+constexpr int kUpRows = 16;
+constexpr int kUpTaps = 6;   // x taps handled (kernel instantiated for 2..kUpTaps)
+
+struct UpPass {
+    const double* wx;
+    const int* ix;
+    const double* wy;
+    const int* iy;
+    const int* span;    // per block of output rows: first and last source row
+    int nx_in, nx_out, kx;
+    int ny_in, ny_out, ky;
+    int rows, nblk, xblocks;
+};
+
+template <int IN_T, int KX>
+__global__ __launch_bounds__(256) void k_resample_up(const void* __restrict__ in, void* __restrict__ out, int om,
+                                                      double divisor, UpPass u, OutLayout ol) {
+    __shared__ double xr[kUpRows][256];
+    const unsigned b = blockIdx.x;
+    const int xb = static_cast<int>(b % static_cast<unsigned>(u.xblocks));
+    const unsigned rest = b / static_cast<unsigned>(u.xblocks);
+    const int yb = static_cast<int>(rest % static_cast<unsigned>(u.nblk));
+    const int z = static_cast<int>(rest / static_cast<unsigned>(u.nblk));
+    const int x = xb * 256 + static_cast<int>(threadIdx.x);
+    if (x >= u.nx_out) return;
+    const int ylo = u.span[2 * yb];
+    const int nr = u.span[2 * yb + 1] - ylo + 1;
+    double w[KX];
+    int ix[KX];
+#pragma unroll
+    for (int t = 0; t < KX; ++t) {
+        w[t] = u.wx[static_cast<size_t>(x) * KX + t];
+        ix[t] = u.ix[static_cast<size_t>(x) * KX + t];
+    }
+    // four source rows at a time, every load issued before any is used (row
+    // index clamped to the block's last source row)
+    for (int j0 = 0; j0 < nr; j0 += 4) {
+        double v[4][KX];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const size_t row = (static_cast<size_t>(z) * u.ny_in + ylo + min(j0 + jj, nr - 1)) * u.nx_in;
+#pragma unroll
+            for (int t = 0; t < KX; ++t) v[jj][t] = load_d<IN_T>(in, row + ix[t]);
+        }
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            double acc = w[0] * v[jj][0];
+#pragma unroll
+            for (int t = 1; t < KX; ++t) acc = acc + w[t] * v[jj][t];
+            if (j0 + jj < nr) xr[j0 + jj][threadIdx.x] = acc;
+        }
+    }
+    const int y0 = yb * u.rows;
+    const int y1 = min(y0 + u.rows, u.ny_out);
+    for (int yo = y0; yo < y1; ++yo) {
+        const double* wy = u.wy + static_cast<size_t>(yo) * u.ky;
+        const int* iy = u.iy + static_cast<size_t>(yo) * u.ky;
+        double acc = wy[0] * xr[iy[0] - ylo][threadIdx.x];
+        for (int t = 1; t < u.ky; ++t) acc = acc + wy[t] * xr[iy[t] - ylo][threadIdx.x];
+        store_out(out, out_row(ol, static_cast<long long>(z) * u.ny_out + yo) * ol.pitch + ol.pad + x, acc, om,
+                  divisor);
     }
 }
 
@@ -448,7 +534,7 @@ __device__ __forceinline__ void store_v(void* out, size_t i, const double (&v)[V
             typedef float FV __attribute__((ext_vector_type(V)));
             FV o;
 #pragma unroll
-            for (int q = 0; q < V; ++q) o[q] = static_cast<float>(v[q] / divisor);
+            for (int q = 0; q < V; ++q) o[q] = static_cast<float>(scaled(v[q], divisor));
             *reinterpret_cast<FV*>(static_cast<float*>(out) + i) = o;
         } else {
             typedef uint8_t BV __attribute__((ext_vector_type(V)));
@@ -495,7 +581,7 @@ __global__ __launch_bounds__(256) void k_resample_y(const void* __restrict__ in,
 #pragma unroll
             for (int q = 0; q < V; ++q) acc[q] = acc[q] + wr[t] * v[q];
         }
-        store_v<V>(out, (static_cast<size_t>(z) * n_out + yo) * ol.pitch + ol.pad + x, acc, om, divisor);
+        store_v<V>(out, out_row(ol, static_cast<long long>(z) * n_out + yo) * ol.pitch + ol.pad + x, acc, om, divisor);
     }
 }
 
@@ -510,17 +596,45 @@ __global__ __launch_bounds__(256) void k_convert(const void* __restrict__ in, in
     if (x >= nx) return;
     double v[V];
     load_v<IN_T, V>(in, static_cast<size_t>(row) * nx + x, in_stride, in_comp, v);
-    store_v<V>(out, static_cast<size_t>(row) * ol.pitch + ol.pad + x, v, om, divisor);
+    store_v<V>(out, out_row(ol, row) * ol.pitch + ol.pad + x, v, om, divisor);
 }
 
-__global__ __launch_bounds__(256) void k_pad_rows(float* __restrict__ buf, long long nrows, int nx, int pitch) {
+// Pads of a padded volume (padded_layout): x ends of the logical rows, then
+// (next launch) the y pad rows of every plane, then (next launch) the z pad
+// planes, each a copy of the nearest logical row or plane.
+__global__ __launch_bounds__(256) void k_pad_x(float* __restrict__ buf, OutLayout ol, long long nrows, int nx) {
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= nrows * 2 * kPad) return;
-    const long long row = i / (2 * kPad);
-    const int j = static_cast<int>(i - row * 2 * kPad);
-    float* r = buf + row * pitch;
-    if (j < kPad) r[j] = r[kPad];
-    else r[kPad + nx + (j - kPad)] = r[kPad + nx - 1];
+    const long long r = i / (2 * kPad);
+    const int j = static_cast<int>(i - r * 2 * kPad);
+    float* row = buf + out_row(ol, r) * ol.pitch;
+    if (j < kPad) row[j] = row[kPad];
+    else row[kPad + nx + (j - kPad)] = row[kPad + nx - 1];
+}
+
+__global__ __launch_bounds__(256) void k_pad_y(float* __restrict__ buf, OutLayout ol, int nz) {
+    // (plane, pad row, element): kPadLo + kPadHi pad rows of `pitch` floats
+    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    const int per_plane = (kPadLo + kPadHi) * ol.pitch;
+    if (i >= static_cast<long long>(nz) * per_plane) return;
+    const int z = static_cast<int>(i / per_plane);
+    const int k = static_cast<int>(i - static_cast<long long>(z) * per_plane);
+    const int pr = k / ol.pitch, e = k - pr * ol.pitch;
+    const size_t plane0 = static_cast<size_t>(z + kPadLo) * ol.plane_rows;    // stored row of pad row 0
+    const int dst = pr < kPadLo ? pr : kPadLo + ol.ny + (pr - kPadLo);
+    const int src = pr < kPadLo ? kPadLo : kPadLo + ol.ny - 1;
+    buf[(plane0 + dst) * ol.pitch + e] = buf[(plane0 + src) * ol.pitch + e];
+}
+
+__global__ __launch_bounds__(256) void k_pad_z(float* __restrict__ buf, OutLayout ol, int nz) {
+    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    const long long plane = static_cast<long long>(ol.plane_rows) * ol.pitch;
+    if (i >= (kPadLo + kPadHi) * plane) return;
+    const int pp = static_cast<int>(i / plane);
+    const long long e = i - pp * plane;
+    const int dst = pp < kPadLo ? pp : kPadLo + nz + (pp - kPadLo);
+    const int src = pp < kPadLo ? kPadLo : kPadLo + nz - 1;
+    buf[dst * plane + e] = buf[src * plane + e];
 }
 
 // ------------------------------------------------------------------ launchers
@@ -635,6 +749,100 @@ bool fused_ok(const Table& tx, const FusedY& fy, Elem in_t) {
     return fy.rows > 0 && !tx.identity && x_lds_per_row(in_t, tx) * 64 <= 65536;
 }
 
+FusedY plan_up_y(const Table& tx, const Table& ty, std::vector<int>& i_arena) {
+    FusedY f;
+    if (tx.identity || ty.identity || tx.k < 2 || tx.k > kUpTaps) return f;
+    const int* idx = &i_arena[ty.i_off];
+    auto block_span = [&](int b, int rows, int& lo, int& hi) {
+        lo = ty.n_in;
+        hi = 0;
+        for (int j = b * rows; j < std::min(ty.n_out, (b + 1) * rows); ++j)
+            for (int k = 0; k < ty.k; ++k) {
+                lo = std::min(lo, idx[static_cast<size_t>(j) * ty.k + k]);
+                hi = std::max(hi, idx[static_cast<size_t>(j) * ty.k + k]);
+            }
+    };
+    // the most output rows per block whose source rows fit kUpRows, and at
+    // least as many output rows as source rows (an upsample in y)
+    int rows = std::min(ty.n_out, 32);
+    for (; rows >= 1; --rows) {
+        int worst = 0;
+        for (int b = 0; b * rows < ty.n_out; ++b) {
+            int lo, hi;
+            block_span(b, rows, lo, hi);
+            worst = std::max(worst, hi - lo + 1);
+        }
+        if (worst <= kUpRows && worst <= rows) break;
+    }
+    if (rows < kFusedMinRows) return f;
+    f.rows = rows;
+    f.nblk = (ty.n_out + rows - 1) / rows;
+    f.s_off = i_arena.size();
+    for (int b = 0; b < f.nblk; ++b) {
+        int lo, hi;
+        block_span(b, rows, lo, hi);
+        i_arena.push_back(lo);
+        i_arena.push_back(hi);
+    }
+    return f;
+}
+
+namespace {
+
+template <int IN_T>
+void launch_up_k(unsigned blocks, const void* in, void* out, int om, double divisor, const UpPass& u, OutLayout ol,
+                 hipStream_t s) {
+    switch (u.kx) {
+        case 2:
+            k_resample_up<IN_T, 2><<<blocks, 256, 0, s>>>(in, out, om, divisor, u, ol);
+            break;
+        case 3:
+            k_resample_up<IN_T, 3><<<blocks, 256, 0, s>>>(in, out, om, divisor, u, ol);
+            break;
+        case 4:
+            k_resample_up<IN_T, 4><<<blocks, 256, 0, s>>>(in, out, om, divisor, u, ol);
+            break;
+        case 5:
+            k_resample_up<IN_T, 5><<<blocks, 256, 0, s>>>(in, out, om, divisor, u, ol);
+            break;
+        default:
+            k_resample_up<IN_T, 6><<<blocks, 256, 0, s>>>(in, out, om, divisor, u, ol);
+    }
+}
+
+}  // namespace
+
+void launch_resample_up(const void* in, Elem in_t, void* out, OutMode om, double divisor, const Table& tx,
+                        const Table& ty, const FusedY& fu, const double* w, const int* idx, int nz, OutLayout ol,
+                        hipStream_t s) {
+    UpPass u;
+    u.wx = w + tx.w_off;
+    u.ix = idx + tx.i_off;
+    u.wy = w + ty.w_off;
+    u.iy = idx + ty.i_off;
+    u.span = idx + fu.s_off;
+    u.nx_in = tx.n_in;
+    u.nx_out = tx.n_out;
+    u.kx = tx.k;
+    u.ny_in = ty.n_in;
+    u.ny_out = ty.n_out;
+    u.ky = ty.k;
+    u.rows = fu.rows;
+    u.nblk = fu.nblk;
+    u.xblocks = (tx.n_out + 255) / 256;
+    const unsigned blocks = static_cast<unsigned>(static_cast<long long>(u.xblocks) * fu.nblk * nz);
+    switch (in_t) {
+        case E_F32:
+            launch_up_k<E_F32>(blocks, in, out, om, divisor, u, ol, s);
+            break;
+        case E_U16:
+            launch_up_k<E_U16>(blocks, in, out, om, divisor, u, ol, s);
+            break;
+        default:
+            launch_up_k<E_F64>(blocks, in, out, om, divisor, u, ol, s);
+    }
+}
+
 void launch_resample_xy(const void* in, Elem in_t, void* out, OutMode om, double divisor, const Table& tx,
                         const Table& ty, const FusedY& fy, const double* w, const int* idx, int nz, OutLayout ol,
                         hipStream_t s) {
@@ -736,9 +944,15 @@ void launch_convert(const void* in, Elem in_t, int in_stride, int in_comp, void*
         launch_conv<1>(in, in_t, in_stride, in_comp, out, om, divisor, nx, nrows, ol, s);
 }
 
-void launch_pad_rows(float* buf, long long nrows, int nx, int pitch, hipStream_t s) {
-    const long long n = nrows * 2 * kPad;
-    k_pad_rows<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(buf, nrows, nx, pitch);
+void launch_pad_volume(float* buf, int nx, int ny, int nz, hipStream_t s) {
+    const OutLayout ol = padded_layout(nx, ny);
+    const long long nrows = static_cast<long long>(ny) * nz;
+    const long long nx_ = nrows * 2 * kPad;
+    k_pad_x<<<static_cast<unsigned>((nx_ + 255) / 256), 256, 0, s>>>(buf, ol, nrows, nx);
+    const long long ny_ = static_cast<long long>(nz) * (kPadLo + kPadHi) * ol.pitch;
+    k_pad_y<<<static_cast<unsigned>((ny_ + 255) / 256), 256, 0, s>>>(buf, ol, nz);
+    const long long nz_ = static_cast<long long>(kPadLo + kPadHi) * ol.plane_rows * ol.pitch;
+    k_pad_z<<<static_cast<unsigned>((nz_ + 255) / 256), 256, 0, s>>>(buf, ol, nz);
 }
 
 }  // namespace wr

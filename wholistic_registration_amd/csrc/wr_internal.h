@@ -69,15 +69,37 @@ constexpr int kXSpan = 191;
 constexpr int kXMaxCols = 32;
 enum OutMode { OUT_F64 = 0, OUT_F32 = 1, OUT_MASK = 2 };
 
-// Replicated voxels at each x end of a padded row (the moving level image).
+// The moving level image (and mask_mov) is stored padded so that every tap
+// of a clamped trilinear gather is a plain load: kPad replicated voxels at
+// both x ends of a row, kPadLo replicated rows / planes before the first and
+// kPadHi after the last in y and z.
 constexpr int kPad = 4;
+constexpr int kPadLo = 1;
+constexpr int kPadHi = 2;
 
-// Output placement of a resample: row `r`, column `x` goes to element
-// r * pitch + pad + x (pitch = row length, pad = 0 for a dense array).
+// Output placement of a resample: logical row r = z * ny + y goes to element
+// row_index(r) * pitch + pad + x.  row_index(r) = r for dense rows
+// (plane_rows = 0); for a volume padded in y and z it is
+// (r / ny) * plane_rows + r % ny + lead.
 struct OutLayout {
     int pitch;
     int pad;
+    int ny = 0;            // logical rows per plane (padded layout)
+    int plane_rows = 0;    // stored rows per plane; 0 = dense rows
+    long long lead = 0;    // stored rows before logical row 0
 };
+
+// Layout of a (nz, ny, nx) volume padded in x, y and z.
+inline OutLayout padded_layout(int nx, int ny) {
+    OutLayout o{nx + 2 * kPad, kPad};
+    o.ny = ny;
+    o.plane_rows = ny + kPadLo + kPadHi;
+    o.lead = static_cast<long long>(kPadLo) * o.plane_rows + kPadLo;
+    return o;
+}
+inline size_t padded_floats(int nx, int ny, int nz) {
+    return static_cast<size_t>(nx + 2 * kPad) * (ny + kPadLo + kPadHi) * (nz + kPadLo + kPadHi);
+}
 
 // Resample along x: in (nz, ny, t.n_in) with element stride `in_stride`
 // (3 for one component of an interleaved field) and component offset
@@ -100,6 +122,13 @@ bool fused_ok(const Table& tx, const FusedY& fy, Elem in_t);
 void launch_resample_xy(const void* in, Elem in_t, void* out, OutMode om, double divisor, const Table& tx,
                         const Table& ty, const FusedY& fy, const double* w, const int* idx, int nz, OutLayout ol,
                         hipStream_t s);
+// Direct x-then-y resample for few-tap x tables (upsampling): output rows in
+// blocks of `rows` reading at most 16 source rows each (rows = 0: not
+// possible); same arena layout as FusedY.
+FusedY plan_up_y(const Table& tx, const Table& ty, std::vector<int>& i_arena);
+void launch_resample_up(const void* in, Elem in_t, void* out, OutMode om, double divisor, const Table& tx,
+                        const Table& ty, const FusedY& fu, const double* w, const int* idx, int nz, OutLayout ol,
+                        hipStream_t s);
 // Resample along y: in (nz, t.n_in, nx) -> out (nz, t.n_out, nx).
 void launch_resample_y(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                        OutMode om, double divisor, const Table& t, const double* w,
@@ -107,8 +136,9 @@ void launch_resample_y(const void* in, Elem in_t, int in_stride, int in_comp, vo
 // Elementwise conversion (identity resample) of nrows rows of nx: out = convert(in).
 void launch_convert(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                     OutMode om, double divisor, int nx, long long nrows, OutLayout ol, hipStream_t s);
-// Fill the kPad replicated voxels at both ends of every padded float row.
-void launch_pad_rows(float* buf, long long nrows, int nx, int pitch, hipStream_t s);
+// Fill the pads of a padded float volume (padded_layout) whose logical rows
+// are written: x ends, then the y pad rows, then the z pad planes.
+void launch_pad_volume(float* buf, int nx, int ny, int nz, hipStream_t s);
 
 // ---------------------------------------------------------------- LK solver
 
@@ -153,11 +183,10 @@ struct LkTile {
 LkTile plan_tile(const LevelGeom& g, int np_x, int nb_y);
 
 struct LkBuffers {
-    int pitch;               // row pitch of the padded mov / mmov images
-    const float* mov;        // level moving image, padded rows, x = 0 of row 0
+    const float* mov;        // level moving image, padded volume (padded_layout), allocation start
     const float* ref;        // level template
     const uint8_t* mref;     // level mask_ref (> 0 after resampling) or null
-    const float* mmov;       // level mask_mov (continuous, padded like mov) or null
+    const float* mmov;       // level mask_mov (continuous, padded like mov, allocation start) or null
     const float* m0;         // dense field at level start, 3 planes, or null
     float* cp;               // control-point field, 3 planes of ncx*ncy*nz
     float* dnei;             // neighbour differences, 3 planes
