@@ -63,7 +63,7 @@ void resample2(const void* in, Elem in_t, int stride, int comp, void* out, OutMo
         launch_resample_y(in, in_t, stride, comp, out, om, div, ty, w, idx, tx.n_in, nz, ol, s);
     } else if (pl.x_first && stride == 1 && comp == 0 && pl.fu.rows > 0) {
         launch_resample_up(in, in_t, out, om, div, tx, ty, pl.fu, w, idx, nz, ol, s);
-    } else if (pl.x_first && stride == 1 && comp == 0 && fused_ok(tx, pl.fy, in_t)) {
+    } else if (pl.x_first && stride == 1 && comp == 0 && fused_ok(tx, ty, pl.fy, in_t)) {
         launch_resample_xy(in, in_t, out, om, div, tx, ty, pl.fy, w, idx, nz, ol, s);
     } else if (pl.x_first) {
         launch_resample_x(in, in_t, stride, comp, tmp, OUT_F64, 1.0, tx, w, idx, ty.n_in, nz, OutLayout{tx.n_out, 0}, s);

@@ -392,7 +392,8 @@ struct YFused {
 
 template <int IN_T>
 __global__ __launch_bounds__(256) void k_resample_xy(const void* __restrict__ in, void* __restrict__ out, int om,
-                                                      double divisor, XPass p, YFused q, OutLayout ol) {
+                                                      double divisor, XPass p, YFused q, OutLayout ol,
+                                                      unsigned ytab_off) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const unsigned b = blockIdx.x;
     const int cb = static_cast<int>(b % static_cast<unsigned>(p.n_cb));
@@ -406,6 +407,16 @@ __global__ __launch_bounds__(256) void k_resample_xy(const void* __restrict__ in
     const int nxo = min(p.xo_blk, p.n_out - xo0);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
     const int lane = threadIdx.x & 63;
+    // the y taps of the block's output rows, staged in LDS (the two output
+    // rows of a wave read them as broadcasts); visible after x_group's barrier
+    const int yo0 = yb * q.rows;
+    const int yo1 = min(yo0 + q.rows, q.n_out);
+    double* swy = reinterpret_cast<double*>(smem_raw + ytab_off);
+    int* siy = reinterpret_cast<int*>(swy + q.rows * q.k);
+    for (int i = threadIdx.x; i < (yo1 - yo0) * q.k; i += 256) {
+        swy[i] = q.w[static_cast<size_t>(yo0) * q.k + i];
+        siy[i] = q.idx[static_cast<size_t>(yo0) * q.k + i] - ylo;
+    }
     double res[kXColsPerWave];
     x_group<IN_T>(in, 1, 0, p, r0, nr, cb, smem_raw, wave, lane, res);
     // x results of the block's source rows: xr[row][column], odd pitch
@@ -422,13 +433,11 @@ __global__ __launch_bounds__(256) void k_resample_xy(const void* __restrict__ in
     // y taps: lanes along the columns, 8 output rows at a time
     const int c = threadIdx.x & (kXMaxCols - 1);
     if (c >= nxo) return;
-    const int yo0 = yb * q.rows;
-    const int yo1 = min(yo0 + q.rows, q.n_out);
     for (int yo = yo0 + static_cast<int>(threadIdx.x) / kXMaxCols; yo < yo1; yo += 256 / kXMaxCols) {
-        const double* wy = q.w + static_cast<size_t>(yo) * q.k;
-        const int* iy = q.idx + static_cast<size_t>(yo) * q.k;
-        double acc = wy[0] * xr[(iy[0] - ylo) * po + c];
-        for (int t = 1; t < q.k; ++t) acc = acc + wy[t] * xr[(iy[t] - ylo) * po + c];
+        const double* wy = swy + (yo - yo0) * q.k;
+        const int* iy = siy + (yo - yo0) * q.k;
+        double acc = wy[0] * xr[iy[0] * po + c];
+        for (int t = 1; t < q.k; ++t) acc = acc + wy[t] * xr[iy[t] * po + c];
         store_out(out, out_row(ol, static_cast<long long>(z) * q.n_out + yo) * ol.pitch + ol.pad + xo0 + c, acc, om,
                   divisor);
     }
@@ -744,9 +753,11 @@ FusedY plan_fused_y(const Table& ty, std::vector<int>& i_arena) {
 }
 
 
-bool fused_ok(const Table& tx, const FusedY& fy, Elem in_t) {
-    // 64 staged source rows must fit the 64 KiB of LDS
-    return fy.rows > 0 && !tx.identity && x_lds_per_row(in_t, tx) * 64 <= 65536;
+bool fused_ok(const Table& tx, const Table& ty, const FusedY& fy, Elem in_t) {
+    // 64 staged source rows and the y taps of a block must fit 64 KiB of LDS
+    const size_t rows = (x_lds_per_row(in_t, tx) * 64 + 15) & ~static_cast<size_t>(15);
+    return fy.rows > 0 && !tx.identity &&
+           rows + static_cast<size_t>(fy.rows) * ty.k * (sizeof(double) + sizeof(int)) <= 65536;
 }
 
 FusedY plan_up_y(const Table& tx, const Table& ty, std::vector<int>& i_arena) {
@@ -857,17 +868,19 @@ void launch_resample_xy(const void* in, Elem in_t, void* out, OutMode om, double
     q.k = ty.k;
     q.rows = fy.rows;
     q.nblk = fy.nblk;
-    const size_t lds = per_row * 64;
+    // 64 staged rows, then the y taps of one block of output rows
+    const unsigned ytab_off = static_cast<unsigned>((per_row * 64 + 15) & ~static_cast<size_t>(15));
+    const size_t lds = ytab_off + static_cast<size_t>(fy.rows) * ty.k * (sizeof(double) + sizeof(int));
     const unsigned blocks = static_cast<unsigned>(static_cast<long long>(tx.n_cb) * fy.nblk * nz);
     switch (in_t) {
         case E_F32:
-            k_resample_xy<E_F32><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol);
+            k_resample_xy<E_F32><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol, ytab_off);
             break;
         case E_U16:
-            k_resample_xy<E_U16><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol);
+            k_resample_xy<E_U16><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol, ytab_off);
             break;
         default:
-            k_resample_xy<E_F64><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol);
+            k_resample_xy<E_F64><<<blocks, 256, lds, s>>>(in, out, om, divisor, p, q, ol, ytab_off);
     }
 }
 

@@ -117,7 +117,7 @@ struct FusedY {
 };
 FusedY plan_fused_y(const Table& ty, std::vector<int>& i_arena);
 // The fused pass applies: both tables non-identity, x first, dense input.
-bool fused_ok(const Table& tx, const FusedY& fy, Elem in_t);
+bool fused_ok(const Table& tx, const Table& ty, const FusedY& fy, Elem in_t);
 // in (nz, ty.n_in, tx.n_in) dense -> out (nz, ty.n_out, tx.n_out) with layout ol.
 void launch_resample_xy(const void* in, Elem in_t, void* out, OutMode om, double divisor, const Table& tx,
                         const Table& ty, const FusedY& fy, const double* w, const int* idx, int nz, OutLayout ol,
