@@ -180,6 +180,8 @@ void wr_destroy(wr_ctx* ctx) {
                       &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c.lvl_events) (void)hipEventDestroy(e);
+    if (c.side) (void)hipStreamDestroy(c.side);
     delete ctx;
 }
 
@@ -338,7 +340,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     // ---- workspace
     const size_t n0 = static_cast<size_t>(nx) * ny * nz;
     WR_TRY(upload_tables(c, wa, ia));
-    WR_TRY(grow(c.tmp, tmp_need));
+    WR_TRY(grow(c.tmp, tmp_need));    // intermediate of the side stream (level images)
+    WR_TRY(grow(c.tmp2, tmp_need));   // intermediate of the main stream (warm start, upsample)
     // the moving image (and mask_mov) of every level is stored padded in x, y
     // and z (padded_layout); the template is dense.  The LK gathers address
     // it with 32-bit byte offsets and 24-bit row products.
@@ -346,10 +349,26 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     WR_ARG(4.0 * padded0 < 4294967295.0, "wr_get_motion: padded moving volume above 4 GiB");
     WR_ARG(static_cast<double>(nz + kPadLo + kPadHi) * (ny + kPadLo + kPadHi) < 16777216.0,
            "wr_get_motion: more than 2^24 padded rows");
-    WR_TRY(grow(c.mov_l, padded0 * sizeof(float)));
-    if (layer > 0 || dtype != WR_F32) WR_TRY(grow(c.ref_l, n0 * sizeof(float)));
-    if (mask_ref) WR_TRY(grow(c.mref_l, n0));
-    if (mask_mov) WR_TRY(grow(c.mmov_l, padded0 * sizeof(float)));
+    // every level keeps its own images: they are built on the context's side
+    // stream, ahead of the iterations that consume them
+    std::vector<size_t> mov_off(layer + 1), ref_off(layer + 1);
+    size_t mov_tot = 0, ref_tot = 0;
+    for (int L = layer; L >= 0; --L) {
+        mov_off[L] = mov_tot;
+        mov_tot += (padded_floats(geo[L].nx, geo[L].ny, nz) + 63) & ~static_cast<size_t>(63);
+        ref_off[L] = ref_tot;
+        ref_tot += (static_cast<size_t>(geo[L].nx) * geo[L].ny * nz + 255) & ~static_cast<size_t>(255);
+    }
+    WR_TRY(grow(c.mov_l, mov_tot * sizeof(float)));
+    WR_TRY(grow(c.ref_l, ref_tot * sizeof(float)));
+    if (mask_ref) WR_TRY(grow(c.mref_l, ref_tot));
+    if (mask_mov) WR_TRY(grow(c.mmov_l, mov_tot * sizeof(float)));
+    if (!c.side) WR_HIP_CHECK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    while (c.lvl_events.size() < static_cast<size_t>(layer + 2)) {
+        hipEvent_t e;
+        WR_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c.lvl_events.push_back(e);
+    }
     WR_TRY(grow(c.m0, 3 * n0 * sizeof(float)));
     if (layer > 0) WR_TRY(grow(c.mprev, 3 * nvox_prev * sizeof(float)));
     WR_TRY(grow(c.cp, 3 * ncp_max * sizeof(float)));
@@ -376,39 +395,56 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     const double* tw = c.tab_w.as<double>();
     const int* ti = c.tab_i.as<int>();
     double* tmp = c.tmp.as<double>();
+    double* tmp2 = c.tmp2.as<double>();
     float* m0 = c.m0.as<float>();
     float* mprev = c.mprev.as<float>();
     std::memset(&c.stats, 0, sizeof(c.stats));
+
+    // ---- level images and masks of every level, on the side stream (coarsest
+    // first); level L's iterations wait for event L.  The side stream's work
+    // is enqueued two levels ahead of the iterations that need it, so neither
+    // stream waits for the host.
+    hipStream_t side = c.side;
+    WR_HIP_CHECK(hipEventRecord(c.lvl_events[layer + 1], s));
+    WR_HIP_CHECK(hipStreamWaitEvent(side, c.lvl_events[layer + 1], 0));
+    std::vector<const float*> ref_lv(layer + 1);
+    auto build_level = [&](int L) -> int {
+        const LevelGeom& g = geo[L];
+        const OutLayout padded = padded_layout(g.nx, g.ny);
+        float* mov_l = c.mov_l.as<float>() + mov_off[L];
+        resample2(mov, et, 1, 0, mov_l, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
+        launch_pad_volume(mov_l, g.nx, g.ny, nz, side);
+        if (L == 0 && dtype == WR_F32) {
+            ref_lv[L] = static_cast<const float*>(ref);
+        } else {
+            float* ref_l = c.ref_l.as<float>() + ref_off[L];
+            resample2(ref, et, 1, 0, ref_l, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), side);
+            ref_lv[L] = ref_l;
+        }
+        if (mask_ref)
+            resample2(mask_ref, E_F32, 1, 0, c.mref_l.as<uint8_t>() + ref_off[L], OUT_MASK, 1.0, down[L], tw, ti, nz,
+                      tmp, dense(g.nx), side);
+        if (mask_mov) {
+            float* mm = c.mmov_l.as<float>() + mov_off[L];
+            resample2(mask_mov, E_F32, 1, 0, mm, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
+            launch_pad_volume(mm, g.nx, g.ny, nz, side);
+        }
+        WR_HIP_CHECK(hipEventRecord(c.lvl_events[L], side));
+        return WR_OK;
+    };
+    for (int L = layer; L >= std::max(0, layer - 1); --L) WR_TRY(build_level(L));
 
     for (int L = layer; L >= 0; --L) {
         const LevelGeom& g = geo[L];
         const LkTile& t = tiles[L];
         const size_t nv = static_cast<size_t>(g.nx) * g.ny * nz;
 
-        // level images (float32) and masks
-        const OutLayout padded = padded_layout(g.nx, g.ny);
-        resample2(mov, et, 1, 0, c.mov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, s);
-        launch_pad_volume(c.mov_l.as<float>(), g.nx, g.ny, nz, s);
-        const float* ref_l;
-        if (L == 0 && dtype == WR_F32) {
-            ref_l = static_cast<const float*>(ref);
-        } else {
-            resample2(ref, et, 1, 0, c.ref_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), s);
-            ref_l = c.ref_l.as<float>();
-        }
-        if (mask_ref)
-            resample2(mask_ref, E_F32, 1, 0, c.mref_l.p, OUT_MASK, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), s);
-        if (mask_mov) {
-            resample2(mask_mov, E_F32, 1, 0, c.mmov_l.p, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, s);
-            launch_pad_volume(c.mmov_l.as<float>(), g.nx, g.ny, nz, s);
-        }
-
         // initial dense field of the level, 3 planes
         if (L == layer) {
             if (motion_init) {
                 const double div[3] = {1.0 * nx / g.nx, 1.0 * ny / g.ny, 1.0 * nz / g.nz};
                 for (int q = 0; q < 3; ++q)
-                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q], down[L], tw, ti, nz, tmp,
+                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q], down[L], tw, ti, nz, tmp2,
                               dense(g.nx), s);
             } else {
                 fill(m0, 3 * nv, 0.0f, s);
@@ -416,7 +452,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         } else {
             const size_t np = static_cast<size_t>(geo[L + 1].nx) * geo[L + 1].ny * nz;
             for (int q = 0; q < 3; ++q)
-                resample2(mprev + q * np, E_F32, 1, 0, m0 + q * nv, OUT_F32, q < 2 ? 0.5 : 1.0, up[L], tw, ti, nz, tmp,
+                resample2(mprev + q * np, E_F32, 1, 0, m0 + q * nv, OUT_F32, q < 2 ? 0.5 : 1.0, up[L], tw, ti, nz, tmp2,
                           dense(g.nx), s);
         }
         launch_cp_init(g, m0, c.cp.as<float>(), s);
@@ -425,11 +461,13 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         launch_ctl_reset(c.ctl.as<Ctl>(), iter, frec, fupd, prm->tol, prm->smoothPenalty * p * p,
                          static_cast<double>(nv), s);
 
+        if (L - 2 >= 0) WR_TRY(build_level(L - 2));
+        WR_HIP_CHECK(hipStreamWaitEvent(s, c.lvl_events[L], 0));
         LkBuffers b;
-        b.mov = c.mov_l.as<float>();
-        b.ref = ref_l;
-        b.mref = mask_ref ? c.mref_l.as<uint8_t>() : nullptr;
-        b.mmov = mask_mov ? c.mmov_l.as<float>() : nullptr;
+        b.mov = c.mov_l.as<float>() + mov_off[L];
+        b.ref = ref_lv[L];
+        b.mref = mask_ref ? c.mref_l.as<uint8_t>() + ref_off[L] : nullptr;
+        b.mmov = mask_mov ? c.mmov_l.as<float>() + mov_off[L] : nullptr;
         b.m0 = m0;
         b.cp = c.cp.as<float>();
         b.dnei = c.dnei.as<float>();

@@ -235,6 +235,8 @@ struct Context {
     DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts, mdpart;
     DevBuf tab_w, tab_i, pen;
     std::vector<hipEvent_t> events;
+    hipStream_t side = nullptr;          // pyramid builds, ahead of the iterations
+    std::vector<hipEvent_t> lvl_events;  // [L]: level L's images are ready; [layer+1]: fork
     wr_stats stats{};
 };
 
