@@ -67,7 +67,8 @@ typedef struct wr_params {
     const int* forced_records;
     const int* forced_updates;
     int tile_patches_x;     /* 0 = autotuned; patches per workgroup along x   */
-    int tile_bands_y;       /* 0 = autotuned; patch rows per workgroup         */
+    int stop_kernels;       /* 0 = automatic; 1 = the stop rules always run in
+                               their own single-workgroup kernels (testing)   */
 } wr_params;
 
 /* Per-level iteration log, filled on the host when wr_get_motion returns

@@ -83,7 +83,7 @@ def test_correct_motion_identity_is_exact(cuda, rng):
     np.testing.assert_array_equal(got, vol)
 
 
-def _run_gpu(mov, ref, opt, cuda, schedule=None):
+def _run_gpu(mov, ref, opt, cuda, schedule=None, stop_kernels=False):
     def mask(key):
         m = np.asarray(opt[key], dtype=np.float32)
         return _to_zyx(m, cuda) if np.any(m) else None
@@ -97,7 +97,7 @@ def _run_gpu(mov, ref, opt, cuda, schedule=None):
                             iter=opt["iter"], r=opt["r"], zRatio=opt["zRatio"],
                             movRange=opt["movRange"], smoothPenalty=opt["smoothPenalty"],
                             tol=opt["tol"], mask_ref=mask("mask_ref"), mask_mov=mask("mask_mov"),
-                            motion_init=init_t, schedule=schedule)
+                            motion_init=init_t, schedule=schedule, stop_kernels=stop_kernels)
     return res, res.motion.cpu().numpy().transpose(2, 1, 0, 3)
 
 
@@ -118,23 +118,26 @@ def test_get_motion_forced_schedule_matches_oracle(cuda, name):
     assert res.final_error == pytest.approx(err, rel=2e-5)
 
 
+@pytest.mark.parametrize("stop_kernels", [False, True])
 @pytest.mark.parametrize("name", sorted(SMALL_CASES))
-def test_get_motion_free_running_matches_reference(cuda, name):
+def test_get_motion_free_running_matches_reference(cuda, name, stop_kernels):
     """Stop rules evaluated on the device reproduce the reference's schedule
-    and field (golden vectors of the reference run)."""
+    and field (golden vectors of the reference run), whether they run inside
+    the launches that follow them or in their own kernels."""
     g = np.load(os.path.join(GOLDEN, f"getmotion_{name}.npz"))
     mov, ref, opt = small_inputs(name)
-    res, got = _run_gpu(mov, ref, opt, cuda)
+    res, got = _run_gpu(mov, ref, opt, cuda, stop_kernels=stop_kernels)
     for L in range(opt["layer"], -1, -1):
         assert res.records[L] == len(g[f"L{L}_currentError"]), L
         assert res.updates[L] == len(g[f"L{L}_max_diff_motion"]), L
     assert np.abs(got - g["motion"]).max() <= FIELD_TOL
 
 
-def test_get_motion_config0_matches_reference(cuda):
+@pytest.mark.parametrize("stop_kernels", [False, True])
+def test_get_motion_config0_matches_reference(cuda, stop_kernels):
     g = np.load(os.path.join(GOLDEN, "getmotion_config0.npz"))
     mov, ref, opt = config0_inputs()
-    res, got = _run_gpu(mov, ref, opt, cuda)
+    res, got = _run_gpu(mov, ref, opt, cuda, stop_kernels=stop_kernels)
     for L in range(CONFIG0["layer"], -1, -1):
         assert res.records[L] == len(g[f"L{L}_currentError"]), L
     r = CONFIG0["r"]

@@ -33,7 +33,7 @@ class WrParams(ctypes.Structure):
         ("forced_records", ctypes.POINTER(ctypes.c_int)),
         ("forced_updates", ctypes.POINTER(ctypes.c_int)),
         ("tile_patches_x", ctypes.c_int),
-        ("tile_bands_y", ctypes.c_int),
+        ("stop_kernels", ctypes.c_int),
     ]
 
 

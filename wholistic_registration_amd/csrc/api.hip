@@ -321,7 +321,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         g.z_ratio_d = prm->zRatio / std::ldexp(1.0, L);
         g.z_ratio = static_cast<float>(g.z_ratio_d);
         g.mov_range = prm->movRange;
-        tiles[L] = plan_tile(g, prm->tile_patches_x, prm->tile_bands_y);
+        tiles[L] = plan_tile(g, prm->tile_patches_x);
         down[L] = make_plan(nx, g.nx, ny, g.ny, KERNEL_CUBIC, wa, ia);
         tmp_need = std::max(tmp_need, tmp_bytes(down[L], nz));
         if (L < layer) {
@@ -334,8 +334,16 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         nblk_max = std::max(nblk_max, static_cast<size_t>(tiles[L].blocks));
         ncpb_max = std::max(ncpb_max, (ncp + 63) / 64);   // neighbour-difference workgroups of >= 64 threads
     }
-    if (prm->forced_records || prm->forced_updates)
+    if (prm->forced_records || prm->forced_updates) {
         WR_ARG(prm->forced_records && prm->forced_updates, "wr_get_motion: forced schedule needs both arrays");
+        // a schedule the reference can produce: every record but possibly the
+        // last is followed by an update, and the last iteration has none
+        for (int L = 0; L <= layer; ++L) {
+            const int fr = prm->forced_records[L], fu = prm->forced_updates[L];
+            WR_ARG(fr >= 1 && fr <= iter && (fu == fr - 1 || (fu == fr && fr < iter)),
+                   "wr_get_motion: forced schedule of level " + std::to_string(L) + " is not a reachable schedule");
+        }
+    }
 
     // ---- workspace
     const size_t n0 = static_cast<size_t>(nx) * ny * nz;
@@ -479,7 +487,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         b.counts = c.counts.as<int>() + L * 3;
         b.mdpart = c.mdpart.as<float>();
         b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
-        for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, L, k, iter, s));
+        for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, k, prm->stop_kernels != 0, s));
         c.stats.lk_launches[L] = iter;
         c.stats.lk_blocks[L] = static_cast<int>(t.blocks);
         if (L == 0) {

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_ctl_reset(Ctl* c, int iter, int frec, i
     c->frec = frec;
     c->fupd = fupd;
     c->maxdiff = 0u;
-    c->old_err[0] = c->old_err[1] = c->old_err[2] = INFINITY;
+    for (int q = 0; q < 3; ++q) c->old_err[0][q] = c->old_err[1][q] = INFINITY;
     c->tol = tol;
     c->s_pen = s_pen;
     c->inv_n = n_vox;   // holds N; the error terms divide by it
@@ -80,11 +80,30 @@ __device__ __forceinline__ void block_max_to(float md, float* __restrict__ dst) 
 }
 
 // Regularised 3x3 solve per CP and the CP update (float64 solve).
+// With fuse_err, every workgroup first evaluates iteration k's error terms and
+// error-based stop rule from the LK launch's partials (the same result in
+// all of them; workgroup 0 records it) instead of a separate k_err_finish.
 __global__ __launch_bounds__(256) void k_solve(Ctl* __restrict__ c, const float* __restrict__ sums,
                                                 const float* __restrict__ dn, float* __restrict__ cp,
                                                 long long ncp, double mov_range, double z_ratio,
-                                                float* __restrict__ mdpart) {
+                                                float* __restrict__ mdpart, int k, int fuse_err,
+                                                const double* __restrict__ errpart, long long nerr,
+                                                const double* __restrict__ penpart, long long npen,
+                                                double* __restrict__ log, int* __restrict__ counts) {
     if (c->done) return;
+    if (fuse_err) {
+        __shared__ int s_stop;
+        double diff, pen;
+        err_terms(c, errpart, nerr, penpart, npen, diff, pen);
+        if (threadIdx.x == 0) {
+            const double cur = diff + pen;
+            const bool stop = err_stop(c, k, cur);
+            if (blockIdx.x == 0) err_record(c, k, diff, pen, cur, stop, log, counts);
+            s_stop = stop ? 1 : 0;
+        }
+        __syncthreads();
+        if (s_stop) return;
+    }
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     float md = 0.0f;
     if (i < ncp) {
@@ -166,21 +185,21 @@ __global__ __launch_bounds__(256) void k_dense_maxdiff(const Ctl* __restrict__ c
 }
 
 // Error terms, log and the error-based stop rule of iteration k.
-__global__ __launch_bounds__(1024) void k_err_finish(Ctl* __restrict__ c, const double* __restrict__ errpart,
+__global__ __launch_bounds__(1024) void k_err_finish(Ctl* __restrict__ c, int k, const double* __restrict__ errpart,
                                                       long long nerr, const double* __restrict__ penpart,
                                                       long long npen, double* __restrict__ log,
                                                       int* __restrict__ counts) {
     if (c->done) return;
-    err_finish_block(c, errpart, nerr, penpart, npen, log, counts);
+    err_finish_block(c, k, errpart, nerr, penpart, npen, log, counts);
 }
 
 // The |delta motion| < 1e-3 stop test of the update just made; `nparts`
 // per-workgroup maxima from k_solve (and k_dense_maxdiff on a level's first
 // update) are reduced here.
-__global__ __launch_bounds__(1024) void k_check(Ctl* __restrict__ c, const float* __restrict__ mdpart, int nparts,
-                                                 double* __restrict__ log, int* __restrict__ counts) {
+__global__ __launch_bounds__(1024) void k_check(Ctl* __restrict__ c, int k, const float* __restrict__ mdpart,
+                                                 int nparts, double* __restrict__ log, int* __restrict__ counts) {
     if (c->done) return;
-    check_finish_block(c, mdpart, nparts, log, counts);
+    check_finish_block(c, k, mdpart, nparts, log, counts);
 }
 
 // Dense field at the end of a level: the CP interpolant when the level made
@@ -238,7 +257,7 @@ inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((
 
 size_t mdpart_slots(long long ncp) { return static_cast<size_t>((ncp + 255) / 256) + kDenseBlocks; }
 
-LkTile plan_tile(const LevelGeom& g, int np_x, int nb_y) {
+LkTile plan_tile(const LevelGeom& g, int np_x) {
     LkTile t{};
     // a workgroup has at most 4 waves of 62 owned columns each
     const int maxp = std::max(1, kMaxTileCols / g.p);
@@ -258,9 +277,7 @@ LkTile plan_tile(const LevelGeom& g, int np_x, int nb_y) {
             }
         }
     }
-    (void)nb_y;
     t.np_x = npx;
-    t.nb_y = 1;
     t.cols = npx * g.p;
     t.rows = g.p;
     t.nbx = (g.nx + t.cols - 1) / t.cols;
@@ -279,26 +296,34 @@ void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, hipStream_t 
     k_cp_init<<<blocks_for(ncp, 256), 256, 0, s>>>(m0, cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy);
 }
 
-int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int level, int k, int iter,
+int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool separate_stop,
                      hipStream_t s) {
-    (void)level;
-    (void)iter;
     const long long ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
     const unsigned cpb = blocks_for(ncp, 256);
-    // the LK launch also computes the neighbour differences (appended
-    // workgroups, one per `threads` CPs)
+    // The LK launch also computes the neighbour differences (appended
+    // workgroups).  On levels where every workgroup can afford to read all
+    // partials, the stop tests run inside the launches that follow them: the
+    // error rule in the update launch, the |delta motion| test of update k - 1
+    // in the LK launch of iteration k; elsewhere in single-workgroup kernels.
+    const long long nerr = t.blocks;
+    const long long npen = (ncp + 63) / 64;                  // upper bound of the LK launch's |D|^2 partials
+    const long long parts_max = static_cast<long long>(cpb) + kDenseBlocks;
+    const bool fuse_err = !separate_stop && static_cast<long long>(cpb) * (nerr + npen) <= (1LL << 22);
+    const bool fuse_check = !separate_stop && (t.blocks + npen) * parts_max <= (1LL << 22);
+    const int prev_parts = static_cast<int>(k == 1 ? cpb + kDenseBlocks : cpb);
     if (b.events) (void)hipEventRecord(b.events[2 * k], s);
-    const long long npen = launch_lk_pass(g, t, b, k, s);
+    const long long np = launch_lk_pass(g, t, b, k, (fuse_check && k > 0) ? prev_parts : 0, s);
     if (b.events) (void)hipEventRecord(b.events[2 * k + 1], s);
-    k_err_finish<<<1, 1024, 0, s>>>(b.ctl, b.errpart, t.blocks, b.penpart, npen, b.log, b.counts);
-    k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d, b.mdpart);
+    if (!fuse_err) k_err_finish<<<1, 1024, 0, s>>>(b.ctl, k, b.errpart, nerr, b.penpart, np, b.log, b.counts);
+    k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d, b.mdpart, k,
+                                fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts);
     int nparts = static_cast<int>(cpb);
     if (k == 0) {
         k_dense_maxdiff<<<kDenseBlocks, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
                                                      b.mdpart + cpb);
         nparts += kDenseBlocks;
     }
-    k_check<<<1, 1024, 0, s>>>(b.ctl, b.mdpart, nparts, b.log, b.counts);
+    if (!fuse_check) k_check<<<1, 1024, 0, s>>>(b.ctl, k, b.mdpart, nparts, b.log, b.counts);
     return WR_OK;
 }
 

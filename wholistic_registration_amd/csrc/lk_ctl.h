@@ -64,21 +64,22 @@ __device__ __forceinline__ void block_sum2(double& a, double& b) {
     }
 }
 
-// Error terms of iteration k, their log entries and the error-based stop
-// rule (calFlow3d_Wei_v1.py:101-131,426,502-530), run by one workgroup.
-__device__ __forceinline__ void err_finish_block(Ctl* __restrict__ c, const double* __restrict__ errpart,
-                                                 long long nerr, const double* __restrict__ penpart, long long npen,
-                                                 double* __restrict__ log, int* __restrict__ counts) {
-    double e = 0.0, pp = 0.0;
-    for (long long i = threadIdx.x; i < nerr; i += blockDim.x) e += errpart[i];
-    for (long long i = threadIdx.x; i < npen; i += blockDim.x) pp += penpart[i];
-    block_sum2(e, pp);
-    if (threadIdx.x != 0) return;
-    const double nvox = c->inv_n;
-    const double diff = e / nvox;
-    const double pen = (pp * c->s_pen) / nvox;
-    const double cur = diff + pen;
-    const int k = c->rec;
+// The error-based stop rule of iteration k (calFlow3d_Wei_v1.py:426,
+// 502-530) from the current error and the last three accepted errors, kept
+// in old_err[k & 1] (the shifted triple goes to old_err[(k + 1) & 1], so a
+// launch whose workgroups all decide can never read a half-updated triple).
+__device__ __forceinline__ bool err_stop(const Ctl* __restrict__ c, int k, double cur) {
+    if (c->frec >= 0) return (k == c->frec - 1) && (c->fupd == c->frec - 1);
+    if (k == c->iter - 1) return true;
+    const double* o = c->old_err[k & 1];
+    const int worse = (o[0] <= cur) + (o[1] <= cur) + (o[2] <= cur);
+    if (worse > 1) return true;
+    return fabs(o[2] - cur) < c->tol;
+}
+
+// Records iteration k's error terms (one thread of the launch).
+__device__ __forceinline__ void err_record(Ctl* __restrict__ c, int k, double diff, double pen, double cur, bool stop,
+                                          double* __restrict__ log, int* __restrict__ counts) {
     double* row = log + static_cast<size_t>(k) * 4;
     row[0] = diff;
     row[1] = pen;
@@ -86,52 +87,82 @@ __device__ __forceinline__ void err_finish_block(Ctl* __restrict__ c, const doub
     c->rec = k + 1;
     counts[0] = k + 1;
     c->last_err = cur;
-    bool stop;
-    if (c->frec >= 0) {
-        stop = (k == c->frec - 1) && (c->fupd == c->frec - 1);
-    } else if (k == c->iter - 1) {
-        stop = true;
-    } else {
-        const int worse = (c->old_err[0] <= cur) + (c->old_err[1] <= cur) + (c->old_err[2] <= cur);
-        if (worse > 1) {
-            stop = true;
-        } else if (fabs(c->old_err[2] - cur) < c->tol) {
-            stop = true;
-        } else {
-            stop = false;
-            c->old_err[0] = c->old_err[1];
-            c->old_err[1] = c->old_err[2];
-            c->old_err[2] = cur;
-        }
+    if (!stop) {
+        const double* o = c->old_err[k & 1];
+        double* n = c->old_err[(k + 1) & 1];
+        n[0] = o[1];
+        n[1] = o[2];
+        n[2] = cur;
     }
     c->maxdiff = 0u;
     if (stop) c->done = 1;
 }
 
-// The |delta motion| < 1e-3 stop test of the update just made
-// (calFlow3d_Wei_v1.py:641-654) over `nparts` per-workgroup maxima, run by
-// one workgroup.
-__device__ __forceinline__ void check_finish_block(Ctl* __restrict__ c, const float* __restrict__ mdpart, int nparts,
-                                                   double* __restrict__ log, int* __restrict__ counts) {
+// Sums of the It^2 and |D|^2 partials and the error terms of iteration k; the
+// result is the same in every workgroup of a launch (same order).  Valid in
+// thread 0.
+__device__ __forceinline__ void err_terms(const Ctl* __restrict__ c, const double* __restrict__ errpart, long long nerr,
+                                          const double* __restrict__ penpart, long long npen, double& diff,
+                                          double& pen) {
+    double e = 0.0, pp = 0.0;
+    for (long long i = threadIdx.x; i < nerr; i += blockDim.x) e += errpart[i];
+    for (long long i = threadIdx.x; i < npen; i += blockDim.x) pp += penpart[i];
+    block_sum2(e, pp);
+    const double nvox = c->inv_n;
+    diff = e / nvox;
+    pen = (pp * c->s_pen) / nvox;
+}
+
+// Error terms of iteration k, their log entries and the error-based stop
+// rule, run by one workgroup.
+__device__ __forceinline__ void err_finish_block(Ctl* __restrict__ c, int k, const double* __restrict__ errpart,
+                                                 long long nerr, const double* __restrict__ penpart, long long npen,
+                                                 double* __restrict__ log, int* __restrict__ counts) {
+    double diff, pen;
+    err_terms(c, errpart, nerr, penpart, npen, diff, pen);
+    if (threadIdx.x != 0) return;
+    const double cur = diff + pen;
+    err_record(c, k, diff, pen, cur, err_stop(c, k, cur), log, counts);
+}
+
+// Block-wide maximum of `nparts` per-workgroup maxima; valid in every thread.
+__device__ __forceinline__ float parts_max(const float* __restrict__ mdpart, int nparts) {
     __shared__ float s_m[16];
+    __shared__ float s_r;
     float md = 0.0f;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) md = fmaxf(md, mdpart[i]);
     for (int off = 32; off > 0; off >>= 1) md = fmaxf(md, __shfl_down(md, off, 64));
     if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = md;
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) md = fmaxf(md, s_m[w]);
-    const int k = c->rec - 1;
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) md = fmaxf(md, s_m[w]);
+        s_r = md;
+    }
+    __syncthreads();
+    return s_r;
+}
+
+// The |delta motion| < 1e-3 stop test of update k (calFlow3d_Wei_v1.py:
+// 641-654) and its record (one thread).
+__device__ __forceinline__ bool check_stop(const Ctl* __restrict__ c, int k, float md) {
+    if (c->frec >= 0) return k == c->frec - 1;
+    return md < 1e-3f;
+}
+__device__ __forceinline__ void check_record(Ctl* __restrict__ c, int k, float md, bool stop, double* __restrict__ log,
+                                            int* __restrict__ counts) {
     c->maxdiff = __float_as_uint(md);
     log[static_cast<size_t>(k) * 4 + 3] = static_cast<double>(md);
-    c->upd += 1;
-    counts[1] = c->upd;
+    c->upd = k + 1;
+    counts[1] = k + 1;
     counts[2] = c->nan;
-    if (c->frec >= 0) {
-        if (k == c->frec - 1) c->done = 1;
-    } else if (md < 1e-3f) {
-        c->done = 1;
-    }
+    if (stop) c->done = 1;
+}
+
+// Run by one workgroup: the test of update k over `nparts` maxima.
+__device__ __forceinline__ void check_finish_block(Ctl* __restrict__ c, int k, const float* __restrict__ mdpart,
+                                                   int nparts, double* __restrict__ log, int* __restrict__ counts) {
+    const float md = parts_max(mdpart, nparts);
+    if (threadIdx.x == 0) check_record(c, k, md, check_stop(c, k, md), log, counts);
 }
 
 }  // namespace wr

@@ -234,7 +234,12 @@ struct PassArgs {
     const float* cp;
     float* sums;
     double* errpart;
-    const Ctl* ctl;
+    Ctl* ctl;
+    int k;                 // iteration of the level
+    int check_parts;       // > 0: apply the |delta m| test of update k - 1 to mdpart[0 .. check_parts)
+    const float* mdpart;
+    double* log;
+    int* counts;
     float* dnei;           // neighbour differences (written by the appended workgroups)
     double* penpart;       // their per-workgroup sums of |D|^2
     long long npen;        // appended neighbour-difference workgroups
@@ -267,6 +272,14 @@ __device__ __forceinline__ void tile_of(long long nb, int nz, int nbx, int& bx, 
 template <bool DENSE, bool MREF, bool MMOV>
 __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     if (a.ctl->done) return;
+    if (a.check_parts > 0) {
+        // the |delta motion| stop test of the previous update, decided alike
+        // by every workgroup from the same maxima; workgroup 0 records it
+        const float md = parts_max(a.mdpart, a.check_parts);
+        const bool stop = check_stop(a.ctl, a.k - 1, md);
+        if (blockIdx.x == 0 && threadIdx.x == 0) check_record(a.ctl, a.k - 1, md, stop, a.log, a.counts);
+        if (stop) return;
+    }
     if (static_cast<long long>(blockIdx.x) >= a.blocks) {
         // appended workgroups: neighbour differences of the CP field
         // (calFlow3d_Wei_v1.py:69-98,488-498) and their |D|^2 partials
@@ -472,7 +485,8 @@ PVol make_pvol(const float* base, int nx, int ny, int nz) {
 
 }  // namespace
 
-long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s) {
+long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, int check_parts,
+                         hipStream_t s) {
     PassArgs a;
     a.mov = make_pvol(b.mov, g.nx, g.ny, g.nz);
     a.mmov = make_pvol(b.mmov ? b.mmov : b.mov, g.nx, g.ny, g.nz);
@@ -485,6 +499,11 @@ long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b
     a.ctl = b.ctl;
     a.dnei = b.dnei;
     a.penpart = b.penpart;
+    a.k = k;
+    a.check_parts = check_parts;
+    a.mdpart = b.mdpart;
+    a.log = b.log;
+    a.counts = b.counts;
     a.nx = g.nx;
     a.ny = g.ny;
     a.nz = g.nz;

@@ -151,7 +151,7 @@ struct Ctl {
     int iter;            // option["iter"]
     int frec, fupd;      // forced schedule of this level, -1 = natural
     unsigned maxdiff;    // float bits of max |delta motion| of the current update
-    double old_err[3];   // the last three accepted errors (calFlow3d:426,528-530)
+    double old_err[2][3];  // the last three accepted errors (calFlow3d:426,528-530), by iteration parity
     double tol;
     double s_pen;        // smoothPenalty * (2r+1)^2
     double inv_n;        // x*y*z of the level (the error terms divide by it)
@@ -173,14 +173,13 @@ constexpr int kMaxTileCols = 248;
 
 struct LkTile {
     int np_x;            // patches per workgroup along x
-    int nb_y;            // patch rows per workgroup
     int cols;            // np_x * p owned columns
-    int rows;            // nb_y * p owned rows
+    int rows;            // p owned rows (one patch row)
     int nbx, nby;        // workgroups along x, y
     long long blocks;    // nbx * nby * nz
 };
 
-LkTile plan_tile(const LevelGeom& g, int np_x, int nb_y);
+LkTile plan_tile(const LevelGeom& g, int np_x);
 
 struct LkBuffers {
     const float* mov;        // level moving image, padded volume (padded_layout), allocation start
@@ -204,14 +203,17 @@ struct LkBuffers {
 size_t mdpart_slots(long long ncp);
 
 // The fused per-voxel pass of iteration k plus the neighbour differences of
-// the CP field (lk_pass.hip); returns the number of |D|^2 partials written.
-long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, hipStream_t s);
+// the CP field (lk_pass.hip); with check_parts > 0 every workgroup first
+// applies the |delta motion| test of update k - 1 to that many maxima in
+// b.mdpart (workgroup 0 records it).  Returns the number of |D|^2 partials.
+long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, int check_parts,
+                         hipStream_t s);
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
                       double inv_n, hipStream_t s);
 void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, hipStream_t s);
-int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int level,
-                     int k, int iter, hipStream_t s);
+int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool separate_stop,
+                     hipStream_t s);
 void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter,
                       float* dense_soa, float* motion_aos, float* coords_aos, hipStream_t s);
 

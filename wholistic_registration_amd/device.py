@@ -90,12 +90,15 @@ class MotionResult:
 
 def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
-               schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False):
+               schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
+               stop_kernels=False):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
     the device.  ``schedule`` ({level: (records, updates)}) forces another
-    run's iteration schedule.
+    run's iteration schedule.  ``stop_kernels`` runs the stop rules in their
+    own single-workgroup kernels on every level (by default that is done only
+    where the launches that follow cannot afford to evaluate them).
     """
     torch = _torch()
     _check_vol(mov, "mov")
@@ -118,6 +121,7 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     prm.zRatio, prm.movRange = float(zRatio), float(movRange)
     prm.smoothPenalty, prm.tol = float(smoothPenalty), float(tol)
     prm.tile_patches_x = int(tile_patches_x)
+    prm.stop_kernels = 1 if stop_kernels else 0
     prm.timing = 1 if timing else 0
     keep = []
     if schedule is not None:
