@@ -26,6 +26,7 @@
 // not depend on scheduling.
 
 #include <cfloat>
+#include <algorithm>
 #include <cmath>
 
 #include "lk_common.h"
@@ -154,7 +155,13 @@ __global__ __launch_bounds__(256) void k_solve(Ctl* __restrict__ c, const float*
 // First update of a level: max |m_1 - m_0| over every voxel (m_0 is the
 // level's initial dense field, not a CP interpolant).  Grid-stride over
 // voxels with a fixed grid; one partial maximum per workgroup.
-constexpr int kDenseBlocks = 2048;
+constexpr int kDenseBlocks = 2048;   // the most workgroups of k_dense_maxdiff
+
+// Workgroups of k_dense_maxdiff on a level: one per 4096 voxels, 256 .. 2048.
+inline int dense_blocks(const LevelGeom& g) {
+    const long long nvox = static_cast<long long>(g.nx) * g.ny * g.nz;
+    return static_cast<int>(std::min<long long>(kDenseBlocks, std::max<long long>(256, nvox >> 12)));
+}
 
 __global__ __launch_bounds__(256) void k_dense_maxdiff(const Ctl* __restrict__ c, const float* __restrict__ m0,
                                                         const float* __restrict__ cp, int nx, int ny, int nz,
@@ -307,23 +314,22 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     // in the LK launch of iteration k; elsewhere in single-workgroup kernels.
     const long long nerr = t.blocks;
     const long long npen = (ncp + 63) / 64;                  // upper bound of the LK launch's |D|^2 partials
-    const long long parts_max = static_cast<long long>(cpb) + kDenseBlocks;
+    const int nd = dense_blocks(g);
+    // maxima behind the |delta motion| test of update j
+    auto parts_of = [&](int j) { return static_cast<long long>(cpb) + (j == 0 ? nd : 0); };
+    auto fuse_check = [&](int j) { return !separate_stop && (t.blocks + npen) * parts_of(j) <= (1LL << 22); };
     const bool fuse_err = !separate_stop && static_cast<long long>(cpb) * (nerr + npen) <= (1LL << 22);
-    const bool fuse_check = !separate_stop && (t.blocks + npen) * parts_max <= (1LL << 22);
-    const int prev_parts = static_cast<int>(k == 1 ? cpb + kDenseBlocks : cpb);
+    const int check_now = (k > 0 && fuse_check(k - 1)) ? static_cast<int>(parts_of(k - 1)) : 0;
     if (b.events) (void)hipEventRecord(b.events[2 * k], s);
-    const long long np = launch_lk_pass(g, t, b, k, (fuse_check && k > 0) ? prev_parts : 0, s);
+    const long long np = launch_lk_pass(g, t, b, k, check_now, s);
     if (b.events) (void)hipEventRecord(b.events[2 * k + 1], s);
     if (!fuse_err) k_err_finish<<<1, 1024, 0, s>>>(b.ctl, k, b.errpart, nerr, b.penpart, np, b.log, b.counts);
     k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d, b.mdpart, k,
                                 fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts);
-    int nparts = static_cast<int>(cpb);
-    if (k == 0) {
-        k_dense_maxdiff<<<kDenseBlocks, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
-                                                     b.mdpart + cpb);
-        nparts += kDenseBlocks;
-    }
-    if (!fuse_check) k_check<<<1, 1024, 0, s>>>(b.ctl, k, b.mdpart, nparts, b.log, b.counts);
+    if (k == 0)
+        k_dense_maxdiff<<<nd, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, b.mdpart + cpb);
+    if (!fuse_check(k))
+        k_check<<<1, 1024, 0, s>>>(b.ctl, k, b.mdpart, static_cast<int>(parts_of(k)), b.log, b.counts);
     return WR_OK;
 }
 
