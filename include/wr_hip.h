@@ -141,6 +141,91 @@ int wr_pyramid_level(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int
 int wr_imresize(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz,
                 int nx_out, int ny_out, int method, double* out, void* stream);
 
+/* ---- automatic masks and error metric of the registration harness
+ * (wbi_registration_3d, src/wholistic_registration/utils/registration.py:
+ * 259-260, 311-312, 328-329).  Volumes and masks are (Z,Y,X) device arrays. */
+
+/* Replaces robust_mean_std(data, percentile)
+ * (src/wholistic_registration/utils/preprocess.py:69-84) on float32 data (uint16
+ * data is converted to float32, as getMask does before calling it):
+ *   out[0] = numpy.percentile(data, percentile)   (linear method, float32)
+ *   out[1] = mean of the values <= out[0]
+ *   out[2] = std  of the values <= out[0]
+ * out is host memory; synchronises the stream. */
+int wr_robust_mean_std(wr_ctx* ctx, const void* data, int dtype, long long n, double percentile,
+                       float* out, void* stream);
+
+/* Replaces getMask(dat_mov, thres_factor)
+ * (src/wholistic_registration/utils/mask.py:59-111): out = 1 where the
+ * float32 z-score |(v - mu) / sigma| > thres_factor, after a 3x3 in-plane
+ * binary opening and a 3x3 binary dilation; mu, sigma from robust_mean_std
+ * at the 95th percentile, or from the host array mu_sigma[2] when not NULL.
+ *   vol  (Z,Y,X) WR_F32 or WR_U16;   out (Z,Y,X) uint8 (0 / 1). */
+int wr_get_mask(wr_ctx* ctx, const void* vol, int dtype, int nx, int ny, int nz, double thres_factor,
+                const float* mu_sigma, uint8_t* out, void* stream);
+
+/* Replaces bwareafilt3_wei(vol, size_range)
+ * (src/wholistic_registration/utils/mask.py:8-56): keeps the 26-connected
+ * components of the nonzero voxels of `in` whose voxel count lies in
+ * [size_min, size_max].  Output as uint8 (out_u8) and/or float32 0 / 1
+ * (out_f32, the form getMotion consumes); either may be NULL, not both.
+ * n_components (host, may be NULL) receives the number of components, which
+ * synchronises the stream. */
+int wr_bwareafilt3(wr_ctx* ctx, const uint8_t* in, int nx, int ny, int nz, double size_min,
+                   double size_max, uint8_t* out_u8, float* out_f32, long long* n_components,
+                   void* stream);
+
+/* Replaces np.mean((a - b) ** 2) of the harness's initial and eventual
+ * errors (src/wholistic_registration/utils/registration.py:328-329) in
+ * numpy's dtypes: uint16 - uint16 wraps modulo 2^16 (difference and square),
+ * float32 steps are rounded in float32, anything with float64 is float64;
+ * summed in float64.  out is host memory; synchronises the stream. */
+int wr_mean_sq_diff(wr_ctx* ctx, const void* a, int dtype_a, const void* b, int dtype_b, long long n,
+                    double* out, void* stream);
+
+/* ---- channel arithmetic and reference picking of the harness */
+
+/* Replaces transform(image, k, method)
+ * (src/wholistic_registration/utils/registration.py:26-36):
+ * out = k * f(in), f = identity (method 0), sqrt (1), log2(1 + x) (2),
+ * log10(1 + x) (3), as float32.  uint16 input: 1 + x wraps modulo 2^16 and f
+ * is evaluated in float32 (numpy's types for uint16 arrays); with method 0
+ * an integer k (k_is_int = 1, 0 <= k <= 65535) multiplies modulo 2^16, a
+ * float k in float64 (both stored as float32). */
+int wr_channel_transform(wr_ctx* ctx, const void* in, int dtype, long long n, double k, int method,
+                         int k_is_int, float* out, void* stream);
+
+/* np.mean(image), np.std(image) of n float32 values (the statistics of
+ * normalize_std, preprocess.py:288-293, and of the calcium reference,
+ * registration.py:244-245): out[0] = mean, out[1] = std, host; synchronises
+ * the stream. */
+int wr_mean_std(wr_ctx* ctx, const float* in, long long n, float* out, void* stream);
+
+/* Replaces normalize_std(mean, std, image)
+ * (src/wholistic_registration/utils/preprocess.py:288-293):
+ * out = ((in - mean(in)) / std(in)) * std + mean, float32. */
+int wr_normalize_std(wr_ctx* ctx, const float* in, long long n, double mean, double std_, float* out,
+                     void* stream);
+
+/* dat_mem + dat_ca_tran (registration.py:306): out = float32(a) + b. */
+int wr_add_volumes(wr_ctx* ctx, const void* a, int dtype_a, const float* b, long long n, float* out,
+                   void* stream);
+
+/* frames[top_indices].mean(axis=0) (reference.py:97 and registration.py:237):
+ * the mean of `count` device volumes of n voxels, summed in the order given.
+ * uint16 frames: out is float64 (exact sums); float32 frames: out is float32
+ * (float32 running sum, then divided by count, as numpy). */
+int wr_frames_mean(wr_ctx* ctx, const void* const* frames, int count, int dtype, long long n, void* out,
+                   void* stream);
+
+/* The correlation core of pick_initial_reference
+ * (src/wholistic_registration/utils/reference.py:39-78): slab holds nt
+ * frames' (nzs, ny, nx) slabs; each is sampled at [::ds, ::ds], converted to
+ * float32, demeaned by its own mean, and cc (host, nt * nt) receives the Gram
+ * matrix of the demeaned rows.  Synchronises the stream. */
+int wr_frame_gram(wr_ctx* ctx, const void* slab, int dtype, int nt, int nzs, int ny, int nx, int ds,
+                  double* cc, void* stream);
+
 /* Launch statistics of the last wr_get_motion on this context, for
  * measurement: LK passes launched per level, the LK kernel's tile shape and
  * grid size, and (when params->timing was 1) the number and summed HIP-event

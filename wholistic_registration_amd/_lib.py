@@ -74,6 +74,18 @@ _SIGNATURES = {
     "wr_imresize": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_get_stats": ([_VP, ctypes.POINTER(WrStats)], _I),
     "wr_smooth_penalty_factor": ([_VP, _VP, _I, _I, _I, _I, _VP, ctypes.POINTER(ctypes.c_double), _VP], _I),
+    "wr_robust_mean_std": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, ctypes.POINTER(ctypes.c_float),
+                            _VP], _I),
+    "wr_get_mask": ([_VP, _VP, _I, _I, _I, _I, ctypes.c_double, ctypes.POINTER(ctypes.c_float), _VP, _VP], _I),
+    "wr_bwareafilt3": ([_VP, _VP, _I, _I, _I, ctypes.c_double, ctypes.c_double, _VP, _VP,
+                        ctypes.POINTER(ctypes.c_longlong), _VP], _I),
+    "wr_mean_sq_diff": ([_VP, _VP, _I, _VP, _I, ctypes.c_longlong, ctypes.POINTER(ctypes.c_double), _VP], _I),
+    "wr_channel_transform": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, _I, _I, _VP, _VP], _I),
+    "wr_mean_std": ([_VP, _VP, ctypes.c_longlong, ctypes.POINTER(ctypes.c_float), _VP], _I),
+    "wr_normalize_std": ([_VP, _VP, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _VP, _VP], _I),
+    "wr_add_volumes": ([_VP, _VP, _I, _VP, ctypes.c_longlong, _VP, _VP], _I),
+    "wr_frames_mean": ([_VP, ctypes.POINTER(_VP), _I, _I, ctypes.c_longlong, _VP, _VP], _I),
+    "wr_frame_gram": ([_VP, _VP, _I, _I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_double), _VP], _I),
 }
 
 _lib = None
@@ -85,7 +97,7 @@ def header_symbols(path=HEADER):
     """Function names declared in the C header."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(wr_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(wr_[a-z0-9_]+)\s*\(", text)))
 
 
 def load():

@@ -76,7 +76,10 @@ def _volume_to_device(a, dev, name):
 def _mask_to_device(m, shape_xyz, dev, name):
     torch = _torch()
     if _is_tensor(m):
-        m = m.detach().to("cpu").numpy()
+        if tuple(m.shape) != tuple(shape_xyz):
+            raise ValueError(f"option[{name!r}] must have the volume's shape {tuple(shape_xyz)}, "
+                             f"got {tuple(m.shape)}")
+        return m.permute(2, 1, 0).to(dev, torch.float32).contiguous()
     arr = np.asarray(m, dtype=np.float32)
     if arr.shape != tuple(shape_xyz):
         raise ValueError(f"option[{name!r}] must have the volume's shape {tuple(shape_xyz)}, "
@@ -91,7 +94,10 @@ def _field_to_device(f, shape_xyz, dev):
     if f is None:
         return None
     if _is_tensor(f):
-        f = f.detach().to("cpu").numpy()
+        if tuple(f.shape) != tuple(shape_xyz) + (3,):
+            raise ValueError(f"option['motion'] must have shape {tuple(shape_xyz) + (3,)}, "
+                             f"got {tuple(f.shape)}")
+        return f.permute(2, 1, 0, 3).to(dev, torch.float32).contiguous()
     arr = np.asarray(f, dtype=np.float32)
     if arr.shape != tuple(shape_xyz) + (3,):
         raise ValueError(f"option['motion'] must have shape {tuple(shape_xyz) + (3,)}, got {arr.shape}")
@@ -117,8 +123,11 @@ def getMotion(dat_mov, dat_ref, option, verbose=False):
     """
     for key in REQUIRED_KEYS:
         option[key]                 # KeyError, as the reference raises
-    option["mask_ref"] = np.asarray(option["mask_ref"], dtype=np.float32)
-    option["mask_mov"] = np.asarray(option["mask_mov"], dtype=np.float32)
+    for key in ("mask_ref", "mask_mov"):
+        if not _is_tensor(option[key]):
+            option[key] = np.asarray(option[key], dtype=np.float32)
+        elif option[key].dtype != _torch().float32:
+            option[key] = option[key].to(_torch().float32)
     dev = _device()
     torch_in = _is_tensor(dat_mov)
     shape_xyz = tuple(dat_mov.shape)

@@ -177,7 +177,8 @@ void wr_destroy(wr_ctx* ctx) {
     Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
-                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen})
+                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen,
+                      &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c.lvl_events) (void)hipEventDestroy(e);
@@ -536,6 +537,232 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         const int n = hcnt[0];
         log->final_error = n > 0 ? hlog[static_cast<size_t>(n - 1) * 4 + 2] : NAN;
     }
+    return WR_OK;
+}
+
+
+// ------------------------------------------------------------------ masks
+
+namespace {
+
+struct MaskWs {
+    SelCtl* c;
+    unsigned* hist;
+    double* part;
+    long long* cpart;
+    unsigned long long* ncomp;
+    double* out;
+};
+
+int mask_ws(Context& c, MaskWs& w) {
+    const size_t o_hist = 256, o_part = o_hist + 3 * 2048 * sizeof(unsigned);
+    const size_t o_cpart = o_part + kStreamBlocks * sizeof(double);
+    const size_t o_nc = o_cpart + kStreamBlocks * sizeof(long long);
+    const size_t o_out = o_nc + 64;
+    WR_TRY(grow(c.msel, o_out + 64));
+    char* b = c.msel.as<char>();
+    w.c = reinterpret_cast<SelCtl*>(b);
+    w.hist = reinterpret_cast<unsigned*>(b + o_hist);
+    w.part = reinterpret_cast<double*>(b + o_part);
+    w.cpart = reinterpret_cast<long long*>(b + o_cpart);
+    w.ncomp = reinterpret_cast<unsigned long long*>(b + o_nc);
+    w.out = reinterpret_cast<double*>(b + o_out);
+    return WR_OK;
+}
+
+}  // namespace
+
+int wr_robust_mean_std(wr_ctx* ctx, const void* data, int dtype, long long n, double percentile, float* out,
+                       void* stream) {
+    WR_ARG(ctx && data && out, "wr_robust_mean_std: null argument");
+    WR_ARG(n > 0, "wr_robust_mean_std: empty array");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_robust_mean_std: dtype must be WR_F32 or WR_U16");
+    WR_ARG(percentile >= 0.0 && percentile <= 100.0, "Percentiles must be in the range [0, 100]");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    MaskWs w;
+    WR_TRY(mask_ws(c, w));
+    launch_robust_stats(data, elem_of(dtype), n, percentile, w.c, w.hist, w.part, w.cpart, s);
+    WR_HIP_CHECK(hipGetLastError());
+    SelCtl h;
+    WR_HIP_CHECK(hipMemcpyAsync(&h, w.c, sizeof(h), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipStreamSynchronize(s));
+    out[0] = h.thr;
+    out[1] = h.mu;
+    out[2] = h.sigma;
+    return WR_OK;
+}
+
+int wr_get_mask(wr_ctx* ctx, const void* vol, int dtype, int nx, int ny, int nz, double thres_factor,
+                const float* mu_sigma, uint8_t* out, void* stream) {
+    WR_ARG(ctx && vol && out, "wr_get_mask: null argument");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_get_mask: empty volume");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_get_mask: dtype must be WR_F32 or WR_U16");
+    WR_ARG(nz <= 65535 && (ny + 15) / 16 <= 65535, "wr_get_mask: more than 65535 planes");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    MaskWs w;
+    WR_TRY(mask_ws(c, w));
+    const long long n = static_cast<long long>(nx) * ny * nz;
+    if (mu_sigma)
+        launch_set_musig(w.c, mu_sigma[0], mu_sigma[1], s);
+    else
+        launch_robust_stats(vol, elem_of(dtype), n, 95.0, w.c, w.hist, w.part, w.cpart, s);
+    launch_getmask(vol, elem_of(dtype), nx, ny, nz, w.c, static_cast<float>(thres_factor), out, s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_bwareafilt3(wr_ctx* ctx, const uint8_t* in, int nx, int ny, int nz, double size_min, double size_max,
+                   uint8_t* out_u8, float* out_f32, long long* n_components, void* stream) {
+    WR_ARG(ctx && in && (out_u8 || out_f32), "wr_bwareafilt3: null argument");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_bwareafilt3: empty volume");
+    WR_ARG(static_cast<double>(nx) * ny * nz < 2147483647.0, "wr_bwareafilt3: volume exceeds 2^31 - 1 voxels");
+    WR_ARG(ny <= 65535 && nz <= 65535, "wr_bwareafilt3: more than 65535 rows or planes");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    MaskWs w;
+    WR_TRY(mask_ws(c, w));
+    const size_t n = static_cast<size_t>(nx) * ny * nz;
+    WR_TRY(grow(c.mlab, n * sizeof(int)));
+    WR_TRY(grow(c.mcnt, n * sizeof(int)));
+    launch_components(in, nx, ny, nz, size_min, size_max, c.mlab.as<int>(), c.mcnt.as<int>(), w.ncomp, out_u8,
+                      out_f32, s);
+    WR_HIP_CHECK(hipGetLastError());
+    if (n_components) {
+        unsigned long long h = 0;
+        WR_HIP_CHECK(hipMemcpyAsync(&h, w.ncomp, sizeof(h), hipMemcpyDeviceToHost, s));
+        WR_HIP_CHECK(hipStreamSynchronize(s));
+        *n_components = static_cast<long long>(h);
+    }
+    return WR_OK;
+}
+
+int wr_mean_sq_diff(wr_ctx* ctx, const void* a, int dtype_a, const void* b, int dtype_b, long long n, double* out,
+                    void* stream) {
+    WR_ARG(ctx && a && b && out, "wr_mean_sq_diff: null argument");
+    WR_ARG(n > 0, "wr_mean_sq_diff: empty arrays");
+    for (int d : {dtype_a, dtype_b})
+        WR_ARG(d == WR_F32 || d == WR_U16 || d == WR_F64, "wr_mean_sq_diff: unsupported dtype");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    MaskWs w;
+    WR_TRY(mask_ws(c, w));
+    launch_mean_sq_diff(a, elem_of(dtype_a), b, elem_of(dtype_b), n, w.part, w.out, s);
+    WR_HIP_CHECK(hipGetLastError());
+    WR_HIP_CHECK(hipMemcpyAsync(out, w.out, sizeof(double), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipStreamSynchronize(s));
+    return WR_OK;
+}
+
+
+// ---------------------------------------------------------------- harness
+
+int wr_channel_transform(wr_ctx* ctx, const void* in, int dtype, long long n, double k, int method, int k_is_int,
+                         float* out, void* stream) {
+    WR_ARG(ctx && in && out, "wr_channel_transform: null argument");
+    WR_ARG(n > 0, "wr_channel_transform: empty array");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_channel_transform: dtype must be WR_F32 or WR_U16");
+    WR_ARG(method >= 0 && method <= 3, "Unknown method to process the image");
+    WR_ARG(!k_is_int || (k >= 0.0 && k <= 65535.0 && k == std::floor(k)),
+           "wr_channel_transform: integer k outside the uint16 range");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_transform(in, elem_of(dtype), n, k, method, k_is_int, out, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+namespace {
+int harness_small(Context& c, double** part, float** ms) {
+    WR_TRY(grow(c.hsmall, 256 + kStreamBlocks * sizeof(double)));
+    *ms = c.hsmall.as<float>();
+    *part = reinterpret_cast<double*>(c.hsmall.as<char>() + 256);
+    return WR_OK;
+}
+}  // namespace
+
+int wr_mean_std(wr_ctx* ctx, const float* in, long long n, float* out, void* stream) {
+    WR_ARG(ctx && in && out, "wr_mean_std: null argument");
+    WR_ARG(n > 0, "wr_mean_std: empty array");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* part;
+    float* ms;
+    WR_TRY(harness_small(c, &part, &ms));
+    launch_mean_std(in, n, part, ms, s);
+    WR_HIP_CHECK(hipGetLastError());
+    WR_HIP_CHECK(hipMemcpyAsync(out, ms, 2 * sizeof(float), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipStreamSynchronize(s));
+    return WR_OK;
+}
+
+int wr_normalize_std(wr_ctx* ctx, const float* in, long long n, double mean, double std_, float* out,
+                     void* stream) {
+    WR_ARG(ctx && in && out, "wr_normalize_std: null argument");
+    WR_ARG(n > 0, "wr_normalize_std: empty array");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* part;
+    float* ms;
+    WR_TRY(harness_small(c, &part, &ms));
+    launch_mean_std(in, n, part, ms, s);
+    launch_normalize(in, n, ms, static_cast<float>(mean), static_cast<float>(std_), out, s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_add_volumes(wr_ctx* ctx, const void* a, int dtype_a, const float* b, long long n, float* out,
+                   void* stream) {
+    WR_ARG(ctx && a && b && out, "wr_add_volumes: null argument");
+    WR_ARG(n > 0, "wr_add_volumes: empty array");
+    WR_ARG(dtype_a == WR_F32 || dtype_a == WR_U16, "wr_add_volumes: dtype must be WR_F32 or WR_U16");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_add(a, elem_of(dtype_a), b, n, out, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_frames_mean(wr_ctx* ctx, const void* const* frames, int count, int dtype, long long n, void* out,
+                   void* stream) {
+    WR_ARG(ctx && frames && out, "wr_frames_mean: null argument");
+    WR_ARG(count > 0 && n > 0, "wr_frames_mean: no frames");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_frames_mean: dtype must be WR_F32 or WR_U16");
+    for (int f = 0; f < count; ++f) WR_ARG(frames[f], "wr_frames_mean: null frame");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int f = 0; f < count; ++f) launch_accumulate(frames[f], elem_of(dtype), n, f == 0, out, s);
+    launch_divide(out, elem_of(dtype), n, count, s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_frame_gram(wr_ctx* ctx, const void* slab, int dtype, int nt, int nzs, int ny, int nx, int ds, double* cc,
+                  void* stream) {
+    WR_ARG(ctx && slab && cc, "wr_frame_gram: null argument");
+    WR_ARG(nt > 0 && nzs > 0 && ny > 0 && nx > 0 && ds > 0, "wr_frame_gram: empty shape");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_frame_gram: dtype must be WR_F32 or WR_U16");
+    WR_ARG(nt <= 65535, "wr_frame_gram: more than 65535 frames");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const long long d = static_cast<long long>(nzs) * ((ny + ds - 1) / ds) * ((nx + ds - 1) / ds);
+    const size_t nt2 = static_cast<size_t>(nt) * nt;
+    const size_t m_bytes = (static_cast<size_t>(nt) * d * sizeof(float) + 255) / 256 * 256;
+    const size_t mean_bytes = (nt * sizeof(float) + 255) / 256 * 256;
+    WR_TRY(grow(c.hbig, m_bytes + mean_bytes + nt2 * sizeof(double)));
+    float* m = c.hbig.as<float>();
+    float* mean = reinterpret_cast<float*>(c.hbig.as<char>() + m_bytes);
+    double* dcc = reinterpret_cast<double*>(c.hbig.as<char>() + m_bytes + mean_bytes);
+    launch_frame_gram(slab, elem_of(dtype), nt, nzs, ny, nx, ds, m, mean, dcc, s);
+    WR_HIP_CHECK(hipGetLastError());
+    WR_HIP_CHECK(hipMemcpyAsync(cc, dcc, nt2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipStreamSynchronize(s));
     return WR_OK;
 }
 

@@ -229,6 +229,68 @@ void launch_warp(const void* in, Elem in_t, int nx, int ny, int nz, const float*
 void launch_penalty(const void* ref, Elem t, const float* mask, int nx, int ny, int nz, double* part,
                     double* out, hipStream_t s);
 
+// ---------------------------------------------------------------- masks
+
+// Device state of the robust statistics (percentile select, mean, std).
+struct SelCtl {
+    unsigned prefix, pmask;  // key bits fixed so far
+    long long rank;          // rank of the wanted order statistic within the prefix class
+    long long less;          // keys below the prefix class
+    long long want_next;     // index of the upper order statistic
+    unsigned key_lo, key_hi; // keys of the two order statistics
+    int need_min;            // the upper one differs from the lower
+    float gamma;             // interpolation weight
+    float thr, mu, sigma;    // percentile, mean and std of the values <= thr
+    long long count;         // number of values <= thr
+};
+
+// Workgroups of the grid-stride streaming reductions (fixed: their partials
+// are reduced in a fixed order).
+constexpr int kStreamBlocks = 2048;
+
+// numpy.percentile's float32 index arithmetic (linear method).
+void percentile_index(long long n, double percentile, long long* prev, long long* next, float* gamma);
+// robust_mean_std of n float32 or uint16 values (as float32): fills c->thr,
+// c->mu, c->sigma.  hist holds 3 * 2048 counters, part and cpart
+// kStreamBlocks partials each.
+void launch_robust_stats(const void* in, Elem t, long long n, double percentile, SelCtl* c, unsigned* hist,
+                         double* part, long long* cpart, hipStream_t s);
+void launch_set_musig(SelCtl* c, float mu, float sigma, hipStream_t s);
+// getMask on a (nz, ny, nx) volume with c->mu, c->sigma: out 0/1 bytes.
+void launch_getmask(const void* in, Elem t, int nx, int ny, int nz, const SelCtl* c, float thres, uint8_t* out,
+                    hipStream_t s);
+// bwareafilt3_wei: 26-connected components of the nonzero bytes of m with an
+// area in [lo, hi]; L and cnt hold one int per voxel; ncomp receives the
+// number of components; o8 / of (either may be null)
+// receive the kept voxels as bytes / float32.
+void launch_components(const uint8_t* m, int nx, int ny, int nz, double lo, double hi, int* L, int* cnt,
+                       unsigned long long* ncomp, uint8_t* o8, float* of, hipStream_t s);
+// mean((a - b) ** 2) in numpy's dtypes, summed in float64; part holds
+// kStreamBlocks partials; *out (device) receives the mean.
+void launch_mean_sq_diff(const void* a, Elem ta, const void* b, Elem tb, long long n, double* part, double* out,
+                         hipStream_t s);
+
+// ---------------------------------------------------------------- harness
+
+// k * f(image) as float32 (method 0 identity, 1 sqrt, 2 log2(1+x), 3 log10(1+x)).
+void launch_transform(const void* in, Elem t, long long n, double k, int method, int k_int, float* out,
+                      hipStream_t s);
+// ms[0] = mean, ms[1] = std of n float32 values (float64 sums); part holds
+// kStreamBlocks partials.
+void launch_mean_std(const float* in, long long n, double* part, float* ms, hipStream_t s);
+// ((x - prev[0]) / prev[1]) * std + mean in float32 steps.
+void launch_normalize(const float* in, long long n, const float* prev, float mean, float std_, float* out,
+                      hipStream_t s);
+// out = float32(a) + b.
+void launch_add(const void* a, Elem ta, const float* b, long long n, float* out, hipStream_t s);
+// acc (float64 for uint16 frames, float32 for float32) = acc + frame (first: = frame).
+void launch_accumulate(const void* frame, Elem t, long long n, int first, void* acc, hipStream_t s);
+void launch_divide(void* acc, Elem t, long long n, int count, hipStream_t s);
+// Demeaned Gram matrix of nt frames' slabs (nt, nzs, ny, nx) sampled at
+// [::ds, ::ds]: m holds nt * D floats, mean nt floats, cc nt * nt doubles.
+void launch_frame_gram(const void* slab, Elem t, int nt, int nzs, int ny, int nx, int ds, float* m, float* mean,
+                       double* cc, hipStream_t s);
+
 // ---------------------------------------------------------------- context
 
 struct Context {
@@ -236,6 +298,8 @@ struct Context {
     DevBuf mov_l, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;
     DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts, mdpart;
     DevBuf tab_w, tab_i, pen;
+    DevBuf msel, mlab, mcnt, mtmp;       // mask statistics, component labels / areas, byte mask
+    DevBuf hsmall, hbig;                 // harness: statistics / Gram scratch, sampled frame matrix
     std::vector<hipEvent_t> events;
     hipStream_t side = nullptr;          // pyramid builds, ahead of the iterations
     std::vector<hipEvent_t> lvl_events;  // [L]: level L's images are ready; [layer+1]: fork

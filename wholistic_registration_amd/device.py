@@ -234,3 +234,198 @@ def smooth_penalty_factor(ref, mask_ref=None):
                                       ctypes.byref(out), _stream())
     _lib.check(rc, "wr_smooth_penalty_factor")
     return float(out.value)
+
+
+def robust_mean_std(data, percentile=95):
+    """``robust_mean_std`` (preprocess.py:69-84) of a float32 or uint16 device
+    tensor, in float32 arithmetic: ``(threshold, mean, std)`` as numpy
+    float32 scalars, where threshold is ``numpy.percentile(data, percentile)``
+    and mean / std are over the values at or below it."""
+    _check_dense(data, "data")
+    out = (ctypes.c_float * 3)()
+    lib = _lib.load()
+    ctx = _lib.context(data.device.index or 0)
+    rc = lib.wr_robust_mean_std(ctx, _ptr(data), dtype_code(data), int(data.numel()), float(percentile),
+                                out, _stream())
+    _lib.check(rc, "wr_robust_mean_std")
+    return np.float32(out[0]), np.float32(out[1]), np.float32(out[2])
+
+
+def get_mask(vol, thres_factor, mu_sigma=None, out=None):
+    """``getMask`` (mask.py:59-111) of a (Z, Y, X) float32 or uint16 device
+    volume: uint8 (Z, Y, X) tensor, 1 where the float32 z-score
+    ``|(v - mu) / sigma|`` exceeds ``thres_factor`` after a 3x3 in-plane
+    opening and dilation.  ``mu_sigma`` overrides the robust statistics."""
+    torch = _torch()
+    _check_vol(vol, "volume")
+    nz, ny, nx = vol.shape
+    if out is None:
+        out = torch.empty(vol.shape, dtype=torch.uint8, device=vol.device)
+    ms = None if mu_sigma is None else (ctypes.c_float * 2)(float(mu_sigma[0]), float(mu_sigma[1]))
+    lib = _lib.load()
+    ctx = _lib.context(vol.device.index or 0)
+    rc = lib.wr_get_mask(ctx, _ptr(vol), dtype_code(vol), nx, ny, nz, float(thres_factor), ms, _ptr(out),
+                         _stream())
+    _lib.check(rc, "wr_get_mask")
+    return out
+
+
+def bwareafilt3(mask, size_range, out_dtype="uint8", count=False):
+    """``bwareafilt3_wei`` (mask.py:8-56) of a (Z, Y, X) uint8 device mask:
+    the 26-connected components whose voxel count lies in
+    ``[size_range[0], size_range[1]]``, as uint8 or float32 (0 / 1).  With
+    ``count=True`` returns ``(mask, number_of_components)``."""
+    torch = _torch()
+    _check_vol(mask, "mask")
+    if mask.dtype != torch.uint8:
+        raise ValueError("mask must be a uint8 tensor")
+    if out_dtype not in ("uint8", "float32"):
+        raise ValueError("out_dtype must be 'uint8' or 'float32'")
+    nz, ny, nx = mask.shape
+    out = torch.empty(mask.shape, dtype=getattr(torch, out_dtype), device=mask.device)
+    n = ctypes.c_longlong(-1)
+    lib = _lib.load()
+    ctx = _lib.context(mask.device.index or 0)
+    rc = lib.wr_bwareafilt3(ctx, _ptr(mask), nx, ny, nz, float(size_range[0]), float(size_range[1]),
+                            _ptr(out) if out_dtype == "uint8" else None,
+                            _ptr(out) if out_dtype == "float32" else None,
+                            ctypes.byref(n) if count else None, _stream())
+    _lib.check(rc, "wr_bwareafilt3")
+    return (out, int(n.value)) if count else out
+
+
+def mean_sq_diff(a, b):
+    """``np.mean((a - b) ** 2)`` (registration.py:328-329) of two device
+    tensors of equal size in numpy's dtypes (float32, uint16, float64):
+    uint16 differences and squares wrap modulo 2^16; returned as numpy
+    float32 when both operands are float32 / uint16 with one float32, else as
+    float64."""
+    torch = _torch()
+    _check_dense(a, "a")
+    _check_dense(b, "b")
+    if a.numel() != b.numel():
+        raise ValueError("a and b must have the same number of elements")
+    out = ctypes.c_double()
+    lib = _lib.load()
+    ctx = _lib.context(a.device.index or 0)
+    rc = lib.wr_mean_sq_diff(ctx, _ptr(a), dtype_code(a), _ptr(b), dtype_code(b), int(a.numel()),
+                             ctypes.byref(out), _stream())
+    _lib.check(rc, "wr_mean_sq_diff")
+    kinds = {a.dtype, b.dtype}
+    if torch.float64 in kinds or kinds == {torch.uint16}:
+        return np.float64(out.value)
+    return np.float32(out.value)
+
+
+def _check_dense(t, name):
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+_METHODS = {"raw": 0, "sqrt": 1, "log2": 2, "log10": 3}
+
+
+def channel_transform(image, k, method):
+    """``transform(image, k, method)`` (registration.py:26-36) of a float32 or
+    uint16 device tensor, as a float32 tensor of the same shape."""
+    torch = _torch()
+    _check_dense(image, "image")
+    if method not in _METHODS:
+        raise ValueError(f"Unknown method to process the image:{method}")
+    k_int = isinstance(k, (int, np.integer)) and not isinstance(k, bool)
+    if k_int and image.dtype == torch.uint16 and method == "raw" and not 0 <= int(k) <= 65535:
+        raise OverflowError(f"Python integer {k} out of bounds for uint16")
+    out = torch.empty(image.shape, dtype=torch.float32, device=image.device)
+    lib = _lib.load()
+    ctx = _lib.context(image.device.index or 0)
+    rc = lib.wr_channel_transform(ctx, _ptr(image), dtype_code(image), int(image.numel()), float(k),
+                                  _METHODS[method], 1 if k_int else 0, _ptr(out), _stream())
+    _lib.check(rc, "wr_channel_transform")
+    return out
+
+
+def mean_std(image):
+    """``(np.mean(image), np.std(image))`` of a float32 device tensor, as
+    numpy float32 scalars (float64 sums)."""
+    torch = _torch()
+    _check_dense(image, "image")
+    if image.dtype != torch.float32:
+        raise ValueError("image must be float32")
+    out = (ctypes.c_float * 2)()
+    lib = _lib.load()
+    ctx = _lib.context(image.device.index or 0)
+    _lib.check(lib.wr_mean_std(ctx, _ptr(image), int(image.numel()), out, _stream()), "wr_mean_std")
+    return np.float32(out[0]), np.float32(out[1])
+
+
+def normalize_std(mean, std, image):
+    """``normalize_std(mean, std, image)`` (preprocess.py:288-293) of a
+    float32 device tensor: rescaled to the given mean and std."""
+    torch = _torch()
+    _check_dense(image, "image")
+    if image.dtype != torch.float32:
+        raise ValueError("image must be float32")
+    out = torch.empty_like(image)
+    lib = _lib.load()
+    ctx = _lib.context(image.device.index or 0)
+    rc = lib.wr_normalize_std(ctx, _ptr(image), int(image.numel()), float(mean), float(std), _ptr(out),
+                              _stream())
+    _lib.check(rc, "wr_normalize_std")
+    return out
+
+
+def add_volumes(a, b):
+    """``a + b`` for a float32/uint16 and a float32 device tensor of equal
+    size, as float32 (numpy's type for these operands)."""
+    torch = _torch()
+    _check_dense(a, "a")
+    _check_dense(b, "b")
+    if b.dtype != torch.float32 or a.numel() != b.numel():
+        raise ValueError("b must be float32 with a's size")
+    out = torch.empty(a.shape, dtype=torch.float32, device=a.device)
+    lib = _lib.load()
+    ctx = _lib.context(a.device.index or 0)
+    rc = lib.wr_add_volumes(ctx, _ptr(a), dtype_code(a), _ptr(b), int(a.numel()), _ptr(out), _stream())
+    _lib.check(rc, "wr_add_volumes")
+    return out
+
+
+def frames_mean(frames):
+    """Mean of a sequence of equally shaped float32 or uint16 device tensors,
+    summed in the given order: float64 for uint16 frames, float32 for
+    float32 (``frames[idx].mean(axis=0)`` in numpy's types)."""
+    torch = _torch()
+    frames = list(frames)
+    if not frames:
+        raise ValueError("no frames")
+    f0 = frames[0]
+    for f in frames:
+        _check_dense(f, "frame")
+        if f.shape != f0.shape or f.dtype != f0.dtype:
+            raise ValueError("frames must share shape and dtype")
+    out_dtype = torch.float64 if f0.dtype == torch.uint16 else torch.float32
+    out = torch.empty(f0.shape, dtype=out_dtype, device=f0.device)
+    ptrs = (ctypes.c_void_p * len(frames))(*[f.data_ptr() for f in frames])
+    lib = _lib.load()
+    ctx = _lib.context(f0.device.index or 0)
+    rc = lib.wr_frames_mean(ctx, ptrs, len(frames), dtype_code(f0), int(f0.numel()), _ptr(out), _stream())
+    _lib.check(rc, "wr_frames_mean")
+    return out
+
+
+def frame_gram(slab, ds):
+    """Gram matrix (float64 numpy (T, T)) of the demeaned float32 rows of a
+    (T, Zs, Y, X) device tensor sampled at ``[:, :, ::ds, ::ds]``."""
+    _check_dense(slab, "slab")
+    if slab.dim() != 4:
+        raise ValueError("slab must be (T, Zs, Y, X)")
+    nt, nzs, ny, nx = slab.shape
+    cc = np.zeros((nt, nt), dtype=np.float64)
+    lib = _lib.load()
+    ctx = _lib.context(slab.device.index or 0)
+    rc = lib.wr_frame_gram(ctx, _ptr(slab), dtype_code(slab), nt, nzs, ny, nx, int(ds),
+                           cc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _stream())
+    _lib.check(rc, "wr_frame_gram")
+    return cc

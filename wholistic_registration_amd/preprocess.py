@@ -1,9 +1,11 @@
-"""Drop-in for the hot-path part of ``src/wholistic_registration/utils/preprocess.py``.
+"""Drop-in for the registration-path part of ``src/wholistic_registration/utils/preprocess.py``.
 
 ``getSmPnltNormFctr(dat_ref, option)`` (preprocess.py:42-66) is the factor the
 harness multiplies the raw smoothness weight by before ``getMotion``
-(registration.py:295).  It runs in libwrhip.so (``wr_smooth_penalty_factor``)
-on the current HIP device.
+(registration.py:295).  ``robust_mean_std(data, percentile)``
+(preprocess.py:69-84) gives the statistics of ``getMask``.  Both run in
+libwrhip.so (``wr_smooth_penalty_factor``, ``wr_robust_mean_std``) on the
+current HIP device.
 """
 
 from __future__ import annotations
@@ -65,8 +67,35 @@ def getSmPnltNormFctr(dat_ref, option):
     ref = _to_zyx(dat_ref, dev, "dat_ref", None if ok else torch.float64)
     m = None
     if mask is not None:
-        m = _to_zyx(np.asarray(mask.detach().cpu().numpy() if isinstance(mask, torch.Tensor) else mask,
-                               dtype=np.float32), dev, "mask_ref")
+        if isinstance(mask, torch.Tensor):
+            m = _to_zyx(mask, dev, "mask_ref", None if mask.dtype == torch.float32 else torch.float32)
+        else:
+            m = _to_zyx(np.asarray(mask, dtype=np.float32), dev, "mask_ref")
         if tuple(m.shape) != tuple(ref.shape):
             raise ValueError("option['mask_ref'] must have the volume's shape")
     return _dev.smooth_penalty_factor(ref, m)
+
+
+def robust_mean_std(data, percentile=95):
+    """Mean and std of the values at or below the ``percentile``-th
+    percentile of ``data`` (preprocess.py:69-84), in float32 arithmetic:
+    float32 data as the reference computes it, uint16 data converted to
+    float32 first (as ``getMask`` does before calling it).  numpy inputs are
+    copied to the device; the percentile selection and the moments run in
+    libwrhip.so.  Returns numpy float32 scalars."""
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise RuntimeError("wholistic_registration_amd needs a HIP device (none is visible)")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    if isinstance(data, torch.Tensor):
+        t = data.reshape(-1)
+        if t.dtype not in (torch.float32, torch.uint16):
+            t = t.to(torch.float32)
+        t = t.to(dev).contiguous()
+    else:
+        a = np.asarray(data).reshape(-1)
+        if a.dtype not in (np.float32, np.uint16):
+            a = a.astype(np.float32)
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    _, mean, std = _dev.robust_mean_std(t, percentile)
+    return mean, std
