@@ -141,3 +141,25 @@ def test_compute_reference_from_block(cuda):
     w_ref, w_idx = harness_oracle.pick_initial_reference(movs, max_corr_frames=3)
     assert np.array_equal(idx, w_idx)
     assert np.array_equal(ref, w_ref)
+
+
+def test_register_series_end_to_end(cuda):
+    """The whole time-series schedule on the GPU: every frame registered once,
+    closer to the scene than before."""
+    from wholistic_registration_amd import series
+    tmpl, movs = harness_stack(seed=903, frames=7)
+    cfg = _config(layer=1, r=3, n_iter=5)
+    cfg.update({
+        "frames": {"frames": list(range(7)), "frame_downsample": 1},
+        "processing": {"batch_size": 2},
+        "save_config": {"save_ref": True, "save_motion": True},
+    })
+    cfg["MetaData"]["fps"] = 1.0
+    cfg["reference"].update({"time_measurement": "frame", "mid_window_size": 3, "window_size": 2,
+                             "mid_stride": 1, "reference_chunk": 2})
+    out = {}
+    series.register_series(cfg, lambda ids, ch: movs[list(ids)], lambda kind, fid, a: out.setdefault(kind, {}).update({fid: a}))
+    assert sorted(out["membrane"]) == list(range(7)) and sorted(out["motion"]) == list(range(7))
+    before = np.mean([np.mean((movs[i] - tmpl) ** 2) for i in range(7)])
+    after = np.mean([np.mean((out["membrane"][i] - tmpl) ** 2) for i in range(7)])
+    assert np.isfinite(after) and after < before
