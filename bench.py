@@ -160,6 +160,53 @@ def crop_checks(tmpl, mov, mask, w, crop, want_cpu, want_epe):
     return out
 
 
+def mask_measure(vol, crop, reps=3):
+    """The harness's automatic mask of one moving volume (getMask +
+    bwareafilt3_wei with the default config's thresFactor 5 and maskRange
+    [5, 4000]) on the device, timed with events on the launch stream, and the
+    oracle's mask of the centre crop on one host thread."""
+    import torch
+
+    from oracle import mask_oracle as MO
+    from wholistic_registration_amd import device as D
+    nz, ny, nx = vol.shape
+    n = vol.numel()
+    s_in = vol.element_size()
+    m8 = torch.empty(vol.shape, dtype=torch.uint8, device=vol.device)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t_mask, t_cc = [], []
+    for _ in range(reps + 1):
+        ev[0].record()
+        D.get_mask(vol, 5, out=m8)
+        ev[1].record()
+        f = D.bwareafilt3(m8, (5, 4000), out_dtype="float32")
+        ev[2].record()
+        torch.cuda.synchronize()
+        t_mask.append(ev[0].elapsed_time(ev[1]))
+        t_cc.append(ev[1].elapsed_time(ev[2]))
+    t_mask, t_cc = float(np.mean(t_mask[1:])), float(np.mean(t_cc[1:]))
+    frac = float((f > 0).float().mean())
+    cz, cy, cx = crop
+    z0, y0, x0 = (nz - cz) // 2, (ny - cy) // 2, (nx - cx) // 2
+    sub = vol[z0:z0 + cz, y0:y0 + cy, x0:x0 + cx].cpu().numpy().transpose(2, 1, 0)
+    t0 = time.perf_counter()
+    MO.bwareafilt3(MO.get_mask(sub, 5), (5, 4000))
+    dt = time.perf_counter() - t0
+    b_mask = (s_in + 1) * n          # read the volume, write the byte mask
+    b_cc = (1 + 4) * n               # read the byte mask, write the float32 mask
+    return {
+        "get_mask_ms": t_mask, "bwareafilt3_ms": t_cc, "masked_fraction": frac,
+        "get_mask_gbs": b_mask / (t_mask * 1e-3) / 1e9, "bwareafilt3_gbs": b_cc / (t_cc * 1e-3) / 1e9,
+        "algorithmic_bytes": {"get_mask": b_mask, "bwareafilt3": b_cc},
+        "voxels_per_s": n / ((t_mask + t_cc) * 1e-3),
+        "cpu_baseline": {"value": sub.size / dt, "unit": "masked voxels/s", "cores": 1, "kind": "port",
+                         "sample": f"mask oracle (numpy percentile/z-score, scipy.ndimage morphology "
+                                   f"and 26-connected labelling) on the centre crop {cz}x{cy}x{cx}, "
+                                   f"one thread, {dt:.2f} s"},
+        "sample": "moving volume of the workload, thresFactor 5, maskRange [5, 4000]",
+    }
+
+
 def truth_epe(motion, d, mask):
     """Endpoint error of the GPU field against the synthetic registration
     field m = d(p + m) (fixed point, evaluated with the library's warp)."""
@@ -190,6 +237,8 @@ def main():
                     help="skip the float64-oracle endpoint-error check on the crop")
     ap.add_argument("--cpu-crop", default="8,512,1024", help="z,y,x of the CPU-baseline crop")
     ap.add_argument("--tile-patches-x", type=int, default=0)
+    ap.add_argument("--no-mask-measure", action="store_true",
+                    help="skip timing the harness's automatic mask kernels")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -283,6 +332,9 @@ def main():
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
         out["roofline"]["traffic_source"] = traffic["source"]
     out["epe_vs_truth"] = truth_epe(motion, field, mask)
+    if not args.no_mask_measure:
+        log("[bench] harness mask of the moving volume")
+        out["harness_masks"] = mask_measure(mov, tuple(int(v) for v in args.cpu_crop.split(",")))
     if not (args.no_cpu_baseline and args.no_epe_check):
         crop = tuple(int(v) for v in args.cpu_crop.split(","))
         log(f"[bench] oracle runs on crop {crop}")

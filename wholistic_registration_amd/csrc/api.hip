@@ -555,7 +555,7 @@ struct MaskWs {
 };
 
 int mask_ws(Context& c, MaskWs& w) {
-    const size_t o_hist = 256, o_part = o_hist + 3 * 2048 * sizeof(unsigned);
+    const size_t o_hist = 256, o_part = o_hist + kHist16Bins * sizeof(unsigned);
     const size_t o_cpart = o_part + kStreamBlocks * sizeof(double);
     const size_t o_nc = o_cpart + kStreamBlocks * sizeof(long long);
     const size_t o_out = o_nc + 64;
@@ -599,7 +599,8 @@ int wr_get_mask(wr_ctx* ctx, const void* vol, int dtype, int nx, int ny, int nz,
     WR_ARG(ctx && vol && out, "wr_get_mask: null argument");
     WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_get_mask: empty volume");
     WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_get_mask: dtype must be WR_F32 or WR_U16");
-    WR_ARG(nz <= 65535 && (ny + 15) / 16 <= 65535, "wr_get_mask: more than 65535 planes");
+    WR_ARG(nz <= 65535, "wr_get_mask: more than 65535 planes");
+    WR_ARG(nx <= kMaskMaxRow, "wr_get_mask: x extent above 8192 voxels");
     Context& c = ctx->c;
     WR_HIP_CHECK(hipSetDevice(c.device));
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -620,7 +621,9 @@ int wr_bwareafilt3(wr_ctx* ctx, const uint8_t* in, int nx, int ny, int nz, doubl
     WR_ARG(ctx && in && (out_u8 || out_f32), "wr_bwareafilt3: null argument");
     WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_bwareafilt3: empty volume");
     WR_ARG(static_cast<double>(nx) * ny * nz < 2147483647.0, "wr_bwareafilt3: volume exceeds 2^31 - 1 voxels");
-    WR_ARG(ny <= 65535 && nz <= 65535, "wr_bwareafilt3: more than 65535 rows or planes");
+    WR_ARG(reinterpret_cast<uintptr_t>(in) % 4 == 0 && reinterpret_cast<uintptr_t>(out_u8) % 4 == 0 &&
+               reinterpret_cast<uintptr_t>(out_f32) % 16 == 0,
+           "wr_bwareafilt3: masks must be 4-byte aligned (float32 output 16-byte aligned)");
     Context& c = ctx->c;
     WR_HIP_CHECK(hipSetDevice(c.device));
     hipStream_t s = static_cast<hipStream_t>(stream);

@@ -15,20 +15,27 @@
 // workgroup in a fixed order, then reduced by one workgroup: the result does
 // not depend on scheduling.
 //
-// getMask.  One workgroup per 64 x 16 tile of one z plane stages the z-score
-// test |(v - mu) / sigma| > thres (float32, correctly rounded division) on the
-// tile plus a 3-voxel halo in LDS, then applies the 3x3 in-plane erosion
-// (outside the volume counts as set), dilation and dilation (outside counts
-// as clear): scikit-image's binary_opening followed by binary_dilation with a
-// 3x3x1 footprint.
+// uint16 data skip the radix passes: one pass builds the exact histogram of
+// the 65536 values (16-bit LDS counters, 128 KB per workgroup), from which
+// one workgroup takes the order statistics, the threshold and both moments.
+//
+// getMask.  The z-score test |(v - mu) / sigma| > thres is monotone on each
+// side of mu, so it is replaced by two exact float cut-offs found by
+// bisection.  The 3x3 in-plane erosion (outside the volume counts as set),
+// dilation and dilation (outside counts as clear) -- scikit-image's
+// binary_opening followed by binary_dilation with a 3x3x1 footprint -- are
+// separable and run on 32-voxel bit words: one thread per word of a row, a
+// rolling window of rows, neighbour words through LDS; the volume is read
+// once (plus a 6-row halo per 32 rows) and the byte mask written once.
 //
 // bwareafilt3_wei.  Union-find connected components over the 26-neighbourhood
 // (each voxel unites with its 13 already-scanned neighbours; a root is linked
 // under the smaller label with atomicMin, the method of Playne and Hawick,
 // "A new algorithm for parallel connected-component labelling on GPUs", IEEE
-// TPDS 2018), path flattening, wave-aggregated area counts per root, and the
-// closed area range test.  The output does not depend on which voxel becomes
-// a root.
+// TPDS 2018), path flattening with wave-aggregated area counts per root, and
+// the closed area range test; labels and areas are touched only for
+// foreground voxels.  The output does not depend on which voxel becomes a
+// root.
 
 #include <cmath>
 #include <cstdint>
@@ -250,97 +257,355 @@ __global__ void k_set_musig(SelCtl* c, float mu, float sigma) {
     c->sigma = sigma;
 }
 
+// ------------------------------------------------- uint16 robust statistics
+
+// One pass: the exact histogram of the uint16 values.  A workgroup counts a
+// chunk of at most 65535 values in 16-bit LDS counters packed two per word
+// (128 KB), then adds its non-zero counts to the global histogram.
+constexpr int kHistChunk = 65528;   // a multiple of 8, below 2^16
+
+constexpr int kHistThreads = 1024;   // one workgroup per CU (its LDS): 16 waves of loads in flight
+
+template <bool VEC>
+__global__ __launch_bounds__(kHistThreads) void k_hist16(const uint16_t* __restrict__ in, long long n,
+                                                         unsigned* __restrict__ ghist) {
+    __shared__ unsigned h[kHist16Bins / 2];
+    for (int i = threadIdx.x; i < kHist16Bins / 2; i += kHistThreads) h[i] = 0u;
+    __syncthreads();
+    const long long start = static_cast<long long>(blockIdx.x) * kHistChunk;
+    const long long end = min(n, start + kHistChunk);
+    auto add = [&](unsigned v) { atomicAdd(&h[v >> 1], 1u << ((v & 1u) * 16u)); };
+    long long tail = start;
+    if (VEC) {
+        const long long nv = (end - start) / 8;
+        const uint4* p = reinterpret_cast<const uint4*>(in + start);
+        for (long long k = threadIdx.x; k < nv; k += kHistThreads) {
+            const uint4 q = p[k];
+            add(q.x & 0xffffu); add(q.x >> 16); add(q.y & 0xffffu); add(q.y >> 16);
+            add(q.z & 0xffffu); add(q.z >> 16); add(q.w & 0xffffu); add(q.w >> 16);
+        }
+        tail = start + nv * 8;
+    }
+    for (long long i = tail + threadIdx.x; i < end; i += kHistThreads) add(in[i]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHist16Bins / 2; i += kHistThreads) {
+        const unsigned c = h[i];
+        if (c & 0xffffu) atomicAdd(&ghist[2 * i], c & 0xffffu);
+        if (c >> 16) atomicAdd(&ghist[2 * i + 1], c >> 16);
+    }
+}
+
+__global__ void k_zero(unsigned* p, int n) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// From the histogram: the two order statistics, numpy's float32 lerp, and the
+// mean and std of the values at or below the threshold, each value's float32
+// (v - mu)^2 weighted by its count.  One workgroup of 1024 threads, 64 bins
+// each, fixed-order reductions.
+__global__ __launch_bounds__(1024) void k_stats16(const unsigned* __restrict__ h, long long prev, long long next,
+                                                  float gamma, SelCtl* __restrict__ c) {
+    __shared__ long long sc[1024];
+    __shared__ double sd[1024];
+    __shared__ float s_ab[2], s_thr, s_mu;
+    const int t = threadIdx.x;
+    const int b0 = t * 64;
+    long long local = 0;
+    for (int j = 0; j < 64; ++j) local += h[b0 + j];
+    sc[t] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = t >= off ? sc[t - off] : 0;
+        __syncthreads();
+        sc[t] += v;
+        __syncthreads();
+    }
+    const long long before = t ? sc[t - 1] : 0;
+    const long long want[2] = {prev, next};
+    for (int q = 0; q < 2; ++q) {
+        if (before <= want[q] && want[q] < sc[t]) {
+            long long cum = before;
+            for (int j = 0; j < 64; ++j) {
+                cum += h[b0 + j];
+                if (want[q] < cum) {
+                    s_ab[q] = static_cast<float>(b0 + j);
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        const float a = s_ab[0], b = s_ab[1], d = b - a;
+        s_thr = gamma >= 0.5f ? b - d * (1.0f - gamma) : a + d * gamma;
+        c->thr = s_thr;
+    }
+    __syncthreads();
+    const float thr = s_thr;
+    double sum = 0.0;
+    long long cnt = 0;
+    for (int j = 0; j < 64; ++j) {
+        const float v = static_cast<float>(b0 + j);
+        if (v <= thr) {
+            cnt += h[b0 + j];
+            sum += static_cast<double>(h[b0 + j]) * static_cast<double>(v);
+        }
+    }
+    __syncthreads();
+    sd[t] = sum;
+    sc[t] = cnt;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) {
+            sd[t] += sd[t + w];
+            sc[t] += sc[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        s_mu = static_cast<float>(sd[0] / static_cast<double>(sc[0]));
+        c->mu = s_mu;
+        c->count = sc[0];
+    }
+    __syncthreads();
+    const float mu = s_mu;
+    const long long total = sc[0];
+    double sq = 0.0;
+    for (int j = 0; j < 64; ++j) {
+        const float v = static_cast<float>(b0 + j);
+        if (v <= thr && h[b0 + j]) {
+            const float d = v - mu;
+            sq += static_cast<double>(h[b0 + j]) * static_cast<double>(d * d);
+        }
+    }
+    __syncthreads();
+    sd[t] = sq;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (t < w) sd[t] += sd[t + w];
+        __syncthreads();
+    }
+    if (t == 0) c->sigma = static_cast<float>(sqrt(sd[0] / static_cast<double>(total)));
+}
+
 // ------------------------------------------------------------------ getMask
 
-constexpr int kMX = 64, kMY = 16;
+__device__ __forceinline__ bool zscore_above(float f, float mu, float sigma, float thres) {
+    return fabsf((f - mu) / sigma) > thres;
+}
+
+// g(f) = (f - mu) / sigma is monotone in f (each step is rounded monotonely)
+// for finite mu and finite sigma > 0, so {g > thres} and {g < -thres} are an
+// upper and a lower set of the floats: their bounds, found by bisection over
+// order-preserving keys, turn the test into two comparisons per voxel.
+__global__ void k_cutoffs(SelCtl* c, float thres) {
+    const float mu = c->mu, sigma = c->sigma;
+    const bool ok = isfinite(mu) && isfinite(sigma) && sigma > 0.0f && !isnan(thres);
+    c->direct = ok ? 0 : 1;
+    if (!ok) return;
+    const unsigned k0 = mono_key(-INFINITY), k1 = mono_key(INFINITY);
+    auto g = [&](unsigned k) { return (key_value(k) - mu) / sigma; };
+    // smallest key with g > thres
+    float hi_cut = NAN;
+    if (g(k1) > thres) {
+        unsigned lo = k0, hi = k1;
+        if (g(k0) > thres) {
+            hi = k0;
+        } else {
+            while (hi - lo > 1u) {
+                const unsigned mid = lo + (hi - lo) / 2u;
+                if (g(mid) > thres) hi = mid; else lo = mid;
+            }
+        }
+        hi_cut = key_value(hi);
+    }
+    // largest key with g < -thres
+    float lo_cut = NAN;
+    if (g(k0) < -thres) {
+        unsigned lo = k0, hi = k1;
+        if (g(k1) < -thres) {
+            lo = k1;
+        } else {
+            while (hi - lo > 1u) {
+                const unsigned mid = lo + (hi - lo) / 2u;
+                if (g(mid) < -thres) lo = mid; else hi = mid;
+            }
+        }
+        lo_cut = key_value(lo);
+    }
+    c->cut_lo = lo_cut;
+    c->cut_hi = hi_cut;
+}
+
+// One workgroup holds `strips` strips of kMTY output rows of one z plane;
+// in a strip, thread w owns the 32 columns [32w, 32w + 32) of every row, as
+// one bit word.  Per input row r: the z-score bits T(r), the horizontal
+// 3-erosion He(r) (neighbour words through LDS; outside the volume counts as
+// set), E(r-1) = He(r-2) & He(r-1) & He(r), its horizontal 3-dilation, the
+// vertical one D1(r-2), the same again for D2(r-3), which is written out.
+constexpr int kMTY = 32;
 
 template <int T>
-__global__ __launch_bounds__(kT) void k_getmask(const void* __restrict__ in, int nx, int ny,
-                                                const SelCtl* __restrict__ c, float thres,
-                                                uint8_t* __restrict__ out) {
-    __shared__ uint8_t st[kMY + 6][kMX + 6];
-    __shared__ uint8_t se[kMY + 4][kMX + 4];
-    __shared__ uint8_t sd[kMY + 2][kMX + 2];
-    const int x0 = blockIdx.x * kMX, y0 = blockIdx.y * kMY;
-    const size_t plane = static_cast<size_t>(blockIdx.z) * ny * nx;
-    const float mu = c->mu, sigma = c->sigma;
-    // z-score test on the tile plus a 3-voxel halo
-    for (int q = threadIdx.x; q < (kMY + 6) * (kMX + 6); q += kT) {
-        const int j = q / (kMX + 6), i = q % (kMX + 6);
-        const int x = x0 - 3 + i, y = y0 - 3 + j;
-        uint8_t v = 0;
-        if (x >= 0 && x < nx && y >= 0 && y < ny) {
-            const float f = load_f32<T>(in, plane + static_cast<size_t>(y) * nx + x);
-            v = fabsf((f - mu) / sigma) > thres;
-        }
-        st[j][i] = v;
-    }
-    __syncthreads();
-    // erosion: neighbours outside the volume count as set
-    for (int q = threadIdx.x; q < (kMY + 4) * (kMX + 4); q += kT) {
-        const int j = q / (kMX + 4), i = q % (kMX + 4);
-        const int x = x0 - 2 + i, y = y0 - 2 + j;
-        uint8_t v = 0;
-        if (x >= 0 && x < nx && y >= 0 && y < ny) {
-            v = 1;
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int xx = x + dx, yy = y + dy;
-                    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny) v &= st[j + 1 + dy][i + 1 + dx];
+__device__ __forceinline__ unsigned zscore_word(const void* in, size_t off, int nvalid, bool vec, bool direct,
+                                                float mu, float sigma, float thres, float lo_cut,
+                                                float hi_cut) {
+    auto test = [&](float f) {
+        return direct ? zscore_above(f, mu, sigma, thres) : (f <= lo_cut || f >= hi_cut);
+    };
+    unsigned w = 0u;
+    if (vec && nvalid == 32) {
+        if constexpr (T == E_U16) {
+            const uint4* p = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(in) + off);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint4 v = p[q];
+                const unsigned u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    w |= static_cast<unsigned>(test(static_cast<float>(u[e] & 0xffffu))) << (8 * q + 2 * e);
+                    w |= static_cast<unsigned>(test(static_cast<float>(u[e] >> 16))) << (8 * q + 2 * e + 1);
                 }
-        }
-        se[j][i] = v;
-    }
-    __syncthreads();
-    // first dilation: neighbours outside the volume count as clear
-    for (int q = threadIdx.x; q < (kMY + 2) * (kMX + 2); q += kT) {
-        const int j = q / (kMX + 2), i = q % (kMX + 2);
-        const int x = x0 - 1 + i, y = y0 - 1 + j;
-        uint8_t v = 0;
-        if (x >= 0 && x < nx && y >= 0 && y < ny)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int xx = x + dx, yy = y + dy;
-                    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny) v |= se[j + 1 + dy][i + 1 + dx];
-                }
-        sd[j][i] = v;
-    }
-    __syncthreads();
-    // second dilation, written out
-    for (int q = threadIdx.x; q < kMY * kMX; q += kT) {
-        const int j = q / kMX, i = q % kMX;
-        const int x = x0 + i, y = y0 + j;
-        if (x >= nx || y >= ny) continue;
-        uint8_t v = 0;
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = x + dx, yy = y + dy;
-                if (xx >= 0 && xx < nx && yy >= 0 && yy < ny) v |= sd[j + 1 + dy][i + 1 + dx];
             }
-        out[plane + static_cast<size_t>(y) * nx + x] = v;
+        } else {
+            const float4* p = reinterpret_cast<const float4*>(static_cast<const float*>(in) + off);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float4 v = p[q];
+                w |= static_cast<unsigned>(test(v.x)) << (4 * q);
+                w |= static_cast<unsigned>(test(v.y)) << (4 * q + 1);
+                w |= static_cast<unsigned>(test(v.z)) << (4 * q + 2);
+                w |= static_cast<unsigned>(test(v.w)) << (4 * q + 3);
+            }
+        }
+    } else {
+        for (int b = 0; b < nvalid; ++b) w |= static_cast<unsigned>(test(load_f32<T>(in, off + b))) << b;
+    }
+    return w;
+}
+
+__device__ __forceinline__ void write_bits(uint8_t* out, unsigned bits, int nvalid, bool vec) {
+    if (vec && nvalid == 32) {
+        unsigned q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const unsigned nb = (bits >> (4 * k)) & 15u;
+            q[k] = (nb & 1u) | ((nb & 2u) << 7) | ((nb & 4u) << 14) | ((nb & 8u) << 21);
+        }
+        uint4* o = reinterpret_cast<uint4*>(out);
+        o[0] = make_uint4(q[0], q[1], q[2], q[3]);
+        o[1] = make_uint4(q[4], q[5], q[6], q[7]);
+    } else {
+        for (int b = 0; b < nvalid; ++b) out[b] = static_cast<uint8_t>((bits >> b) & 1u);
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(kT) void k_getmask_bits(const void* __restrict__ in, int nx, int ny, int W,
+                                                     int strips, bool vec_in, bool vec_out,
+                                                     const SelCtl* __restrict__ c, float thres,
+                                                     uint8_t* __restrict__ out) {
+    __shared__ unsigned ex[3][3 * kT];
+    const int t = threadIdx.x;
+    const int s = t / W, w = t - s * W;
+    const bool act = s < strips;
+    const size_t plane = static_cast<size_t>(blockIdx.y) * ny * nx;
+    const int ys = (blockIdx.x * strips + s) * kMTY;
+    const int slot = s * (W + 2) + w + 1;
+    const int x0 = 32 * w;
+    const int nvalid = act ? min(32, nx - x0) : 32;
+    const unsigned valid = nvalid >= 32 ? ~0u : ((1u << nvalid) - 1u);
+    const bool direct = c->direct != 0;
+    const float mu = c->mu, sigma = c->sigma, lo_cut = c->cut_lo, hi_cut = c->cut_hi;
+    unsigned he0 = 0u, he1 = 0u, hd0 = 0u, hd1 = 0u, hq0 = 0u, hq1 = 0u;
+    for (int step = 0; step < kMTY + 6; ++step) {
+        const int r = ys - 3 + step;
+        unsigned tw = ~0u;
+        if (act && r >= 0 && r < ny)
+            tw = zscore_word<T>(in, plane + static_cast<size_t>(r) * nx + x0, nvalid, vec_in, direct, mu, sigma,
+                                thres, lo_cut, hi_cut) | ~valid;
+        if (act) ex[0][slot] = tw;
+        __syncthreads();
+        unsigned lw = ~0u, rw = ~0u;
+        if (act) {
+            if (w > 0) lw = ex[0][slot - 1];
+            if (w < W - 1) rw = ex[0][slot + 1];
+        }
+        const unsigned he = tw & ((tw << 1) | (lw >> 31)) & ((tw >> 1) | (rw << 31));
+        const int re = r - 1;
+        const unsigned e = (re >= 0 && re < ny) ? (he0 & he1 & he) & valid : 0u;
+        he0 = he1;
+        he1 = he;
+        if (act) ex[1][slot] = e;
+        __syncthreads();
+        lw = rw = 0u;
+        if (act) {
+            if (w > 0) lw = ex[1][slot - 1];
+            if (w < W - 1) rw = ex[1][slot + 1];
+        }
+        const unsigned hd = e | (e << 1) | (lw >> 31) | (e >> 1) | (rw << 31);
+        const int rd = r - 2;
+        const unsigned d1 = (rd >= 0 && rd < ny) ? (hd0 | hd1 | hd) & valid : 0u;
+        hd0 = hd1;
+        hd1 = hd;
+        if (act) ex[2][slot] = d1;
+        __syncthreads();
+        lw = rw = 0u;
+        if (act) {
+            if (w > 0) lw = ex[2][slot - 1];
+            if (w < W - 1) rw = ex[2][slot + 1];
+        }
+        const unsigned hq = d1 | (d1 << 1) | (lw >> 31) | (d1 >> 1) | (rw << 31);
+        const unsigned d2 = hq0 | hq1 | hq;
+        hq0 = hq1;
+        hq1 = hq;
+        const int ro = r - 3;
+        if (act && step >= 6 && ro < ny)
+            write_bits(out + plane + static_cast<size_t>(ro) * nx + x0, d2 & valid, nvalid, vec_out);
     }
 }
 
 // ------------------------------------------------------ connected components
+//
+// The streaming kernels (init, filter) take four voxels (one 32-bit word of
+// mask bytes) per thread and skip empty words; the union and flattening
+// kernels take one voxel per thread, to hide the latency of their tree walks.
+// Labels and areas are only written and read for foreground voxels.
 
-__device__ __forceinline__ int ld_label(const int* L, int i) {
-    return __hip_atomic_load(L + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// Plain loads: a stale parent read from a cache is still an ancestor of the
+// voxel (parents only ever move to smaller labels of the same set), and the
+// atomicMin results drive unite's progress, so the walks stay correct.
 __device__ __forceinline__ int find_root(const int* L, int x) {
-    int p = ld_label(L, x);
+    int p = L[x];
     while (p != x) {       // labels only point to smaller indices: terminates
         x = p;
-        p = ld_label(L, x);
+        p = L[x];
     }
     return x;
 }
 
-// Link the trees of a and b: the larger root goes under the smaller.
+// find_root with path halving: every visited voxel is re-linked to its
+// grandparent, an ancestor in any interleaving (a link made by a successful
+// atomicMin is never raised again: a voxel's parent is only rewritten to an
+// ancestor of the parent it was read with), so the trees stay shallow while
+// the unions run.
+__device__ __forceinline__ int find_halve(int* L, int x) {
+    for (;;) {
+        const int p = L[x];
+        if (p == x) return x;
+        const int g = L[p];
+        if (g == p) return p;
+        L[x] = g;
+        x = g;
+    }
+}
+
+// Link the trees of a and b: the larger root goes under the smaller.  Each
+// retry lowers a + b, so the loop ends.
 __device__ void unite(int* L, int a, int b) {
     for (;;) {
-        a = find_root(L, a);
-        b = find_root(L, b);
+        a = find_halve(L, a);
+        b = find_halve(L, b);
         if (a == b) return;
         if (a < b) {
             const int t = a;
@@ -353,55 +618,87 @@ __device__ void unite(int* L, int a, int b) {
     }
 }
 
-__global__ __launch_bounds__(kT) void k_ccl_init(const uint8_t* __restrict__ m, int n, int* __restrict__ L,
-                                                 int* __restrict__ cnt, unsigned long long* __restrict__ ncomp) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i == 0) *ncomp = 0ull;
-    if (i >= n) return;
-    L[i] = m[i] ? i : -1;
-    cnt[i] = 0;
+__device__ __forceinline__ unsigned mask_word(const uint8_t* m, int i0, int n) {
+    if (i0 + 4 <= n) return *reinterpret_cast<const unsigned*>(m + i0);
+    unsigned w = 0u;
+    for (int b = 0; i0 + b < n; ++b) w |= static_cast<unsigned>(m[i0 + b]) << (8 * b);
+    return w;
 }
 
-// Each foreground voxel unites with its 13 foreground neighbours that come
-// earlier in (z, y, x) order: together they cover all 26 neighbour pairs.
+__device__ __forceinline__ bool fg_of(unsigned word, int b) { return ((word >> (8 * b)) & 0xffu) != 0u; }
+
+__global__ __launch_bounds__(kT) void k_ccl_init(const uint8_t* __restrict__ m, int n, int* __restrict__ L,
+                                                 int* __restrict__ cnt, unsigned long long* __restrict__ ncomp) {
+    const int q = blockIdx.x * kT + threadIdx.x;
+    if (q == 0) *ncomp = 0ull;
+    const int i0 = 4 * q;
+    if (i0 >= n) return;
+    const unsigned word = mask_word(m, i0, n);
+    if (!word) return;
+    for (int b = 0; b < 4; ++b)
+        if (fg_of(word, b)) {
+            L[i0 + b] = i0 + b;
+            cnt[i0 + b] = 0;
+        }
+}
+
+// Each foreground voxel unites with enough of its 13 earlier neighbours in
+// (z, y, x) order that every 26-adjacent pair ends up connected.  A skipped
+// neighbour q is always 26-adjacent to a neighbour r that is united, and the
+// pair (q, r) lies earlier in the scan, so it is connected by induction:
+//   plane z - 1: the voxel below if set (its 8 in-plane neighbours hang on
+//     it); otherwise the set edge voxels of the 3x3, and a set corner only
+//     when both edges beside it are clear;
+//   plane z: the left voxel if set, else the one above if set; the upper
+//     corners only when neither covers them (upper-left needs left and above
+//     clear, upper-right needs above clear).
 __global__ __launch_bounds__(kT) void k_ccl_merge(const uint8_t* __restrict__ m, int nx, int ny, int nz,
                                                   int* __restrict__ L) {
-    const int x = blockIdx.x * kT + threadIdx.x;
-    const int y = blockIdx.y, z = blockIdx.z;
-    if (x >= nx) return;
-    const int i = (z * ny + y) * nx + x;
-    if (!m[i]) return;
-    for (int dz = -1; dz <= 0; ++dz) {
-        const int zz = z + dz;
-        if (zz < 0) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            if (dz == 0 && dy > 0) break;
-            const int yy = y + dy;
-            if (yy < 0 || yy >= ny) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                if (dz == 0 && dy == 0 && dx >= 0) break;
-                const int xx = x + dx;
-                if (xx < 0 || xx >= nx) continue;
-                const int j = (zz * ny + yy) * nx + xx;
-                if (m[j]) unite(L, i, j);
-            }
+    const int n = nx * ny * nz;
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n || !m[i]) return;
+    const int x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+    const bool xl = x > 0, xr = x < nx - 1, yu = y > 0, yd = y < ny - 1;
+    if (z > 0) {
+        const int b = i - nx * ny;
+        if (m[b]) {
+            unite(L, i, b);
+        } else {
+            const bool eu = yu && m[b - nx], ed = yd && m[b + nx], el = xl && m[b - 1], er = xr && m[b + 1];
+            if (eu) unite(L, i, b - nx);
+            if (ed) unite(L, i, b + nx);
+            if (el) unite(L, i, b - 1);
+            if (er) unite(L, i, b + 1);
+            if (yu && xl && !eu && !el && m[b - nx - 1]) unite(L, i, b - nx - 1);
+            if (yu && xr && !eu && !er && m[b - nx + 1]) unite(L, i, b - nx + 1);
+            if (yd && xl && !ed && !el && m[b + nx - 1]) unite(L, i, b + nx - 1);
+            if (yd && xr && !ed && !er && m[b + nx + 1]) unite(L, i, b + nx + 1);
         }
+    }
+    const bool left = xl && m[i - 1];
+    const bool up = yu && m[i - nx];
+    if (left) unite(L, i, i - 1);
+    else if (up) unite(L, i, i - nx);
+    if (!up) {
+        if (!left && xl && yu && m[i - nx - 1]) unite(L, i, i - nx - 1);
+        if (xr && yu && m[i - nx + 1]) unite(L, i, i - nx + 1);
     }
 }
 
-__global__ __launch_bounds__(kT) void k_ccl_flatten(int n, int* __restrict__ L) {
+// Flatten every foreground voxel onto its root and count the areas: one
+// atomic per distinct root per wave.
+__global__ __launch_bounds__(kT) void k_ccl_resolve(const uint8_t* __restrict__ m, int n, int* __restrict__ L,
+                                                    int* __restrict__ cnt, unsigned long long* __restrict__ ncomp) {
     const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n) return;
-    if (L[i] >= 0) L[i] = find_root(L, i);
-}
-
-// Area per root: one atomic per distinct root per wave.
-__global__ __launch_bounds__(kT) void k_ccl_count(const int* __restrict__ L, int n, int* __restrict__ cnt,
-                                                  unsigned long long* __restrict__ ncomp) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    int lab = i < n ? L[i] : -1;
-    if (lab >= 0 && lab == i) atomicAdd(ncomp, 1ull);
+    const bool fg = i < n && m[i];
+    if (!__ballot(fg)) return;   // uniform: no foreground in this wave
     const int lane = threadIdx.x & 63;
+    int lab = -1;
+    if (fg) {
+        lab = find_root(L, i);
+        L[i] = lab;
+        if (lab == i) atomicAdd(ncomp, 1ull);
+    }
     unsigned long long active = __ballot(lab >= 0);
     while (active) {
         const int leader = __ffsll(static_cast<long long>(active)) - 1;
@@ -413,19 +710,33 @@ __global__ __launch_bounds__(kT) void k_ccl_count(const int* __restrict__ L, int
     }
 }
 
-__global__ __launch_bounds__(kT) void k_ccl_filter(const int* __restrict__ L, const int* __restrict__ cnt, int n,
-                                                   double lo, double hi, uint8_t* __restrict__ o8,
-                                                   float* __restrict__ of) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n) return;
-    const int lab = L[i];
-    bool keep = false;
-    if (lab >= 0) {
-        const double a = static_cast<double>(cnt[lab]);
-        keep = lo <= a && a <= hi;
+__global__ __launch_bounds__(kT) void k_ccl_filter(const uint8_t* __restrict__ m, const int* __restrict__ L,
+                                                   const int* __restrict__ cnt, int n, double lo, double hi,
+                                                   uint8_t* __restrict__ o8, float* __restrict__ of) {
+    const int i0 = 4 * (blockIdx.x * kT + threadIdx.x);
+    if (i0 >= n) return;
+    const unsigned word = mask_word(m, i0, n);
+    bool keep[4] = {false, false, false, false};
+    if (word)
+        for (int b = 0; b < 4; ++b)
+            if (fg_of(word, b)) {
+                const double a = static_cast<double>(cnt[L[i0 + b]]);
+                keep[b] = lo <= a && a <= hi;
+            }
+    if (i0 + 4 <= n) {
+        if (o8)
+            *reinterpret_cast<unsigned*>(o8 + i0) = static_cast<unsigned>(keep[0]) | (static_cast<unsigned>(keep[1]) << 8) |
+                                                    (static_cast<unsigned>(keep[2]) << 16) |
+                                                    (static_cast<unsigned>(keep[3]) << 24);
+        if (of)
+            *reinterpret_cast<float4*>(of + i0) = make_float4(keep[0] ? 1.0f : 0.0f, keep[1] ? 1.0f : 0.0f,
+                                                              keep[2] ? 1.0f : 0.0f, keep[3] ? 1.0f : 0.0f);
+    } else {
+        for (int b = 0; i0 + b < n; ++b) {
+            if (o8) o8[i0 + b] = keep[b];
+            if (of) of[i0 + b] = keep[b] ? 1.0f : 0.0f;
+        }
     }
-    if (o8) o8[i] = keep;
-    if (of) of[i] = keep ? 1.0f : 0.0f;
 }
 
 // ----------------------------------------------------- mean squared difference
@@ -545,34 +856,51 @@ void launch_robust_stats(const void* in, Elem t, long long n, double percentile,
     long long prev, next;
     float gamma;
     percentile_index(n, percentile, &prev, &next, &gamma);
-    if (t == E_U16)
-        robust_t<E_U16>(in, n, prev, next, gamma, c, hist, part, cpart, s);
-    else
+    if (t == E_U16) {
+        // one pass: the exact histogram of the 65536 values
+        k_zero<<<kHist16Bins / kT, kT, 0, s>>>(hist, kHist16Bins);
+        const unsigned g = static_cast<unsigned>((n + kHistChunk - 1) / kHistChunk);
+        const uint16_t* p = static_cast<const uint16_t*>(in);
+        if (reinterpret_cast<uintptr_t>(p) % 16 == 0)
+            k_hist16<true><<<g, kHistThreads, 0, s>>>(p, n, hist);
+        else
+            k_hist16<false><<<g, kHistThreads, 0, s>>>(p, n, hist);
+        k_stats16<<<1, 1024, 0, s>>>(hist, prev, next, gamma, c);
+    } else {
         robust_t<E_F32>(in, n, prev, next, gamma, c, hist, part, cpart, s);
+    }
 }
 
 void launch_set_musig(SelCtl* c, float mu, float sigma, hipStream_t s) {
     k_set_musig<<<1, 1, 0, s>>>(c, mu, sigma);
 }
 
-void launch_getmask(const void* in, Elem t, int nx, int ny, int nz, const SelCtl* c, float thres, uint8_t* out,
+void launch_getmask(const void* in, Elem t, int nx, int ny, int nz, SelCtl* c, float thres, uint8_t* out,
                     hipStream_t s) {
-    const dim3 grid((nx + kMX - 1) / kMX, (ny + kMY - 1) / kMY, nz);
+    k_cutoffs<<<1, 1, 0, s>>>(c, thres);
+    const int W = (nx + 31) / 32;
+    const int strips = kT / W;
+    const dim3 grid((ny + strips * kMTY - 1) / (strips * kMTY), nz);
+    const bool in16 = reinterpret_cast<uintptr_t>(in) % 16 == 0;
+    const bool vec_out = nx % 16 == 0 && reinterpret_cast<uintptr_t>(out) % 16 == 0;
     if (t == E_U16)
-        k_getmask<E_U16><<<grid, kT, 0, s>>>(in, nx, ny, c, thres, out);
+        k_getmask_bits<E_U16><<<grid, kT, 0, s>>>(in, nx, ny, W, strips, in16 && nx % 8 == 0, vec_out, c, thres,
+                                                  out);
     else
-        k_getmask<E_F32><<<grid, kT, 0, s>>>(in, nx, ny, c, thres, out);
+        k_getmask_bits<E_F32><<<grid, kT, 0, s>>>(in, nx, ny, W, strips, in16 && nx % 4 == 0, vec_out, c, thres,
+                                                  out);
 }
 
 void launch_components(const uint8_t* m, int nx, int ny, int nz, double lo, double hi, int* L, int* cnt,
                        unsigned long long* ncomp, uint8_t* o8, float* of, hipStream_t s) {
     const int n = nx * ny * nz;
-    const unsigned g = static_cast<unsigned>((n + kT - 1) / kT);
+    const long long words = (static_cast<long long>(n) + 3) / 4;
+    const unsigned g = static_cast<unsigned>((words + kT - 1) / kT);
+    const unsigned g1 = static_cast<unsigned>((static_cast<long long>(n) + kT - 1) / kT);
     k_ccl_init<<<g, kT, 0, s>>>(m, n, L, cnt, ncomp);
-    k_ccl_merge<<<dim3((nx + kT - 1) / kT, ny, nz), kT, 0, s>>>(m, nx, ny, nz, L);
-    k_ccl_flatten<<<g, kT, 0, s>>>(n, L);
-    k_ccl_count<<<g, kT, 0, s>>>(L, n, cnt, ncomp);
-    k_ccl_filter<<<g, kT, 0, s>>>(L, cnt, n, lo, hi, o8, of);
+    k_ccl_merge<<<g1, kT, 0, s>>>(m, nx, ny, nz, L);
+    k_ccl_resolve<<<g1, kT, 0, s>>>(m, n, L, cnt, ncomp);
+    k_ccl_filter<<<g, kT, 0, s>>>(m, L, cnt, n, lo, hi, o8, of);
 }
 
 void launch_mean_sq_diff(const void* a, Elem ta, const void* b, Elem tb, long long n, double* part, double* out,

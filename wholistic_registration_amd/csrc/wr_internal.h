@@ -242,7 +242,14 @@ struct SelCtl {
     float gamma;             // interpolation weight
     float thr, mu, sigma;    // percentile, mean and std of the values <= thr
     long long count;         // number of values <= thr
+    // the z-score test |(v - mu) / sigma| > thres as v <= cut_lo || v >= cut_hi
+    // (NaN cut-off = empty side); direct = 1 evaluates the test itself
+    float cut_lo, cut_hi;
+    int direct;
 };
+
+// Counters of the uint16 histogram of the robust statistics (one per value).
+constexpr int kHist16Bins = 65536;
 
 // Workgroups of the grid-stride streaming reductions (fixed: their partials
 // are reduced in a fixed order).
@@ -251,13 +258,15 @@ constexpr int kStreamBlocks = 2048;
 // numpy.percentile's float32 index arithmetic (linear method).
 void percentile_index(long long n, double percentile, long long* prev, long long* next, float* gamma);
 // robust_mean_std of n float32 or uint16 values (as float32): fills c->thr,
-// c->mu, c->sigma.  hist holds 3 * 2048 counters, part and cpart
-// kStreamBlocks partials each.
+// c->mu, c->sigma.  hist holds kHist16Bins counters (3 * 2048 of them for
+// float32 input), part and cpart kStreamBlocks partials each.
 void launch_robust_stats(const void* in, Elem t, long long n, double percentile, SelCtl* c, unsigned* hist,
                          double* part, long long* cpart, hipStream_t s);
 void launch_set_musig(SelCtl* c, float mu, float sigma, hipStream_t s);
+// Widest row of the getMask kernel (256 words of 32 voxels).
+constexpr int kMaskMaxRow = 8192;
 // getMask on a (nz, ny, nx) volume with c->mu, c->sigma: out 0/1 bytes.
-void launch_getmask(const void* in, Elem t, int nx, int ny, int nz, const SelCtl* c, float thres, uint8_t* out,
+void launch_getmask(const void* in, Elem t, int nx, int ny, int nz, SelCtl* c, float thres, uint8_t* out,
                     hipStream_t s);
 // bwareafilt3_wei: 26-connected components of the nonzero bytes of m with an
 // area in [lo, hi]; L and cnt hold one int per voxel; ncomp receives the
