@@ -113,7 +113,7 @@ def algorithmic_bytes(w, records, s_in, s_p=4, s_m=1):
     return tot
 
 
-def crop_checks(tmpl, mov, mask, w, crop, want_cpu, want_epe):
+def crop_checks(tmpl, mov, mask, w, crop, want_cpu, want_epe, precisions=()):
     """On the centre crop of the pair: time the oracle in the reference's
     dtype flow (the CPU baseline, one thread: scipy.ndimage is
     single-threaded), and compare the GPU field with the all-float64 oracle
@@ -151,13 +151,53 @@ def crop_checks(tmpl, mov, mask, w, crop, want_cpu, want_epe):
         dev = tmpl.device
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(dev)
         mref_t = t(mk.astype(np.float32)) if mk.any() else None
-        res = D.get_motion(t(mv), t(ref), mask_ref=mref_t, layer=w["layer"], iter=w["iter"],
-                           r=w["r"], zRatio=1.0, movRange=5.0, smoothPenalty=sp, tol=1e-3)
-        got = res.motion.cpu().numpy().transpose(2, 1, 0, 3)
-        out["epe_vs_fp64"] = dict(epe_stats(got, m64),
-                                  sample=f"GPU float32 path vs all-float64 oracle on the centre "
-                                         f"crop {cz}x{cy}x{cx} (z,y,x), both free-running")
+        for prec in ("fp32",) + tuple(precisions):
+            res = D.get_motion(t(mv), t(ref), mask_ref=mref_t, layer=w["layer"], iter=w["iter"],
+                               r=w["r"], zRatio=1.0, movRange=5.0, smoothPenalty=sp, tol=1e-3,
+                               precision=prec)
+            got = res.motion.cpu().numpy().transpose(2, 1, 0, 3)
+            key = "epe_vs_fp64" if prec == "fp32" else f"epe_vs_fp64_{prec}"
+            out[key] = dict(epe_stats(got, m64),
+                            sample=f"GPU {prec} level images vs all-float64 oracle on the centre "
+                                   f"crop {cz}x{cy}x{cx} (z,y,x), both free-running")
     return out
+
+
+def precision_measure(w, n0, step, motion_fp32, field, mask, steps, prec):
+    """The getMotion + correctMotion step with bfloat16 / float16 level
+    images: throughput, the level-0 LK pass roofline (2-byte template and
+    moving image plus the mask byte per voxel), and the endpoint error of its
+    field against the float32 field and the generating field."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    res = step(False, prec)          # warm-up (and workspace) for this storage
+    torch.cuda.synchronize()
+    lk_ms = lk_n = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = step(True, prec)
+        st = D.stats()
+        lk_ms += st.lk_ms[0]
+        lk_n += st.lk_exec[0]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    motion = res.motion
+    e = torch.sqrt(((motion - motion_fp32) ** 2).sum(-1))
+    keep = mask <= 0 if mask is not None else torch.ones_like(e, dtype=torch.bool)
+    ek = e[keep].float()
+    bytes_per_launch = (2 * 2 + (1 if mask is not None else 0)) * n0
+    avg_ms = lk_ms / max(lk_n, 1)
+    return {
+        "value": n0 / dt, "ms_per_step": 1e3 * dt,
+        "records_per_level": {f"L{L}": res.records[L] for L in range(w["layer"], -1, -1)},
+        "roofline": {"kernel": "k_lk_pass (level 0)", "achieved": bytes_per_launch / (avg_ms * 1e-3) / 1e9,
+                     "unit": "GB/s", "frac": bytes_per_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch},
+        "epe_vs_fp32_field": {"epe_mean": float(ek.mean()), "epe_max": float(ek.max()),
+                              "sample": "full volume outside the mask"},
+        "epe_vs_truth": truth_epe(motion, field, mask),
+    }
 
 
 def mask_measure(vol, crop, reps=3):
@@ -237,6 +277,9 @@ def main():
                     help="skip the float64-oracle endpoint-error check on the crop")
     ap.add_argument("--cpu-crop", default="8,512,1024", help="z,y,x of the CPU-baseline crop")
     ap.add_argument("--tile-patches-x", type=int, default=0)
+    ap.add_argument("--precisions", default="bf16,fp16",
+                    help="reduced-precision level-image variants to time beside the float32 path "
+                         "(comma list of bf16, fp16; 'none' to skip)")
     ap.add_argument("--no-mask-measure", action="store_true",
                     help="skip timing the harness's automatic mask kernels")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -277,9 +320,10 @@ def main():
     params = dict(layer=w["layer"], iter=w["iter"], r=w["r"], zRatio=1.0, movRange=5.0,
                   smoothPenalty=sp, tol=1e-3, tile_patches_x=args.tile_patches_x)
 
-    def step(timing):
-        res = D.get_motion(mov, tmpl, mask_ref=mask, motion_out=motion, timing=timing, **params)
-        D.correct_motion(mov, motion, out=warped)
+    def step(timing, precision="fp32", motion_out=motion):
+        res = D.get_motion(mov, tmpl, mask_ref=mask, motion_out=motion_out, timing=timing,
+                           precision=precision, **params)
+        D.correct_motion(mov, motion_out, out=warped)
         return res
 
     for i in range(args.warmup):
@@ -332,6 +376,17 @@ def main():
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
         out["roofline"]["traffic_source"] = traffic["source"]
     out["epe_vs_truth"] = truth_epe(motion, field, mask)
+    precisions = [] if args.precisions == "none" else args.precisions.split(",")
+    if precisions:
+        variants = {}
+        m2 = torch.empty_like(motion)
+        for prec in precisions:
+            log(f"[bench] {prec} level images")
+            variants[prec] = precision_measure(w, n0, lambda t, p: step(t, p, m2), motion, field, mask,
+                                               args.steps, prec)
+        out["precision_variants"] = dict(variants, note=(
+            "same workload and step with the pyramid level images stored as bfloat16 / float16; "
+            "interpolation, sums and solves stay float32 / float64; value = registered voxels/s"))
     if not args.no_mask_measure:
         log("[bench] harness mask of the moving volume")
         out["harness_masks"] = mask_measure(mov, tuple(int(v) for v in args.cpu_crop.split(",")))
@@ -339,7 +394,7 @@ def main():
         crop = tuple(int(v) for v in args.cpu_crop.split(","))
         log(f"[bench] oracle runs on crop {crop}")
         out.update(crop_checks(tmpl, mov, mask, w, crop, not args.no_cpu_baseline,
-                               not args.no_epe_check))
+                               not args.no_epe_check, precisions))
     print(json.dumps(out), flush=True)
     _ = _lib, math
 

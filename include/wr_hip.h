@@ -69,6 +69,10 @@ typedef struct wr_params {
     int tile_patches_x;     /* 0 = autotuned; patches per workgroup along x   */
     int stop_kernels;       /* 0 = automatic; 1 = the stop rules always run in
                                their own single-workgroup kernels (testing)   */
+    int precision;          /* storage of the pyramid level images the LK
+                               iterations read: 0 = float32 (the reference's
+                               values), 1 = bfloat16, 2 = float16; every sum,
+                               interpolation and solve stays float32/float64 */
 } wr_params;
 
 /* Per-level iteration log, filled on the host when wr_get_motion returns

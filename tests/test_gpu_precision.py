@@ -1,0 +1,68 @@
+"""Reduced-precision storage of the pyramid level images (bfloat16 / float16,
+wr_params.precision): the endpoint-error gate against the all-float64 oracle.
+
+Both runs are free-running (each takes its own iteration schedule).  The gate
+is the workload definition's: mean endpoint error <= 0.1 voxel against the
+float64 oracle; the float32 path on the same inputs is reported beside it.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import lk_oracle as O
+from oracle.metrics import epe_stats
+from tests.golden.cases import config0_inputs, small_inputs
+
+pytestmark = pytest.mark.gpu
+
+EPE_GATE = 0.1
+
+
+def _run(mov, ref, opt, cuda, precision):
+    import torch
+
+    from wholistic_registration_amd import device as D
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).transpose(2, 1, 0))).to(cuda)  # noqa: E731
+    mr = np.asarray(opt["mask_ref"], dtype=np.float32)
+    mm = np.asarray(opt["mask_mov"], dtype=np.float32)
+    init = opt.get("motion")
+    res = D.get_motion(t(mov), t(ref), layer=opt["layer"], iter=opt["iter"], r=opt["r"], zRatio=opt["zRatio"],
+                       movRange=opt["movRange"], smoothPenalty=float(opt["smoothPenalty"]), tol=opt["tol"],
+                       mask_ref=t(mr) if mr.any() else None, mask_mov=t(mm) if mm.any() else None,
+                       motion_init=(torch.from_numpy(np.ascontiguousarray(
+                           np.asarray(init, dtype=np.float32).transpose(2, 1, 0, 3))).to(cuda)
+                           if init is not None and np.any(init) else None),
+                       precision=precision)
+    return res.motion.cpu().numpy().transpose(2, 1, 0, 3)
+
+
+@pytest.mark.parametrize("case", ["plain", "masked", "odd", "config0"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_reduced_precision_epe_gate(cuda, case, precision):
+    mov, ref, opt = config0_inputs() if case == "config0" else small_inputs(case)
+    m64, _, _, _ = O.get_motion(mov, ref, dict(opt), mode="fp64", keep_images=False)
+    got = _run(mov, ref, opt, cuda, precision)
+    f32 = _run(mov, ref, opt, cuda, "fp32")
+    e = epe_stats(got, m64)
+    e32 = epe_stats(f32, m64)
+    print(f"{case} {precision}: EPE vs fp64 mean {e['epe_mean']:.2e} max {e['epe_max']:.2e}; "
+          f"fp32 path mean {e32['epe_mean']:.2e}")
+    assert np.isfinite(got).all()
+    assert e["epe_mean"] <= EPE_GATE
+
+
+def test_fp32_precision_is_the_default(cuda):
+    """precision="fp32" is the unchanged float32 path (bit for bit)."""
+    mov, ref, opt = small_inputs("plain")
+    a = _run(mov, ref, opt, cuda, "fp32")
+    from wholistic_registration_amd import calFlow3d_Wei_v1 as wr
+    b, _, _, _ = wr.getMotion(mov, ref, dict(opt))
+    assert np.array_equal(a, np.asarray(b))
+
+
+def test_unknown_precision_rejected(cuda):
+    mov, ref, opt = small_inputs("plain")
+    with pytest.raises(ValueError):
+        _run(mov, ref, opt, cuda, "fp8")

@@ -34,6 +34,7 @@ class WrParams(ctypes.Structure):
         ("forced_updates", ctypes.POINTER(ctypes.c_int)),
         ("tile_patches_x", ctypes.c_int),
         ("stop_kernels", ctypes.c_int),
+        ("precision", ctypes.c_int),
     ]
 
 

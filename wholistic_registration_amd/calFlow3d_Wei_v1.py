@@ -144,7 +144,8 @@ def getMotion(dat_mov, dat_ref, option, verbose=False):
                           zRatio=option["zRatio"], movRange=option["movRange"],
                           smoothPenalty=option["smoothPenalty"], tol=option["tol"],
                           mask_ref=mref, mask_mov=mmov, motion_init=init, want_coords=True,
-                          schedule=option.get("forced_schedule"))
+                          schedule=option.get("forced_schedule"),
+                          precision=option.get("precision", "fp32"))
     log = res.error_log()
     for L in range(res.layer, -1, -1):
         if verbose:

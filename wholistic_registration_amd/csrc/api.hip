@@ -291,7 +291,10 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     WR_ARG(2 * r + 1 <= kMaxTileCols, "wr_get_motion: r above 123 (patch wider than one workgroup tile)");
     WR_ARG(prm->zRatio != 0.0 && std::isfinite(prm->zRatio), "wr_get_motion: zRatio must be finite, non-zero");
     WR_ARG(prm->movRange > 0.0, "wr_get_motion: movRange must be positive");
+    WR_ARG(prm->precision >= 0 && prm->precision <= 2, "wr_get_motion: precision must be 0, 1 or 2");
     const int p = 2 * r + 1;
+    const int st = prm->precision;        // storage of the level images (Store)
+    const size_t es = static_cast<size_t>(store_bytes(st));
 
     Context& c = ctx->c;
     WR_HIP_CHECK(hipSetDevice(c.device));
@@ -368,8 +371,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         ref_off[L] = ref_tot;
         ref_tot += (static_cast<size_t>(geo[L].nx) * geo[L].ny * nz + 255) & ~static_cast<size_t>(255);
     }
-    WR_TRY(grow(c.mov_l, mov_tot * sizeof(float)));
-    WR_TRY(grow(c.ref_l, ref_tot * sizeof(float)));
+    WR_TRY(grow(c.mov_l, mov_tot * es));
+    WR_TRY(grow(c.ref_l, ref_tot * es));
     if (mask_ref) WR_TRY(grow(c.mref_l, ref_tot));
     if (mask_mov) WR_TRY(grow(c.mmov_l, mov_tot * sizeof(float)));
     if (!c.side) WR_HIP_CHECK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
@@ -416,18 +419,19 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     hipStream_t side = c.side;
     WR_HIP_CHECK(hipEventRecord(c.lvl_events[layer + 1], s));
     WR_HIP_CHECK(hipStreamWaitEvent(side, c.lvl_events[layer + 1], 0));
-    std::vector<const float*> ref_lv(layer + 1);
+    std::vector<const void*> ref_lv(layer + 1);
+    const OutMode om_level = store_mode(st);
     auto build_level = [&](int L) -> int {
         const LevelGeom& g = geo[L];
         const OutLayout padded = padded_layout(g.nx, g.ny);
-        float* mov_l = c.mov_l.as<float>() + mov_off[L];
-        resample2(mov, et, 1, 0, mov_l, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
-        launch_pad_volume(mov_l, g.nx, g.ny, nz, side);
-        if (L == 0 && dtype == WR_F32) {
-            ref_lv[L] = static_cast<const float*>(ref);
+        void* mov_l = c.mov_l.as<char>() + mov_off[L] * es;
+        resample2(mov, et, 1, 0, mov_l, om_level, 1.0, down[L], tw, ti, nz, tmp, padded, side);
+        launch_pad_volume(mov_l, static_cast<int>(es), g.nx, g.ny, nz, side);
+        if (L == 0 && dtype == WR_F32 && st == ST_F32) {
+            ref_lv[L] = ref;
         } else {
-            float* ref_l = c.ref_l.as<float>() + ref_off[L];
-            resample2(ref, et, 1, 0, ref_l, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), side);
+            void* ref_l = c.ref_l.as<char>() + ref_off[L] * es;
+            resample2(ref, et, 1, 0, ref_l, om_level, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), side);
             ref_lv[L] = ref_l;
         }
         if (mask_ref)
@@ -436,7 +440,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         if (mask_mov) {
             float* mm = c.mmov_l.as<float>() + mov_off[L];
             resample2(mask_mov, E_F32, 1, 0, mm, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
-            launch_pad_volume(mm, g.nx, g.ny, nz, side);
+            launch_pad_volume(mm, 4, g.nx, g.ny, nz, side);
         }
         WR_HIP_CHECK(hipEventRecord(c.lvl_events[L], side));
         return WR_OK;
@@ -473,7 +477,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         if (L - 2 >= 0) WR_TRY(build_level(L - 2));
         WR_HIP_CHECK(hipStreamWaitEvent(s, c.lvl_events[L], 0));
         LkBuffers b;
-        b.mov = c.mov_l.as<float>() + mov_off[L];
+        b.store = st;
+        b.mov = c.mov_l.as<char>() + mov_off[L] * es;
         b.ref = ref_lv[L];
         b.mref = mask_ref ? c.mref_l.as<uint8_t>() + ref_off[L] : nullptr;
         b.mmov = mask_mov ? c.mmov_l.as<float>() + mov_off[L] : nullptr;

@@ -66,7 +66,7 @@ __device__ __forceinline__ Samp split(float c, float hi) {
 // a clamped gather is an unconditional load: one per-lane 32-bit offset from
 // the allocation start plus the tap row's fixed offset.
 struct PVol {
-    const float* base;       // allocation start
+    const unsigned char* base;   // allocation start
     unsigned pitchB, planeB; // row and plane strides in bytes
     unsigned plane_rows;     // stored rows per plane
     unsigned origin;         // byte offset of voxel (0, 0, 0)
@@ -76,20 +76,73 @@ struct PVol {
 
 typedef float f4v __attribute__((ext_vector_type(4), aligned(4)));
 typedef float f2v __attribute__((ext_vector_type(2), aligned(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2), aligned(4)));
 
-// 16 or 8 bytes at byte offset off of the allocation (4-byte aligned).
-__device__ __forceinline__ f4v ld4(const PVol& v, unsigned off) {
-    return *reinterpret_cast<const f4v*>(reinterpret_cast<const unsigned char*>(v.base) + off);
+// Bytes per stored voxel of a level image of storage S (Store).
+template <int S>
+constexpr unsigned kEs = S == ST_F32 ? 4u : 2u;
+
+// The two 16-bit values of a word as float32 (bfloat16: the high half of a
+// float32; float16: converted).
+template <int S>
+__device__ __forceinline__ float lo16(unsigned w) {
+    if constexpr (S == ST_BF16) return __uint_as_float(w << 16);
+    else return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xffffu)));
 }
+template <int S>
+__device__ __forceinline__ float hi16(unsigned w) {
+    if constexpr (S == ST_BF16) return __uint_as_float(w & 0xffff0000u);
+    else return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16)));
+}
+
+// Four or two consecutive voxels at byte offset off of the allocation, as
+// float32: one 16- or 8-byte load for float32 storage (4-byte aligned), one
+// 8- or 4-byte load for 16-bit storage (2-byte aligned: the hardware serves
+// unaligned global loads).
+template <int S>
+__device__ __forceinline__ f4v ld4(const PVol& v, unsigned off) {
+    if constexpr (S == ST_F32) {
+        return *reinterpret_cast<const f4v*>(v.base + off);
+    } else {
+        const u2v q = *reinterpret_cast<const u2v*>(v.base + off);
+        f4v r;
+        r.x = lo16<S>(q.x);
+        r.y = hi16<S>(q.x);
+        r.z = lo16<S>(q.y);
+        r.w = hi16<S>(q.y);
+        return r;
+    }
+}
+template <int S>
 __device__ __forceinline__ f2v ld2(const PVol& v, unsigned off) {
-    return *reinterpret_cast<const f2v*>(reinterpret_cast<const unsigned char*>(v.base) + off);
+    if constexpr (S == ST_F32) {
+        return *reinterpret_cast<const f2v*>(v.base + off);
+    } else {
+        const unsigned q = *reinterpret_cast<const unsigned*>(v.base + off);
+        f2v r;
+        r.x = lo16<S>(q);
+        r.y = hi16<S>(q);
+        return r;
+    }
+}
+
+// One template voxel (dense level image of storage S).
+template <int S>
+__device__ __forceinline__ float ld_ref(const void* ref, int i) {
+    if constexpr (S == ST_F32) {
+        return static_cast<const float*>(ref)[i];
+    } else {
+        const unsigned w = static_cast<const uint16_t*>(ref)[i];
+        return lo16<S>(w);
+    }
 }
 
 // Byte offset of voxel (fx, fy, fz) from the origin: (fz * plane_rows + fy)
 // rows (fewer than 2^24) of pitchB (< 2^24) bytes, 24-bit products.
+template <int S>
 __device__ __forceinline__ unsigned voxel_off(const PVol& v, int fx, int fy, int fz) {
     const unsigned row = __umul24(static_cast<unsigned>(fz), v.plane_rows) + static_cast<unsigned>(fy);
-    return __umul24(row, v.pitchB) + static_cast<unsigned>(fx) * 4u;
+    return __umul24(row, v.pitchB) + static_cast<unsigned>(fx) * kEs<S>;
 }
 
 // Sample of the 4-point line l[0..3] = f(clamp(c0 - 1 + q)) at the clipped
@@ -153,22 +206,23 @@ __device__ __forceinline__ Coord make_coord(const PVol& v, float cx, float cy, f
 // (X0, Y0, Z0) = (fx - 1, fy - 1, fz - 1), which the pads keep inside the
 // allocation.  Where the clamped reference would repeat an edge row or
 // plane, the pads hold it.
+template <int S>
 __device__ __forceinline__ Taps gather(const PVol& v, const Coord& c) {
-    const unsigned py = v.pitchB, pz = v.planeB;
-    const unsigned a = v.origin + voxel_off(v, c.sx.f - 1, c.sy.f, c.sz.f) - pz - py;
+    const unsigned py = v.pitchB, pz = v.planeB, e = kEs<S>;
+    const unsigned a = v.origin + voxel_off<S>(v, c.sx.f - 1, c.sy.f, c.sz.f) - pz - py;
     Taps t;
-    t.q11 = ld4(v, a + pz + py);
-    t.q21 = ld4(v, a + pz + 2u * py);
-    t.q12 = ld4(v, a + 2u * pz + py);
-    t.q22 = ld4(v, a + 2u * pz + 2u * py);
-    t.e01 = ld2(v, a + pz + 4u);
-    t.e02 = ld2(v, a + 2u * pz + 4u);
-    t.e31 = ld2(v, a + pz + 3u * py + 4u);
-    t.e32 = ld2(v, a + 2u * pz + 3u * py + 4u);
-    t.f10 = ld2(v, a + py + 4u);
-    t.f20 = ld2(v, a + 2u * py + 4u);
-    t.f13 = ld2(v, a + 3u * pz + py + 4u);
-    t.f23 = ld2(v, a + 3u * pz + 2u * py + 4u);
+    t.q11 = ld4<S>(v, a + pz + py);
+    t.q21 = ld4<S>(v, a + pz + 2u * py);
+    t.q12 = ld4<S>(v, a + 2u * pz + py);
+    t.q22 = ld4<S>(v, a + 2u * pz + 2u * py);
+    t.e01 = ld2<S>(v, a + pz + e);
+    t.e02 = ld2<S>(v, a + 2u * pz + e);
+    t.e31 = ld2<S>(v, a + pz + 3u * py + e);
+    t.e32 = ld2<S>(v, a + 2u * pz + 3u * py + e);
+    t.f10 = ld2<S>(v, a + py + e);
+    t.f20 = ld2<S>(v, a + 2u * py + e);
+    t.f13 = ld2<S>(v, a + 3u * pz + py + e);
+    t.f23 = ld2<S>(v, a + 3u * pz + 2u * py + e);
     return t;
 }
 
@@ -213,13 +267,13 @@ __device__ __forceinline__ Grad eval(const PVol& v, const Coord& c, const Taps& 
     return o;
 }
 
-// Trilinear sample only (4 x 8-byte rows).
+// Trilinear sample only (4 x 8-byte rows of a float32 volume).
 __device__ __forceinline__ float tri(const PVol& v, float cx, float cy, float cz) {
     const Samp sx = split(cx, v.hx), sy = split(cy, v.hy), sz = split(cz, v.hz);
-    const unsigned a = v.origin + voxel_off(v, sx.f, sy.f, sz.f);
+    const unsigned a = v.origin + voxel_off<ST_F32>(v, sx.f, sy.f, sz.f);
     const unsigned py = v.pitchB, pz = v.planeB;
-    const f2v r00 = ld2(v, a), r01 = ld2(v, a + py);
-    const f2v r10 = ld2(v, a + pz), r11 = ld2(v, a + pz + py);
+    const f2v r00 = ld2<ST_F32>(v, a), r01 = ld2<ST_F32>(v, a + py);
+    const f2v r10 = ld2<ST_F32>(v, a + pz), r11 = ld2<ST_F32>(v, a + pz + py);
     const float p = lerpf(lerpf(r00.x, r00.y, sx.t), lerpf(r01.x, r01.y, sx.t), sy.t);
     const float q = lerpf(lerpf(r10.x, r10.y, sx.t), lerpf(r11.x, r11.y, sx.t), sy.t);
     return lerpf(p, q, sz.t);
@@ -228,7 +282,7 @@ __device__ __forceinline__ float tri(const PVol& v, float cx, float cy, float cz
 struct PassArgs {
     PVol mov;
     PVol mmov;
-    const float* ref;
+    const void* ref;
     const uint8_t* mref;
     const float* m0;
     const float* cp;
@@ -269,7 +323,7 @@ __device__ __forceinline__ void tile_of(long long nb, int nz, int nbx, int& bx, 
 // lane on each side; waves walk the rows independently (the 3x3 mean's
 // horizontal neighbours come from cross-lane shuffles), and the next row's
 // gathers are issued before the current row is evaluated.
-template <bool DENSE, bool MREF, bool MMOV>
+template <bool DENSE, bool MREF, bool MMOV, int S>
 __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     if (a.ctl->done) return;
     if (a.check_parts > 0) {
@@ -384,10 +438,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     if (wave * kWaveCols < a.cols) {   // uniform: the wave has tile columns
         for (int s = 0; s < nsteps; ++s) {
             const Coord cc = coord_of(s);
-            const Taps tc = gather(a.mov, cc);
+            const Taps tc = gather<S>(a.mov, cc);
             // template / mask row of this step: uniform base, per-lane column
             const int vrow = z * plane + clampi(y0 - 1 + s, 0, a.ny - 1) * a.nx;
-            const float rc = (a.ref + vrow)[xc];
+            const float rc = ld_ref<S>(a.ref, vrow + xc);
             const int yy = y0 - 1 + s;
             const Grad g = eval(a.mov, cc, tc);
             const float gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
@@ -466,14 +520,14 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     }
 }
 
-PVol make_pvol(const float* base, int nx, int ny, int nz) {
+PVol make_pvol(const void* base, unsigned es, int nx, int ny, int nz) {
     const OutLayout ol = padded_layout(nx, ny);
     PVol v;
-    v.base = base;
-    v.pitchB = static_cast<unsigned>(ol.pitch) * 4u;
+    v.base = static_cast<const unsigned char*>(base);
+    v.pitchB = static_cast<unsigned>(ol.pitch) * es;
     v.plane_rows = static_cast<unsigned>(ol.plane_rows);
     v.planeB = v.pitchB * v.plane_rows;
-    v.origin = static_cast<unsigned>((ol.lead * ol.pitch + kPad) * 4);
+    v.origin = static_cast<unsigned>((ol.lead * ol.pitch + kPad) * es);
     v.nx = nx;
     v.ny = ny;
     v.nz = nz;
@@ -488,8 +542,8 @@ PVol make_pvol(const float* base, int nx, int ny, int nz) {
 long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, int check_parts,
                          hipStream_t s) {
     PassArgs a;
-    a.mov = make_pvol(b.mov, g.nx, g.ny, g.nz);
-    a.mmov = make_pvol(b.mmov ? b.mmov : b.mov, g.nx, g.ny, g.nz);
+    a.mov = make_pvol(b.mov, static_cast<unsigned>(store_bytes(b.store)), g.nx, g.ny, g.nz);
+    a.mmov = make_pvol(b.mmov ? static_cast<const void*>(b.mmov) : b.mov, 4u, g.nx, g.ny, g.nz);
     a.ref = b.ref;
     a.mref = b.mref;
     a.m0 = b.m0;
@@ -526,12 +580,14 @@ long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b
     const dim3 grid(static_cast<unsigned>(t.blocks + a.npen));
     const bool mr = b.mref != nullptr, mm = b.mmov != nullptr;
     using Fn = void (*)(PassArgs);
-    static const Fn table[8] = {k_lk_pass<false, false, false>, k_lk_pass<false, false, true>,
-                                k_lk_pass<false, true, false>,  k_lk_pass<false, true, true>,
-                                k_lk_pass<true, false, false>,  k_lk_pass<true, false, true>,
-                                k_lk_pass<true, true, false>,   k_lk_pass<true, true, true>};
+#define WR_LK_TABLE(S)                                                                                   \
+    {k_lk_pass<false, false, false, S>, k_lk_pass<false, false, true, S>, k_lk_pass<false, true, false, S>, \
+     k_lk_pass<false, true, true, S>,   k_lk_pass<true, false, false, S>, k_lk_pass<true, false, true, S>,  \
+     k_lk_pass<true, true, false, S>,   k_lk_pass<true, true, true, S>}
+    static const Fn table[3][8] = {WR_LK_TABLE(ST_F32), WR_LK_TABLE(ST_BF16), WR_LK_TABLE(ST_F16)};
+#undef WR_LK_TABLE
     const int sel = (k == 0 ? 4 : 0) | (mr ? 2 : 0) | (mm ? 1 : 0);
-    hipLaunchKernelGGL(table[sel], grid, dim3(threads), 0, s, a);
+    hipLaunchKernelGGL(table[b.store][sel], grid, dim3(threads), 0, s, a);
     return a.npen;
 }
 

@@ -199,9 +199,21 @@ __device__ __forceinline__ double scaled(double v, double divisor) {
     return v / divisor;
 }
 
+// 16-bit storage of a level image: the float32 value rounded to nearest even.
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    const unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);   // NaN stays NaN
+    return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ uint16_t to_f16(float f) {
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(f));
+}
+
 __device__ __forceinline__ void store_out(void* out, size_t i, double v, int om, double divisor) {
     if (om == OUT_F64) static_cast<double*>(out)[i] = v;
     else if (om == OUT_F32) static_cast<float*>(out)[i] = static_cast<float>(scaled(v, divisor));
+    else if (om == OUT_BF16) static_cast<uint16_t*>(out)[i] = to_bf16(static_cast<float>(scaled(v, divisor)));
+    else if (om == OUT_F16) static_cast<uint16_t*>(out)[i] = to_f16(static_cast<float>(scaled(v, divisor)));
     else static_cast<uint8_t*>(out)[i] = v > 0.0 ? 1 : 0;
 }
 
@@ -545,6 +557,15 @@ __device__ __forceinline__ void store_v(void* out, size_t i, const double (&v)[V
 #pragma unroll
             for (int q = 0; q < V; ++q) o[q] = static_cast<float>(scaled(v[q], divisor));
             *reinterpret_cast<FV*>(static_cast<float*>(out) + i) = o;
+        } else if (om == OUT_BF16 || om == OUT_F16) {
+            typedef uint16_t HV __attribute__((ext_vector_type(V)));
+            HV o;
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                const float f = static_cast<float>(scaled(v[q], divisor));
+                o[q] = om == OUT_BF16 ? to_bf16(f) : to_f16(f);
+            }
+            *reinterpret_cast<HV*>(static_cast<uint16_t*>(out) + i) = o;
         } else {
             typedef uint8_t BV __attribute__((ext_vector_type(V)));
             BV o;
@@ -611,17 +632,19 @@ __global__ __launch_bounds__(256) void k_convert(const void* __restrict__ in, in
 // Pads of a padded volume (padded_layout): x ends of the logical rows, then
 // (next launch) the y pad rows of every plane, then (next launch) the z pad
 // planes, each a copy of the nearest logical row or plane.
-__global__ __launch_bounds__(256) void k_pad_x(float* __restrict__ buf, OutLayout ol, long long nrows, int nx) {
+template <class T>
+__global__ __launch_bounds__(256) void k_pad_x(T* __restrict__ buf, OutLayout ol, long long nrows, int nx) {
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= nrows * 2 * kPad) return;
     const long long r = i / (2 * kPad);
     const int j = static_cast<int>(i - r * 2 * kPad);
-    float* row = buf + out_row(ol, r) * ol.pitch;
+    T* row = buf + out_row(ol, r) * ol.pitch;
     if (j < kPad) row[j] = row[kPad];
     else row[kPad + nx + (j - kPad)] = row[kPad + nx - 1];
 }
 
-__global__ __launch_bounds__(256) void k_pad_y(float* __restrict__ buf, OutLayout ol, int nz) {
+template <class T>
+__global__ __launch_bounds__(256) void k_pad_y(T* __restrict__ buf, OutLayout ol, int nz) {
     // (plane, pad row, element): kPadLo + kPadHi pad rows of `pitch` floats
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     const int per_plane = (kPadLo + kPadHi) * ol.pitch;
@@ -635,7 +658,8 @@ __global__ __launch_bounds__(256) void k_pad_y(float* __restrict__ buf, OutLayou
     buf[(plane0 + dst) * ol.pitch + e] = buf[(plane0 + src) * ol.pitch + e];
 }
 
-__global__ __launch_bounds__(256) void k_pad_z(float* __restrict__ buf, OutLayout ol, int nz) {
+template <class T>
+__global__ __launch_bounds__(256) void k_pad_z(T* __restrict__ buf, OutLayout ol, int nz) {
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     const long long plane = static_cast<long long>(ol.plane_rows) * ol.pitch;
     if (i >= (kPadLo + kPadHi) * plane) return;
@@ -887,7 +911,9 @@ void launch_resample_xy(const void* in, Elem in_t, void* out, OutMode om, double
 namespace {
 
 size_t elem_bytes(Elem e) { return e == E_U16 ? 2 : (e == E_F32 ? 4 : 8); }
-size_t out_bytes(OutMode om) { return om == OUT_F64 ? 8 : (om == OUT_F32 ? 4 : 1); }
+size_t out_bytes(OutMode om) {
+    return om == OUT_F64 ? 8 : (om == OUT_F32 ? 4 : (om == OUT_BF16 || om == OUT_F16 ? 2 : 1));
+}
 
 // V-element vector access is possible: dense input, rows and output rows that
 // start on multiples of V elements, and V-aligned base pointers.
@@ -957,15 +983,25 @@ void launch_convert(const void* in, Elem in_t, int in_stride, int in_comp, void*
         launch_conv<1>(in, in_t, in_stride, in_comp, out, om, divisor, nx, nrows, ol, s);
 }
 
-void launch_pad_volume(float* buf, int nx, int ny, int nz, hipStream_t s) {
+namespace {
+template <class T>
+void pad_volume(T* buf, int nx, int ny, int nz, hipStream_t s) {
     const OutLayout ol = padded_layout(nx, ny);
     const long long nrows = static_cast<long long>(ny) * nz;
     const long long nx_ = nrows * 2 * kPad;
-    k_pad_x<<<static_cast<unsigned>((nx_ + 255) / 256), 256, 0, s>>>(buf, ol, nrows, nx);
+    k_pad_x<T><<<static_cast<unsigned>((nx_ + 255) / 256), 256, 0, s>>>(buf, ol, nrows, nx);
     const long long ny_ = static_cast<long long>(nz) * (kPadLo + kPadHi) * ol.pitch;
-    k_pad_y<<<static_cast<unsigned>((ny_ + 255) / 256), 256, 0, s>>>(buf, ol, nz);
+    k_pad_y<T><<<static_cast<unsigned>((ny_ + 255) / 256), 256, 0, s>>>(buf, ol, nz);
     const long long nz_ = static_cast<long long>(kPadLo + kPadHi) * ol.plane_rows * ol.pitch;
-    k_pad_z<<<static_cast<unsigned>((nz_ + 255) / 256), 256, 0, s>>>(buf, ol, nz);
+    k_pad_z<T><<<static_cast<unsigned>((nz_ + 255) / 256), 256, 0, s>>>(buf, ol, nz);
+}
+}  // namespace
+
+void launch_pad_volume(void* buf, int esize, int nx, int ny, int nz, hipStream_t s) {
+    if (esize == 2)
+        pad_volume(static_cast<uint16_t*>(buf), nx, ny, nz, s);
+    else
+        pad_volume(static_cast<float*>(buf), nx, ny, nz, s);
 }
 
 }  // namespace wr

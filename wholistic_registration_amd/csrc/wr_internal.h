@@ -67,7 +67,12 @@ constexpr int kMaxRow = 8192;
 // the widest column group.
 constexpr int kXSpan = 191;
 constexpr int kXMaxCols = 32;
-enum OutMode { OUT_F64 = 0, OUT_F32 = 1, OUT_MASK = 2 };
+enum OutMode { OUT_F64 = 0, OUT_F32 = 1, OUT_MASK = 2, OUT_BF16 = 3, OUT_F16 = 4 };
+
+// Storage of the pyramid level images read by the LK pass (wr_params.precision).
+enum Store { ST_F32 = 0, ST_BF16 = 1, ST_F16 = 2 };
+inline int store_bytes(int st) { return st == ST_F32 ? 4 : 2; }
+inline OutMode store_mode(int st) { return st == ST_F32 ? OUT_F32 : (st == ST_BF16 ? OUT_BF16 : OUT_F16); }
 
 // The moving level image (and mask_mov) is stored padded so that every tap
 // of a clamped trilinear gather is a plain load: kPad replicated voxels at
@@ -136,9 +141,10 @@ void launch_resample_y(const void* in, Elem in_t, int in_stride, int in_comp, vo
 // Elementwise conversion (identity resample) of nrows rows of nx: out = convert(in).
 void launch_convert(const void* in, Elem in_t, int in_stride, int in_comp, void* out,
                     OutMode om, double divisor, int nx, long long nrows, OutLayout ol, hipStream_t s);
-// Fill the pads of a padded float volume (padded_layout) whose logical rows
-// are written: x ends, then the y pad rows, then the z pad planes.
-void launch_pad_volume(float* buf, int nx, int ny, int nz, hipStream_t s);
+// Fill the pads of a padded volume (padded_layout) of `esize`-byte elements
+// whose logical rows are written: x ends, then the y pad rows, then the z pad
+// planes.
+void launch_pad_volume(void* buf, int esize, int nx, int ny, int nz, hipStream_t s);
 
 // ---------------------------------------------------------------- LK solver
 
@@ -182,8 +188,9 @@ struct LkTile {
 LkTile plan_tile(const LevelGeom& g, int np_x);
 
 struct LkBuffers {
-    const float* mov;        // level moving image, padded volume (padded_layout), allocation start
-    const float* ref;        // level template
+    int store;               // element type of mov and ref (Store)
+    const void* mov;         // level moving image, padded volume (padded_layout), allocation start
+    const void* ref;         // level template
     const uint8_t* mref;     // level mask_ref (> 0 after resampling) or null
     const float* mmov;       // level mask_mov (continuous, padded like mov, allocation start) or null
     const float* m0;         // dense field at level start, 3 planes, or null

@@ -88,10 +88,13 @@ class MotionResult:
         return out
 
 
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+
 def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
                schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
-               stop_kernels=False):
+               stop_kernels=False, precision="fp32"):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
@@ -99,6 +102,9 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     run's iteration schedule.  ``stop_kernels`` runs the stop rules in their
     own single-workgroup kernels on every level (by default that is done only
     where the launches that follow cannot afford to evaluate them).
+    ``precision`` stores the pyramid level images the iterations read as
+    ``"fp32"`` (the reference's values), ``"bf16"`` or ``"fp16"``; every
+    interpolation, sum and solve stays float32 / float64.
     """
     torch = _torch()
     _check_vol(mov, "mov")
@@ -122,6 +128,9 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     prm.smoothPenalty, prm.tol = float(smoothPenalty), float(tol)
     prm.tile_patches_x = int(tile_patches_x)
     prm.stop_kernels = 1 if stop_kernels else 0
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
+    prm.precision = PRECISIONS[precision]
     prm.timing = 1 if timing else 0
     keep = []
     if schedule is not None:
