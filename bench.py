@@ -200,6 +200,52 @@ def precision_measure(w, n0, step, motion_fp32, field, mask, steps, prec):
     }
 
 
+def batch_measure(n_frames, worker_counts, seed=2):
+    """Frames of a time series against one template (BASELINE config 2's
+    frame geometry): a 64 x 512 x 1024 uint16 template with 100k blobs, frames
+    pull-warped by their own 6-mode fields (max |d| = 6) with fresh Poisson
+    noise; layer 4, r = 8, 15 iterations, no mask.  Registered voxels/s with
+    one context (frames in sequence) and with several concurrent contexts
+    (FrameBatch), on the same frames."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import synth
+    from wholistic_registration_amd.batch import FrameBatch
+    dev = torch.device("cuda", 0)
+    zyx = (64, 512, 1024)
+    rng = np.random.default_rng(seed)
+    clean = synth.render_template_torch(zyx, synth.draw_blobs(rng, zyx, 100_000), dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    tmpl = torch.poisson(clean, generator=g).round_().clamp_(0, 65535).to(torch.uint16).contiguous()
+    frames = []
+    for i in range(n_frames):
+        d = synth.field_torch(zyx, synth.draw_modes(np.random.default_rng(1000 + i), 6), 6.0, dev)
+        warped = D.correct_motion(clean, (-d).contiguous())
+        frames.append(torch.poisson(warped.clamp_min(0), generator=g).round_().clamp_(0, 65535)
+                      .to(torch.uint16).contiguous())
+        del d, warped
+    sp = float(D.smooth_penalty_factor(tmpl.float().contiguous())) * RAW_PENALTY
+    params = dict(layer=4, iter=15, r=8, zRatio=1.0, movRange=5.0, smoothPenalty=sp, tol=1e-3)
+    n_vox = n_frames * zyx[0] * zyx[1] * zyx[2]
+    out = {"workload": f"{n_frames} frames of {zyx[0]}x{zyx[1]}x{zyx[2]} (z,y,x) uint16 against one "
+                       f"template, layer 4, r 8, 15 iterations, getMotion + correctMotion per frame",
+           "frames": n_frames}
+    warped_out = torch.empty((n_frames,) + zyx, dtype=torch.float32, device=dev)
+    for wk in worker_counts:
+        with FrameBatch(workers=wk) as fb:
+            fb.register(frames[: min(n_frames, 2 * wk)], tmpl, **params)   # workspaces, warm caches
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fb.register(frames, tmpl, warped_out=warped_out, **params)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        out[f"workers_{wk}"] = {"value": n_vox / dt, "ms_per_frame": 1e3 * dt / n_frames}
+    out["unit"] = "registered voxels/s"
+    return out
+
+
 def mask_measure(vol, crop, reps=3):
     """The harness's automatic mask of one moving volume (getMask +
     bwareafilt3_wei with the default config's thresFactor 5 and maskRange
@@ -280,6 +326,8 @@ def main():
     ap.add_argument("--precisions", default="bf16,fp16",
                     help="reduced-precision level-image variants to time beside the float32 path "
                          "(comma list of bf16, fp16; 'none' to skip)")
+    ap.add_argument("--batch-frames", type=int, default=32,
+                    help="frames of the time-series measurement (config 2 frame size; 0 to skip)")
     ap.add_argument("--no-mask-measure", action="store_true",
                     help="skip timing the harness's automatic mask kernels")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -387,6 +435,9 @@ def main():
         out["precision_variants"] = dict(variants, note=(
             "same workload and step with the pyramid level images stored as bfloat16 / float16; "
             "interpolation, sums and solves stay float32 / float64; value = registered voxels/s"))
+    if args.batch_frames > 0:
+        log(f"[bench] {args.batch_frames} frames against one template")
+        out["batched_frames"] = batch_measure(args.batch_frames, (1, 2, 4, 8))
     if not args.no_mask_measure:
         log("[bench] harness mask of the moving volume")
         out["harness_masks"] = mask_measure(mov, tuple(int(v) for v in args.cpu_crop.split(",")))

@@ -35,6 +35,7 @@ class WrParams(ctypes.Structure):
         ("tile_patches_x", ctypes.c_int),
         ("stop_kernels", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("reuse_template", ctypes.c_int),
     ]
 
 

@@ -295,6 +295,25 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     const int p = 2 * r + 1;
     const int st = prm->precision;        // storage of the level images (Store)
     const size_t es = static_cast<size_t>(store_bytes(st));
+    Context::TemplateKey key;
+    key.valid = 1;
+    key.nx = nx;
+    key.ny = ny;
+    key.nz = nz;
+    key.layer = layer;
+    key.dtype = dtype;
+    key.precision = st;
+    key.masked = mask_ref != nullptr;
+    key.ref = ref;
+    const bool reuse = prm->reuse_template != 0;
+    if (reuse) {
+        const Context::TemplateKey& k = ctx->c.tmpl;
+        WR_ARG(k.valid && k.nx == nx && k.ny == ny && k.nz == nz && k.layer == layer && k.dtype == dtype &&
+                   k.precision == st && k.masked == key.masked && k.ref == ref,
+               "wr_get_motion: reuse_template without a previous call on the same template, size, dtype, "
+               "layer and precision");
+    }
+    ctx->c.tmpl.valid = 0;   // set again once this call's template levels are enqueued
 
     Context& c = ctx->c;
     WR_HIP_CHECK(hipSetDevice(c.device));
@@ -431,10 +450,10 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             ref_lv[L] = ref;
         } else {
             void* ref_l = c.ref_l.as<char>() + ref_off[L] * es;
-            resample2(ref, et, 1, 0, ref_l, om_level, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), side);
+            if (!reuse) resample2(ref, et, 1, 0, ref_l, om_level, 1.0, down[L], tw, ti, nz, tmp, dense(g.nx), side);
             ref_lv[L] = ref_l;
         }
-        if (mask_ref)
+        if (mask_ref && !reuse)
             resample2(mask_ref, E_F32, 1, 0, c.mref_l.as<uint8_t>() + ref_off[L], OUT_MASK, 1.0, down[L], tw, ti, nz,
                       tmp, dense(g.nx), side);
         if (mask_mov) {
@@ -504,6 +523,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         else launch_level_end(g, b, L, iter, nullptr, motion_out, coords_out, s);
         WR_HIP_CHECK(hipGetLastError());
     }
+
+    c.tmpl = key;   // every template level of this call is enqueued
 
     // ---- logs back to the host
     std::vector<double> hlog(static_cast<size_t>(layer + 1) * iter * 4);

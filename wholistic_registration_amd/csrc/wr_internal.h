@@ -320,6 +320,12 @@ struct Context {
     hipStream_t side = nullptr;          // pyramid builds, ahead of the iterations
     std::vector<hipEvent_t> lvl_events;  // [L]: level L's images are ready; [layer+1]: fork
     wr_stats stats{};
+    // geometry of the template level images (and mask_ref levels) the last
+    // wr_get_motion built, for wr_params.reuse_template; valid = 0: none
+    struct TemplateKey {
+        int valid = 0, nx = 0, ny = 0, nz = 0, layer = 0, dtype = 0, precision = 0, masked = 0;
+        const void* ref = nullptr;
+    } tmpl;
 };
 
 }  // namespace wr

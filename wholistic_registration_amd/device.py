@@ -94,7 +94,7 @@ PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
 def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
                schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
-               stop_kernels=False, precision="fp32"):
+               stop_kernels=False, precision="fp32", reuse_template=False, context=None, stream=None):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
@@ -104,7 +104,11 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     where the launches that follow cannot afford to evaluate them).
     ``precision`` stores the pyramid level images the iterations read as
     ``"fp32"`` (the reference's values), ``"bf16"`` or ``"fp16"``; every
-    interpolation, sum and solve stays float32 / float64.
+    interpolation, sum and solve stays float32 / float64.  ``reuse_template``
+    skips rebuilding the template's pyramid when the previous call on the same
+    context had the same ``ref`` tensor (unchanged), size, layer and precision.
+    ``context`` (a handle from ``_lib``) and ``stream`` (a torch stream)
+    default to the device's shared context and torch's current stream.
     """
     torch = _torch()
     _check_vol(mov, "mov")
@@ -131,6 +135,7 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
     prm.precision = PRECISIONS[precision]
+    prm.reuse_template = 1 if reuse_template else 0
     prm.timing = 1 if timing else 0
     keep = []
     if schedule is not None:
@@ -150,10 +155,11 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     log.max_diff_motion = bufs["md"].ctypes.data_as(dp)
 
     lib = _lib.load()
-    ctx = _lib.context(dev.index if dev.index is not None else 0)
+    ctx = context if context is not None else _lib.context(dev.index if dev.index is not None else 0)
+    strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
     rc = lib.wr_get_motion(ctx, _ptr(mov), _ptr(ref), dtype_code(mov), nx, ny, nz,
                            _ptr(mask_ref), _ptr(mask_mov), _ptr(motion_init), ctypes.byref(prm),
-                           _ptr(motion_out), _ptr(coords), ctypes.byref(log), _stream())
+                           _ptr(motion_out), _ptr(coords), ctypes.byref(log), strm)
     _lib.check(rc, "wr_get_motion")
     return MotionResult(
         motion=motion_out, coords=coords,
@@ -172,7 +178,7 @@ def stats(device=0):
     return st
 
 
-def correct_motion(vol, motion, out=None):
+def correct_motion(vol, motion, out=None, context=None, stream=None):
     """``correctMotion`` (calFlow3d_Wei_v1.py:676-706) on device tensors."""
     torch = _torch()
     _check_vol(vol, "volume")
@@ -183,9 +189,10 @@ def correct_motion(vol, motion, out=None):
         out = torch.empty(vol.shape, dtype=torch.float32, device=vol.device)
     nz, ny, nx = vol.shape
     lib = _lib.load()
-    ctx = _lib.context(vol.device.index or 0)
+    ctx = context if context is not None else _lib.context(vol.device.index or 0)
+    strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
     rc = lib.wr_correct_motion(ctx, _ptr(vol), dtype_code(vol), nx, ny, nz, _ptr(motion),
-                               _ptr(out), _stream())
+                               _ptr(out), strm)
     _lib.check(rc, "wr_correct_motion")
     return out
 
