@@ -246,6 +246,44 @@ def batch_measure(n_frames, worker_counts, seed=2):
     return out
 
 
+def splat_measure(reps=3):
+    """The cross-resolution path's fixed-plane projections on one
+    64 x 512 x 1024 frame of warped coordinates (the voxel grid plus a smooth
+    6-mode field, max |d| = 6) onto 8 reference planes: max splat with the
+    2x2 footprint, Gaussian-weighted splat.  Kernel time per call with events
+    on the launch stream; algorithmic bytes = the samples read (16 B each)
+    plus the output planes written."""
+    import torch
+
+    from wholistic_registration_amd import calFlowCrossResolution as X
+    from wholistic_registration_amd import synth
+    dev = torch.device("cuda", 0)
+    zyx = (64, 512, 1024)
+    d = synth.field_torch(zyx, synth.draw_modes(np.random.default_rng(5), 6), 6.0, dev)
+    z, y, x = torch.meshgrid(*[torch.arange(n, device=dev, dtype=torch.float32) for n in zyx], indexing="ij")
+    coords = torch.stack([x + d[..., 0], y + d[..., 1], z + d[..., 2]], dim=-1).permute(2, 1, 0, 3).contiguous()
+    values = (x * 0.01 + y * 0.02).permute(2, 1, 0).contiguous()
+    ref = torch.empty(zyx[::-1], device=dev)          # (x, y, z) shape only
+    planes = torch.linspace(4, 60, 8, device=dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t_max, t_w = [], []
+    for _ in range(reps + 1):
+        ev[0].record()
+        X.project_coords_to_fixed_planes_gpu(coords, ref, planes, values_xyk=values, z_window=1.5)
+        ev[1].record()
+        X.project_coords_to_fixed_planes_weighted_gpu(coords, ref, planes, values_xyk=values, z_window=1.5)
+        ev[2].record()
+        torch.cuda.synchronize()
+        t_max.append(ev[0].elapsed_time(ev[1]))
+        t_w.append(ev[1].elapsed_time(ev[2]))
+    n = values.numel()
+    out_bytes = 8 * zyx[1] * zyx[2] * 4
+    t_max, t_w = float(np.mean(t_max[1:])), float(np.mean(t_w[1:]))
+    return {"samples": n, "planes": 8, "max_ms": t_max, "weighted_ms": t_w,
+            "max_gbs": (16 * n + out_bytes) / (t_max * 1e-3) / 1e9,
+            "weighted_gbs": (16 * n + 2 * out_bytes) / (t_w * 1e-3) / 1e9}
+
+
 def mask_measure(vol, crop, reps=3):
     """The harness's automatic mask of one moving volume (getMask +
     bwareafilt3_wei with the default config's thresFactor 5 and maskRange
@@ -439,6 +477,8 @@ def main():
         log(f"[bench] {args.batch_frames} frames against one template")
         out["batched_frames"] = batch_measure(args.batch_frames, (1, 2, 4, 8))
     if not args.no_mask_measure:
+        log("[bench] fixed-plane projections")
+        out["plane_projections"] = splat_measure()
         log("[bench] harness mask of the moving volume")
         out["harness_masks"] = mask_measure(mov, tuple(int(v) for v in args.cpu_crop.split(",")))
     if not (args.no_cpu_baseline and args.no_epe_check):

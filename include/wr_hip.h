@@ -236,6 +236,34 @@ int wr_frames_mean(wr_ctx* ctx, const void* const* frames, int count, int dtype,
 int wr_frame_gram(wr_ctx* ctx, const void* slab, int dtype, int nt, int nzs, int ny, int nx, int ds,
                   double* cc, void* stream);
 
+/* ---- fixed-plane projections of the cross-resolution path */
+
+/* Replaces the max splat of project_coords_to_fixed_planes_gpu
+ * (src/wholistic_registration/utils/calFlowCrossResolution.py:480-568 kernel,
+ * :878-1141 host side) for given values: n samples (values[n], coords[n][3]
+ * as reference x, y, z) are splatted onto the nplanes planes target_z
+ * (device) with |z - z0| <= z_window: to the nearest pixel (nearest = 1) or
+ * to the footprint floor(x) - r .. floor(x) + 1 + r (same in y), pixels mapped
+ * to the output grid by integer division by downsample_xy.  out (device,
+ * nplanes x xout x yout, y fastest, xout = ceil(xref / ds)) keeps each
+ * pixel's maximum; with fill = 1 empty pixels get fill_value, else -inf. */
+int wr_project_planes_max(wr_ctx* ctx, const float* values, const float* coords, long long n,
+                          const float* target_z, int nplanes, int xref, int yref, int downsample_xy,
+                          double z_window, int nearest, int extra_radius, int fill, double fill_value,
+                          float* out, void* stream);
+
+/* Replaces the weighted splat of project_coords_to_fixed_planes_weighted_gpu
+ * (calFlowCrossResolution.py:571-669 kernel, :700-875 host side) for given
+ * values: z weight mode 0 = hard, 1 = triangular, 2 = Gaussian (z_sigma <= 0:
+ * z_window / 2) times bilinear weights at (x / ds, y / ds); samples with x
+ * outside [0, xref - 1] or y outside [0, yref - 1] are dropped.
+ * weight_out = sum of weights, out = sum(v * w) / sum(w) where sum(w) > eps,
+ * else fill_value (both device, nplanes x xout x yout). */
+int wr_project_planes_weighted(wr_ctx* ctx, const float* values, const float* coords, long long n,
+                               const float* target_z, int nplanes, int xref, int yref,
+                               int downsample_xy, double z_window, double z_sigma, int mode, double eps,
+                               double fill_value, float* out, float* weight_out, void* stream);
+
 /* Launch statistics of the last wr_get_motion on this context, for
  * measurement: LK passes launched per level, the LK kernel's tile shape and
  * grid size, and (when params->timing was 1) the number and summed HIP-event

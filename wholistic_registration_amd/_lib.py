@@ -88,6 +88,10 @@ _SIGNATURES = {
     "wr_add_volumes": ([_VP, _VP, _I, _VP, ctypes.c_longlong, _VP, _VP], _I),
     "wr_frames_mean": ([_VP, ctypes.POINTER(_VP), _I, _I, ctypes.c_longlong, _VP, _VP], _I),
     "wr_frame_gram": ([_VP, _VP, _I, _I, _I, _I, _I, _I, ctypes.POINTER(ctypes.c_double), _VP], _I),
+    "wr_project_planes_max": ([_VP, _VP, _VP, ctypes.c_longlong, _VP, _I, _I, _I, _I, ctypes.c_double, _I, _I, _I,
+                               ctypes.c_double, _VP, _VP], _I),
+    "wr_project_planes_weighted": ([_VP, _VP, _VP, ctypes.c_longlong, _VP, _I, _I, _I, _I, ctypes.c_double,
+                                    ctypes.c_double, _I, ctypes.c_double, ctypes.c_double, _VP, _VP, _VP], _I),
 }
 
 _lib = None

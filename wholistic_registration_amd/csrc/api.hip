@@ -178,7 +178,7 @@ void wr_destroy(wr_ctx* ctx) {
     (void)hipSetDevice(c.device);
     for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
                       &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen,
-                      &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig})
+                      &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig, &c.splat})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c.lvl_events) (void)hipEventDestroy(e);
@@ -792,6 +792,52 @@ int wr_frame_gram(wr_ctx* ctx, const void* slab, int dtype, int nt, int nzs, int
     WR_HIP_CHECK(hipGetLastError());
     WR_HIP_CHECK(hipMemcpyAsync(cc, dcc, nt2 * sizeof(double), hipMemcpyDeviceToHost, s));
     WR_HIP_CHECK(hipStreamSynchronize(s));
+    return WR_OK;
+}
+
+
+// ---------------------------------------------------------------- splats
+
+int wr_project_planes_max(wr_ctx* ctx, const float* values, const float* coords, long long n, const float* target_z,
+                          int nplanes, int xref, int yref, int downsample_xy, double z_window, int nearest,
+                          int extra_radius, int fill, double fill_value, float* out, void* stream) {
+    WR_ARG(ctx && (n == 0 || (values && coords)) && target_z && out, "wr_project_planes_max: null argument");
+    WR_ARG(n >= 0 && nplanes > 0 && xref > 0 && yref > 0, "wr_project_planes_max: empty shape");
+    WR_ARG(downsample_xy >= 1, "downsample_xy should be >= 1.");
+    WR_ARG(extra_radius >= 0, "xy_extra_radius should be >= 0.");
+    const int xout = (xref + downsample_xy - 1) / downsample_xy, yout = (yref + downsample_xy - 1) / downsample_xy;
+    WR_ARG(static_cast<double>(nplanes) * xout * yout < 9.0e18, "wr_project_planes_max: output too large");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    WR_TRY(grow(c.splat, 512));
+    unsigned* kmin = c.splat.as<unsigned>();
+    float* shift = reinterpret_cast<float*>(c.splat.as<char>() + 256);
+    launch_splat_max(values, coords, n, target_z, nplanes, xref, yref, xout, yout, static_cast<float>(z_window),
+                     downsample_xy, nearest, extra_radius, fill, static_cast<float>(fill_value), kmin, shift, out,
+                     static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_project_planes_weighted(wr_ctx* ctx, const float* values, const float* coords, long long n,
+                               const float* target_z, int nplanes, int xref, int yref, int downsample_xy,
+                               double z_window, double z_sigma, int mode, double eps, double fill_value, float* out,
+                               float* weight_out, void* stream) {
+    WR_ARG(ctx && (n == 0 || (values && coords)) && target_z && out && weight_out,
+           "wr_project_planes_weighted: null argument");
+    WR_ARG(n >= 0 && nplanes > 0 && xref > 0 && yref > 0, "wr_project_planes_weighted: empty shape");
+    WR_ARG(downsample_xy >= 1, "downsample_xy should be >= 1.");
+    WR_ARG(mode >= 0 && mode <= 2, "z_weight_mode should be 'hard', 'triangular', or 'gaussian'.");
+    const int xout = (xref + downsample_xy - 1) / downsample_xy, yout = (yref + downsample_xy - 1) / downsample_xy;
+    const size_t nout = static_cast<size_t>(nplanes) * xout * yout;
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    WR_TRY(grow(c.splat, 512 + nout * sizeof(float)));
+    float* sum_val = reinterpret_cast<float*>(c.splat.as<char>() + 512);
+    launch_splat_weighted(values, coords, n, target_z, nplanes, xref, yref, xout, yout, static_cast<float>(z_window),
+                          static_cast<float>(z_sigma), downsample_xy, mode, static_cast<float>(eps),
+                          static_cast<float>(fill_value), sum_val, weight_out, out, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
     return WR_OK;
 }
 

@@ -307,6 +307,19 @@ void launch_divide(void* acc, Elem t, long long n, int count, hipStream_t s);
 void launch_frame_gram(const void* slab, Elem t, int nt, int nzs, int ny, int nx, int ds, float* m, float* mean,
                        double* cc, hipStream_t s);
 
+// ---------------------------------------------------------------- splats
+
+// Max splat of n samples onto nplanes target planes (out: nplanes * xout *
+// yout floats, (plane, x, y)); kmin / shift: device scratch.
+void launch_splat_max(const float* values, const float* coords, long long n, const float* target_z, int nplanes,
+                      int xref, int yref, int xout, int yout, float z_window, int ds, int nearest, int radius,
+                      int fill, float fill_value, unsigned* kmin, float* shift, float* out, hipStream_t s);
+// Weighted splat: sum_val and sum_w accumulate, out = sum_val / sum_w where sum_w > eps.
+void launch_splat_weighted(const float* values, const float* coords, long long n, const float* target_z,
+                           int nplanes, int xref, int yref, int xout, int yout, float z_window, float z_sigma, int ds,
+                           int mode, float eps, float fill_value, float* sum_val, float* sum_w, float* out,
+                           hipStream_t s);
+
 // ---------------------------------------------------------------- context
 
 struct Context {
@@ -316,6 +329,7 @@ struct Context {
     DevBuf tab_w, tab_i, pen;
     DevBuf msel, mlab, mcnt, mtmp;       // mask statistics, component labels / areas, byte mask
     DevBuf hsmall, hbig;                 // harness: statistics / Gram scratch, sampled frame matrix
+    DevBuf splat;                        // splats: shift scratch and the weighted sums
     std::vector<hipEvent_t> events;
     hipStream_t side = nullptr;          // pyramid builds, ahead of the iterations
     std::vector<hipEvent_t> lvl_events;  // [L]: level L's images are ready; [layer+1]: fork
