@@ -20,8 +20,14 @@
 //                   their ratio where sum(w) > eps.
 // Unlike the reference's launch (one thread per sample and plane), a thread
 // takes one sample and walks the planes, so each coordinate is read once.
-// Outputs are (plane, x_out, y_out), y fastest.
+// Samples are stored with z fastest, so the lanes of a wave usually hold the
+// z column of one (x, y) and hit the same output pixels: when every
+// contributing lane of a wave has the same footprint, the wave reduces the
+// contributions first (maximum, or sums) and one lane issues one atomic per
+// pixel; otherwise every lane issues its own.  Outputs are (plane, x_out,
+// y_out), y fastest.
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 
@@ -78,17 +84,39 @@ struct SplatGeom {
     float z_window;
 };
 
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The lane of the wave holding the first set bit of `mask`, and whether every
+// set lane has the same (a, b) as that lane.
+__device__ __forceinline__ bool same_key(unsigned long long mask, bool on, int a, int b, int& first) {
+    first = __ffsll(static_cast<long long>(mask)) - 1;
+    const int ka = __shfl(a, first, 64), kb = __shfl(b, first, 64);
+    return __ballot(on && (a != ka || b != kb)) == 0ull;
+}
+
 __global__ __launch_bounds__(kT) void k_splat_max(const float* __restrict__ values, const float* __restrict__ coords,
                                                   const float* __restrict__ target_z, SplatGeom g, int nearest,
                                                   int radius, const float* __restrict__ shift,
                                                   float* __restrict__ out) {
     const long long i = static_cast<long long>(blockIdx.x) * kT + threadIdx.x;
-    if (i >= g.n) return;
-    const float val = values[i] + *shift;
-    if (!isfinite(val)) return;
-    const float x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
-    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) return;
-    int xs, xe, ys, ye;
+    bool ok = i < g.n;
+    const float val = ok ? values[i] + *shift : 0.0f;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (ok) {
+        x = coords[3 * i];
+        y = coords[3 * i + 1];
+        z = coords[3 * i + 2];
+    }
+    ok = ok && isfinite(val) && isfinite(x) && isfinite(y) && isfinite(z);
+    if (!__ballot(ok)) return;           // uniform: no sample of this wave contributes
+    int xs = 0, xe = -1, ys = 0, ye = -1;
     if (nearest) {
         xs = xe = static_cast<int>(floorf(x + 0.5f));
         ys = ye = static_cast<int>(floorf(y + 0.5f));
@@ -101,8 +129,33 @@ __global__ __launch_bounds__(kT) void k_splat_max(const float* __restrict__ valu
     }
     const int bits = __float_as_int(val);
     const long long plane_sz = static_cast<long long>(g.xout) * g.yout;
+    const int lane = threadIdx.x & 63;
     for (int p = 0; p < g.nplanes; ++p) {
-        if (fabsf(z - target_z[p]) > g.z_window) continue;
+        const bool on = ok && fabsf(z - target_z[p]) <= g.z_window;
+        const unsigned long long mask = __ballot(on);
+        if (!mask) continue;
+        int first;
+        if (same_key(mask, on, xs, ys, first)) {
+            // one footprint for the wave: its maximum per pixel, one atomic each
+            const int xs0 = __shfl(xs, first, 64), ys0 = __shfl(ys, first, 64);
+            const int xe0 = __shfl(xe, first, 64), ye0 = __shfl(ye, first, 64);
+            for (int xi = xs0; xi <= xe0; ++xi) {
+                if (xi < 0 || xi >= g.xref) continue;
+                const int xo = xi / g.ds;
+                if (xo >= g.xout) continue;
+                for (int yi = ys0; yi <= ye0; ++yi) {
+                    if (yi < 0 || yi >= g.yref) continue;
+                    const int yo = yi / g.ds;
+                    if (yo >= g.yout) continue;
+                    const int m = wave_max(on ? bits : INT_MIN);
+                    if (lane == first)
+                        atomicMax(reinterpret_cast<int*>(out) + p * plane_sz + static_cast<long long>(xo) * g.yout + yo,
+                                  m);
+                }
+            }
+            continue;
+        }
+        if (!on) continue;
         for (int xi = xs; xi <= xe; ++xi) {
             if (xi < 0 || xi >= g.xref) continue;
             const int xo = xi / g.ds;
@@ -134,20 +187,25 @@ __global__ __launch_bounds__(kT) void k_splat_weighted(const float* __restrict__
                                                        int mode, float* __restrict__ sum_val,
                                                        float* __restrict__ sum_w) {
     const long long i = static_cast<long long>(blockIdx.x) * kT + threadIdx.x;
-    if (i >= g.n) return;
-    const float val = values[i];
-    if (!isfinite(val)) return;
-    const float x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
-    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) return;
-    if (x < 0.0f || x > static_cast<float>(g.xref - 1) || y < 0.0f || y > static_cast<float>(g.yref - 1)) return;
+    bool ok = i < g.n;
+    const float val = ok ? values[i] : 0.0f;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (ok) {
+        x = coords[3 * i];
+        y = coords[3 * i + 1];
+        z = coords[3 * i + 2];
+    }
+    ok = ok && isfinite(val) && isfinite(x) && isfinite(y) && isfinite(z) && x >= 0.0f &&
+         x <= static_cast<float>(g.xref - 1) && y >= 0.0f && y <= static_cast<float>(g.yref - 1);
+    if (!__ballot(ok)) return;           // uniform: no sample of this wave contributes
     const float dsf = static_cast<float>(g.ds);
     const float xo_f = x / dsf, yo_f = y / dsf;
     const int xo0 = static_cast<int>(floorf(xo_f)), yo0 = static_cast<int>(floorf(yo_f));
     const float dx = xo_f - static_cast<float>(xo0), dy = yo_f - static_cast<float>(yo0);
     const long long plane_sz = static_cast<long long>(g.xout) * g.yout;
+    const int lane = threadIdx.x & 63;
     for (int p = 0; p < g.nplanes; ++p) {
         const float dz = fabsf(z - target_z[p]);
-        if (dz > g.z_window) continue;
         float wz = 1.0f;
         if (mode == 1) {
             wz = 1.0f - dz / (g.z_window + 1e-6f);
@@ -156,7 +214,35 @@ __global__ __launch_bounds__(kT) void k_splat_weighted(const float* __restrict__
             const float sigma = z_sigma > 0.0f ? z_sigma : g.z_window * 0.5f;
             wz = expf(-0.5f * dz * dz / (sigma * sigma + 1e-6f));
         }
-        if (wz <= 0.0f || !isfinite(wz)) continue;
+        const bool on = ok && dz <= g.z_window && wz > 0.0f && isfinite(wz);
+        const unsigned long long mask = __ballot(on);
+        if (!mask) continue;
+        int first;
+        if (same_key(mask, on, xo0, yo0, first)) {
+            // one 2x2 footprint for the wave: per pixel the wave's sums, one atomic pair
+            const int bx = __shfl(xo0, first, 64), by = __shfl(yo0, first, 64);
+            for (int ax = 0; ax <= 1; ++ax) {
+                const int xo = bx + ax;
+                if (xo < 0 || xo >= g.xout) continue;
+                const float wx = ax == 0 ? 1.0f - dx : dx;
+                for (int ay = 0; ay <= 1; ++ay) {
+                    const int yo = by + ay;
+                    if (yo < 0 || yo >= g.yout) continue;
+                    const float wy = ay == 0 ? 1.0f - dy : dy;
+                    const float w = on ? wx * wy * wz : 0.0f;
+                    const bool use = w > 0.0f && isfinite(w);
+                    const float sv = wave_sum(use ? val * w : 0.0f);
+                    const float sw = wave_sum(use ? w : 0.0f);
+                    if (lane == first && sw > 0.0f) {
+                        const long long o = p * plane_sz + static_cast<long long>(xo) * g.yout + yo;
+                        atomicAdd(sum_val + o, sv);
+                        atomicAdd(sum_w + o, sw);
+                    }
+                }
+            }
+            continue;
+        }
+        if (!on) continue;
         for (int ax = 0; ax <= 1; ++ax) {
             const int xo = xo0 + ax;
             if (xo < 0 || xo >= g.xout) continue;
