@@ -225,7 +225,10 @@ def wbi_registration_3d(
         if channels["dual_channel"]:
             corrected_ca = calFlow3d_Wei_v1.correctMotion(ca_zyx.permute(2, 1, 0), motion_current)
         corrected_mem = calFlow3d_Wei_v1.correctMotion(dat_mem, motion_current)
-        corrected_mov = calFlow3d_Wei_v1.correctMotion(dat_mov, motion_current)
+        # without a calcium channel the moving volume is the membrane volume:
+        # its warp is the same array (the warp is deterministic)
+        corrected_mov = (corrected_mem if mov_zyx is mem_zyx
+                         else calFlow3d_Wei_v1.correctMotion(dat_mov, motion_current))
         initial_error = _mse(dat_mov, dat_ref, dev)
         eventual_error = _mse(corrected_mov, dat_ref, dev)
         if verbose:
