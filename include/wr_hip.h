@@ -79,6 +79,14 @@ typedef struct wr_params {
                                precision: its pyramid levels are not rebuilt
                                (frames of a time series against one template);
                                a mismatch of those is an error */
+    int init_nx;            /* with init_ny: 0, 0 = motion_init is (nz, ny, nx, 3); else
+                               motion_init is (nz, init_ny, init_nx, 3), still
+                               in full-resolution voxel units: the reference's
+                               warm start imresizes option["motion"] of any
+                               x-y size to the coarsest level and divides by
+                               the full-to-level size ratio
+                               (calFlow3d_Wei_v1.py:362-377) */
+    int init_ny;
 } wr_params;
 
 /* Per-level iteration log, filled on the host when wr_get_motion returns
@@ -277,6 +285,61 @@ typedef struct wr_stats {
     double lk_ms[WR_MAX_LEVELS];  /* summed duration of those passes, ms    */
 } wr_stats;
 int wr_get_stats(wr_ctx* ctx, wr_stats* out);
+
+/* ---- Learned initialisation.  No counterpart in the reference: these are
+ * the layers of a 3-D U-Net whose predicted field enters getMotion through
+ * its own warm start, option["motion"] (calFlow3d_Wei_v1.py:362-377;
+ * wr_params.init_nx / init_ny take it at the network's x-y size).
+ * Tensors are device, channels-last (batch, d, h, w, c) bfloat16 (uint16
+ * bits) unless float32 is stated; weights (cout, 27, cin), tap
+ * (kd * 3 + kh) * 3 + kw at offset (kd - 1, kh - 1, kw - 1); padding 1.
+ * Every call is asynchronous on `stream`.  Buffers read or written 8
+ * channels at a time must be 16-byte aligned. */
+
+/* y = conv3x3x3(x, w) + bias (float32, nullable), then LeakyReLU(slope) when
+ * leaky = 1; float32 accumulation (MFMA); cin % 8 == 0, cout % 16 == 0. */
+int wr_net_conv3d(wr_ctx* ctx, const void* x, const void* w, const float* bias, void* y, int batch, int d, int h,
+                  int wd, int cin, int cout, int leaky, double slope, void* stream);
+/* dx (batch, d, h, w, cin) = gradient of the convolution's input for dy
+ * (batch, d, h, w, cout); cin, cout multiples of 16. */
+int wr_net_conv3d_bwd_data(wr_ctx* ctx, const void* dy, const void* w, void* dx, int batch, int d, int h, int wd,
+                           int cin, int cout, void* stream);
+/* dw (float32, cout x 27 x cin) += gradient of the weights, db (float32, cout,
+ * nullable) += gradient of the bias; w % 8 == 0.  Float32 atomics: the
+ * summation order varies between runs. */
+int wr_net_conv3d_bwd_weight(wr_ctx* ctx, const void* dy, const void* x, float* dw, float* db, int batch, int d,
+                             int h, int wd, int cin, int cout, void* stream);
+/* grad *= (y > 0 ? 1 : slope) over n values (the LeakyReLU backward from its output). */
+int wr_net_leaky_bwd(wr_ctx* ctx, void* grad, const void* y, long long n, double slope, void* stream);
+/* y (batch, d/2, h/2, w/2, c) = scale * sum of the 2x2x2 children in x
+ * (batch, d, h, w, c): average pooling (scale 1/8) and the backward of
+ * nearest upsampling (scale 1). */
+int wr_net_down2(wr_ctx* ctx, const void* x, void* y, int batch, int d, int h, int wd, int c, double scale,
+                 void* stream);
+/* y (batch, d, h, w, c) = (y +, when accumulate = 1) scale * the parent voxel
+ * in x (batch, d/2, h/2, w/2, c): nearest upsampling (scale 1) and the
+ * backward of average pooling (scale 1/8). */
+int wr_net_up2(wr_ctx* ctx, const void* x, void* y, int batch, int d, int h, int wd, int c, double scale,
+               int accumulate, void* stream);
+/* y[v] = (a[v], b[v]) over nvox voxels; wr_net_split is its backward. */
+int wr_net_concat(wr_ctx* ctx, const void* a, int ca, const void* b, int cb, long long nvox, void* y, void* stream);
+int wr_net_split(wr_ctx* ctx, const void* g, int ca, int cb, long long nvox, void* da, void* db, void* stream);
+/* *loss (float32, device) += scale * sum over voxels and channels c < ct of
+ * (pscale * p[v][c] - t[v][c])^2, t float32 (nvox, ct); dp (nvox, cp) = the
+ * gradient with respect to p (zero in channels ct..cp-1). */
+int wr_net_mse(wr_ctx* ctx, const void* p, int cp, const float* t, int ct, long long nvox, double pscale,
+               double scale, void* dp, float* loss, void* stream);
+/* One Adam step (Kingma & Ba 2015) on n float32 parameters w with gradient
+ * g and moments m, v; step counts from 1; w16 (nullable) receives bfloat16
+ * copies of w. */
+int wr_net_adam(wr_ctx* ctx, float* w, const float* g, float* m, float* v, void* w16, long long n, double lr,
+                double beta1, double beta2, double eps, int step, void* stream);
+int wr_net_to_bf16(wr_ctx* ctx, const float* in, long long n, void* out, void* stream);
+/* out (n, 8) = ((mov - mean) / std, (ref - mean) / std, 0, ...) with the
+ * mean and standard deviation of ref's n float32 values. */
+int wr_net_input(wr_ctx* ctx, const float* mov, const float* ref, long long n, void* out, void* stream);
+/* field (n, 3) float32 = scale * p[:, 0:3] of p (n, cp). */
+int wr_net_output(wr_ctx* ctx, const void* p, int cp, long long n, double scale, float* field, void* stream);
 
 #ifdef __cplusplus
 }

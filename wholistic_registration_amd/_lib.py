@@ -36,6 +36,8 @@ class WrParams(ctypes.Structure):
         ("stop_kernels", ctypes.c_int),
         ("precision", ctypes.c_int),
         ("reuse_template", ctypes.c_int),
+        ("init_nx", ctypes.c_int),
+        ("init_ny", ctypes.c_int),
     ]
 
 
@@ -92,6 +94,21 @@ _SIGNATURES = {
                                ctypes.c_double, _VP, _VP], _I),
     "wr_project_planes_weighted": ([_VP, _VP, _VP, ctypes.c_longlong, _VP, _I, _I, _I, _I, ctypes.c_double,
                                     ctypes.c_double, _I, ctypes.c_double, ctypes.c_double, _VP, _VP, _VP], _I),
+    "wr_net_conv3d": ([_VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double, _VP], _I),
+    "wr_net_conv3d_bwd_data": ([_VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP], _I),
+    "wr_net_conv3d_bwd_weight": ([_VP, _VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP], _I),
+    "wr_net_leaky_bwd": ([_VP, _VP, _VP, ctypes.c_longlong, ctypes.c_double, _VP], _I),
+    "wr_net_down2": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, ctypes.c_double, _VP], _I),
+    "wr_net_up2": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, ctypes.c_double, _I, _VP], _I),
+    "wr_net_concat": ([_VP, _VP, _I, _VP, _I, ctypes.c_longlong, _VP, _VP], _I),
+    "wr_net_split": ([_VP, _VP, _I, _I, ctypes.c_longlong, _VP, _VP, _VP], _I),
+    "wr_net_mse": ([_VP, _VP, _I, _VP, _I, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _VP, _VP, _VP],
+                   _I),
+    "wr_net_adam": ([_VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_longlong, ctypes.c_double, ctypes.c_double,
+                     ctypes.c_double, ctypes.c_double, _I, _VP], _I),
+    "wr_net_to_bf16": ([_VP, _VP, ctypes.c_longlong, _VP, _VP], _I),
+    "wr_net_input": ([_VP, _VP, _VP, ctypes.c_longlong, _VP, _VP], _I),
+    "wr_net_output": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, _VP, _VP], _I),
 }
 
 _lib = None

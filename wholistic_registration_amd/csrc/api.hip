@@ -178,7 +178,7 @@ void wr_destroy(wr_ctx* ctx) {
     (void)hipSetDevice(c.device);
     for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
                       &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen,
-                      &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig, &c.splat})
+                      &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig, &c.splat, &c.netw})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c.lvl_events) (void)hipEventDestroy(e);
@@ -324,6 +324,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     std::vector<LevelGeom> geo(layer + 1);
     std::vector<LkTile> tiles(layer + 1);
     std::vector<Plan2> down(layer + 1), up(layer + 1);
+    Plan2 init_plan{};
+    WR_ARG((prm->init_nx > 0) == (prm->init_ny > 0), "wr_get_motion: init_nx and init_ny must both be set or both 0");
     std::vector<double> wa;
     std::vector<int> ia;
     size_t tmp_need = 0, ncp_max = 0, nblk_max = 0, ncpb_max = 0, nvox_prev = 0;
@@ -347,6 +349,10 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         tiles[L] = plan_tile(g, prm->tile_patches_x);
         down[L] = make_plan(nx, g.nx, ny, g.ny, KERNEL_CUBIC, wa, ia);
         tmp_need = std::max(tmp_need, tmp_bytes(down[L], nz));
+        if (L == layer && motion_init && prm->init_nx > 0) {
+            init_plan = make_plan(prm->init_nx, g.nx, prm->init_ny, g.ny, KERNEL_CUBIC, wa, ia);
+            tmp_need = std::max(tmp_need, tmp_bytes(init_plan, nz));
+        }
         if (L < layer) {
             up[L] = make_plan(geo[L + 1].nx, g.nx, geo[L + 1].ny, g.ny, KERNEL_TENT, wa, ia);
             tmp_need = std::max(tmp_need, tmp_bytes(up[L], nz));
@@ -476,7 +482,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             if (motion_init) {
                 const double div[3] = {1.0 * nx / g.nx, 1.0 * ny / g.ny, 1.0 * nz / g.nz};
                 for (int q = 0; q < 3; ++q)
-                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q], down[L], tw, ti, nz, tmp2,
+                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q],
+                              prm->init_nx > 0 ? init_plan : down[L], tw, ti, nz, tmp2,
                               dense(g.nx), s);
             } else {
                 fill(m0, 3 * nv, 0.0f, s);
@@ -837,6 +844,195 @@ int wr_project_planes_weighted(wr_ctx* ctx, const float* values, const float* co
     launch_splat_weighted(values, coords, n, target_z, nplanes, xref, yref, xout, yout, static_cast<float>(z_window),
                           static_cast<float>(z_sigma), downsample_xy, mode, static_cast<float>(eps),
                           static_cast<float>(fill_value), sum_val, weight_out, out, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+// ---------------------------------------------------------------- learned initialisation
+
+}  // extern "C"
+
+namespace {
+bool al16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+int net_geom(int batch, int d, int h, int w, NetGeom* g) {
+    WR_ARG(batch > 0 && d > 0 && h > 0 && w > 0, "net: empty shape");
+    WR_ARG(static_cast<double>(batch) * d * h * w * 128.0 < 9.0e18, "net: volume too large");
+    WR_ARG(static_cast<double>(d) * h * w / 256.0 < 2147483647.0, "net: volume too large for one launch");
+    WR_ARG(batch <= 65535, "net: batch above 65535");
+    g->d = d;
+    g->h = h;
+    g->w = w;
+    return WR_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int wr_net_conv3d(wr_ctx* ctx, const void* x, const void* w, const float* bias, void* y, int batch, int d, int h,
+                  int wd, int cin, int cout, int leaky, double slope, void* stream) {
+    WR_ARG(ctx && x && w && y, "wr_net_conv3d: null argument");
+    WR_ARG(cin > 0 && cin % 8 == 0 && cout > 0 && cout % 16 == 0 && cout / 16 <= 65535,
+           "wr_net_conv3d: cin must be a multiple of 8 and cout of 16");
+    WR_ARG(al16(x) && al16(w) && al16(y), "wr_net_conv3d: x, w and y must be 16-byte aligned");
+    NetGeom g;
+    WR_TRY(net_geom(batch, d, h, wd, &g));
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_conv_fwd(static_cast<const uint16_t*>(x), static_cast<const uint16_t*>(w), bias, static_cast<uint16_t*>(y),
+                    g, batch, cin, cout, leaky, static_cast<float>(slope), static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_conv3d_bwd_data(wr_ctx* ctx, const void* dy, const void* w, void* dx, int batch, int d, int h, int wd,
+                           int cin, int cout, void* stream) {
+    WR_ARG(ctx && dy && w && dx, "wr_net_conv3d_bwd_data: null argument");
+    WR_ARG(cin > 0 && cin % 16 == 0 && cout > 0 && cout % 16 == 0 && cin / 16 <= 65535,
+           "wr_net_conv3d_bwd_data: cin and cout must be multiples of 16");
+    WR_ARG(al16(dy) && al16(dx), "wr_net_conv3d_bwd_data: dy and dx must be 16-byte aligned");
+    NetGeom g;
+    WR_TRY(net_geom(batch, d, h, wd, &g));
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    const size_t nw = 27ull * cin * cout;
+    WR_TRY(grow(c.netw, nw * sizeof(uint16_t)));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint16_t* wf = c.netw.as<uint16_t>();
+    launch_flip_weights(static_cast<const uint16_t*>(w), cin, cout, wf, s);
+    launch_conv_fwd(static_cast<const uint16_t*>(dy), wf, nullptr, static_cast<uint16_t*>(dx), g, batch, cout, cin, 0,
+                    0.0f, s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_conv3d_bwd_weight(wr_ctx* ctx, const void* dy, const void* x, float* dw, float* db, int batch, int d,
+                             int h, int wd, int cin, int cout, void* stream) {
+    WR_ARG(ctx && dy && x && dw, "wr_net_conv3d_bwd_weight: null argument");
+    WR_ARG(cin > 0 && cin % 8 == 0 && cout > 0 && cout % 16 == 0 && cout / 16 <= 65535,
+           "wr_net_conv3d_bwd_weight: cin must be a multiple of 8 and cout of 16");
+    WR_ARG(wd % 8 == 0, "wr_net_conv3d_bwd_weight: w must be a multiple of 8");
+    NetGeom g;
+    WR_TRY(net_geom(batch, d, h, wd, &g));
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_conv_wgrad(static_cast<const uint16_t*>(dy), static_cast<const uint16_t*>(x), g, batch, cin, cout, dw, db,
+                      static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_leaky_bwd(wr_ctx* ctx, void* grad, const void* y, long long n, double slope, void* stream) {
+    WR_ARG(ctx && grad && y, "wr_net_leaky_bwd: null argument");
+    WR_ARG(n > 0 && n % 8 == 0, "wr_net_leaky_bwd: n must be a positive multiple of 8");
+    WR_ARG(al16(grad) && al16(y), "wr_net_leaky_bwd: buffers must be 16-byte aligned");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_lrelu_bwd(static_cast<uint16_t*>(grad), static_cast<const uint16_t*>(y), n, static_cast<float>(slope),
+                     static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_down2(wr_ctx* ctx, const void* x, void* y, int batch, int d, int h, int wd, int c, double scale,
+                 void* stream) {
+    WR_ARG(ctx && x && y, "wr_net_down2: null argument");
+    WR_ARG(c > 0 && c % 8 == 0, "wr_net_down2: c must be a multiple of 8");
+    WR_ARG(d % 2 == 0 && h % 2 == 0 && wd % 2 == 0, "wr_net_down2: d, h and w must be even");
+    WR_ARG(al16(x) && al16(y), "wr_net_down2: buffers must be 16-byte aligned");
+    NetGeom g;
+    WR_TRY(net_geom(batch, d, h, wd, &g));
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_down2(static_cast<const uint16_t*>(x), g, batch, c, static_cast<float>(scale), static_cast<uint16_t*>(y),
+                 static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_up2(wr_ctx* ctx, const void* x, void* y, int batch, int d, int h, int wd, int c, double scale,
+               int accumulate, void* stream) {
+    WR_ARG(ctx && x && y, "wr_net_up2: null argument");
+    WR_ARG(c > 0 && c % 8 == 0, "wr_net_up2: c must be a multiple of 8");
+    WR_ARG(d % 2 == 0 && h % 2 == 0 && wd % 2 == 0, "wr_net_up2: d, h and w must be even");
+    WR_ARG(al16(x) && al16(y), "wr_net_up2: buffers must be 16-byte aligned");
+    NetGeom g;
+    WR_TRY(net_geom(batch, d, h, wd, &g));
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_up2(static_cast<const uint16_t*>(x), g, batch, c, static_cast<float>(scale), accumulate,
+               static_cast<uint16_t*>(y), static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_concat(wr_ctx* ctx, const void* a, int ca, const void* b, int cb, long long nvox, void* y, void* stream) {
+    WR_ARG(ctx && a && b && y, "wr_net_concat: null argument");
+    WR_ARG(ca > 0 && cb > 0 && ca % 8 == 0 && cb % 8 == 0 && nvox > 0, "wr_net_concat: bad channel counts");
+    WR_ARG(al16(a) && al16(b) && al16(y), "wr_net_concat: buffers must be 16-byte aligned");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_concat(static_cast<const uint16_t*>(a), ca, static_cast<const uint16_t*>(b), cb, nvox,
+                  static_cast<uint16_t*>(y), static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_split(wr_ctx* ctx, const void* g, int ca, int cb, long long nvox, void* da, void* db, void* stream) {
+    WR_ARG(ctx && g && da && db, "wr_net_split: null argument");
+    WR_ARG(ca > 0 && cb > 0 && ca % 8 == 0 && cb % 8 == 0 && nvox > 0, "wr_net_split: bad channel counts");
+    WR_ARG(al16(g) && al16(da) && al16(db), "wr_net_split: buffers must be 16-byte aligned");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_split(static_cast<const uint16_t*>(g), ca, cb, nvox, static_cast<uint16_t*>(da), static_cast<uint16_t*>(db),
+                 static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_mse(wr_ctx* ctx, const void* p, int cp, const float* t, int ct, long long nvox, double pscale,
+               double scale, void* dp, float* loss, void* stream) {
+    WR_ARG(ctx && p && t && dp && loss, "wr_net_mse: null argument");
+    WR_ARG(nvox > 0 && ct > 0 && ct <= cp, "wr_net_mse: bad sizes");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_mse(static_cast<const uint16_t*>(p), cp, t, ct, nvox, static_cast<float>(pscale),
+               static_cast<float>(scale), static_cast<uint16_t*>(dp), loss, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_adam(wr_ctx* ctx, float* w, const float* g, float* m, float* v, void* w16, long long n, double lr,
+                double beta1, double beta2, double eps, int step, void* stream) {
+    WR_ARG(ctx && w && g && m && v, "wr_net_adam: null argument");
+    WR_ARG(n > 0 && step >= 1, "wr_net_adam: n must be positive and step at least 1");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_adam(w, g, m, v, static_cast<uint16_t*>(w16), n, static_cast<float>(lr), static_cast<float>(beta1),
+                static_cast<float>(beta2), static_cast<float>(eps), step, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_to_bf16(wr_ctx* ctx, const float* in, long long n, void* out, void* stream) {
+    WR_ARG(ctx && in && out && n > 0, "wr_net_to_bf16: bad argument");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_to_bf16(in, n, static_cast<uint16_t*>(out), static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_input(wr_ctx* ctx, const float* mov, const float* ref, long long n, void* out, void* stream) {
+    WR_ARG(ctx && mov && ref && out && n > 0, "wr_net_input: bad argument");
+    WR_ARG(al16(out), "wr_net_input: out must be 16-byte aligned");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* part;
+    float* ms;
+    WR_TRY(harness_small(c, &part, &ms));
+    launch_mean_std(ref, n, part, ms, s);
+    launch_net_input(mov, ref, n, ms, static_cast<uint16_t*>(out), s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_output(wr_ctx* ctx, const void* p, int cp, long long n, double scale, float* field, void* stream) {
+    WR_ARG(ctx && p && field && n > 0 && cp >= 3, "wr_net_output: bad argument");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_net_output(static_cast<const uint16_t*>(p), cp, n, static_cast<float>(scale), field,
+                      static_cast<hipStream_t>(stream));
     WR_HIP_CHECK(hipGetLastError());
     return WR_OK;
 }

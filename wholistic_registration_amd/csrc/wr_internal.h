@@ -320,6 +320,46 @@ void launch_splat_weighted(const float* values, const float* coords, long long n
                            int mode, float eps, float fill_value, float* sum_val, float* sum_w, float* out,
                            hipStream_t s);
 
+// ---------------------------------------------------------------- learned initialisation
+
+// Extents of one channels-last activation volume (B, d, h, w, C).
+struct NetGeom {
+    int d, h, w;
+};
+// 3x3x3 convolution, padding 1: y (B, d, h, w, cout) = act(conv(x, w) + bias);
+// bfloat16 x, w (cout, 27, cin), y; float32 bias (nullable); cin % 8 == 0,
+// cout % 16 == 0; act: LeakyReLU with slope.
+void launch_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, NetGeom g, int batch,
+                     int cin, int cout, int act, float slope, hipStream_t s);
+// wf (cin, 27, cout) = w (cout, 27, cin) with the taps reversed.
+void launch_flip_weights(const uint16_t* w, int cin, int cout, uint16_t* wf, hipStream_t s);
+// dw (cout, 27, cin) += sum over voxels of dy (x) x, db (cout, nullable) +=
+// sum of dy; float32; g.w % 8 == 0.
+void launch_conv_wgrad(const uint16_t* dy, const uint16_t* x, NetGeom g, int batch, int cin, int cout, float* dw,
+                       float* db, hipStream_t s);
+// gr *= (y > 0 ? 1 : slope) over n values (n % 8 == 0).
+void launch_lrelu_bwd(uint16_t* gr, const uint16_t* y, long long n, float slope, hipStream_t s);
+// y (B, d/2, h/2, w/2, c) = scale * sum of the 8 children of x (B, d, h, w, c).
+void launch_down2(const uint16_t* x, NetGeom g, int batch, int c, float scale, uint16_t* y, hipStream_t s);
+// y (B, d, h, w, c) (+)= scale * x at the parent voxel of (B, d/2, h/2, w/2, c).
+void launch_up2(const uint16_t* x, NetGeom g, int batch, int c, float scale, int accumulate, uint16_t* y,
+                hipStream_t s);
+void launch_concat(const uint16_t* a, int ca, const uint16_t* b, int cb, long long nvox, uint16_t* y, hipStream_t s);
+void launch_split(const uint16_t* gr, int ca, int cb, long long nvox, uint16_t* da, uint16_t* db, hipStream_t s);
+// loss += scale * sum over voxels and channels c < ct of (pscale p - t)^2;
+// dp = d loss / d p (zero in channels ct..cp-1).
+void launch_mse(const uint16_t* p, int cp, const float* t, int ct, long long nvox, float pscale, float scale,
+                uint16_t* dp, float* loss, hipStream_t s);
+// One Adam step (step counts from 1); w16 (nullable) receives bfloat16 copies.
+void launch_adam(float* w, const float* gr, float* m, float* v, uint16_t* w16, long long n, float lr, float b1,
+                 float b2, float eps, int step, hipStream_t s);
+void launch_to_bf16(const float* in, long long n, uint16_t* out, hipStream_t s);
+// (n, 8) bfloat16: (mov - ms[0]) / ms[1], (ref - ms[0]) / ms[1], zeros.
+void launch_net_input(const float* mov, const float* ref, long long n, const float* ms, uint16_t* out,
+                      hipStream_t s);
+// field (n, 3) float32 = scale * p[:, 0:3] of p (n, cp) bfloat16.
+void launch_net_output(const uint16_t* p, int cp, long long n, float scale, float* field, hipStream_t s);
+
 // ---------------------------------------------------------------- context
 
 struct Context {
@@ -330,6 +370,7 @@ struct Context {
     DevBuf msel, mlab, mcnt, mtmp;       // mask statistics, component labels / areas, byte mask
     DevBuf hsmall, hbig;                 // harness: statistics / Gram scratch, sampled frame matrix
     DevBuf splat;                        // splats: shift scratch and the weighted sums
+    DevBuf netw;                         // learned initialisation: flipped weights, input statistics
     std::vector<hipEvent_t> events;
     hipStream_t side = nullptr;          // pyramid builds, ahead of the iterations
     std::vector<hipEvent_t> lvl_events;  // [L]: level L's images are ready; [layer+1]: fork

@@ -121,6 +121,8 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
             raise ValueError(f"{name} must be float32 with the volume's shape")
     _check_vol(motion_init, "motion_init", 4, 3)
     nz, ny, nx = mov.shape
+    if motion_init is not None and (motion_init.shape[0] != nz or motion_init.dtype != torch.float32):
+        raise ValueError("motion_init must be float32 (Z, Y', X', 3) with the volume's Z")
     dev = mov.device if device is None else device
     if motion_out is None:
         motion_out = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device=dev)
@@ -136,6 +138,9 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
     prm.precision = PRECISIONS[precision]
     prm.reuse_template = 1 if reuse_template else 0
+    if motion_init is not None and tuple(motion_init.shape[1:3]) != (ny, nx):
+        # a warm start of another x-y size (full-resolution units)
+        prm.init_ny, prm.init_nx = int(motion_init.shape[1]), int(motion_init.shape[2])
     prm.timing = 1 if timing else 0
     keep = []
     if schedule is not None:

@@ -1,0 +1,205 @@
+"""Training of the learned initialisation on generated frame pairs.
+
+A pair is a 64 x 512 x 1024 (z, y, x) uint16 frame and its template: a
+template of 100k Gaussian blobs over a background of 100 with Poisson noise,
+and a moving frame pull-warped from it by a 6-mode sinusoidal field (max
+|d| = 6 voxels) with fresh Poisson noise.  The network reads the pyramid
+level 3 images of both (64 x 64 x 128) and learns the registration field at
+that level, in full-resolution voxel units:
+
+* ``target="lk"``: the converged field of the plain pyramid LK on the pair
+  (layer 4, r = 8, 15 iterations), resampled to level 3 -- the method's own
+  result, which the learned start has to reproduce with fewer iterations;
+* ``target="truth"``: the generating registration field m = d(p + m).
+
+Run ``python -m wholistic_registration_amd.train_init --out weights.safetensors``
+on the GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import device as D
+from . import learned as Lr
+from . import synth
+
+FRAME = (64, 512, 1024)
+LK_PARAMS = dict(layer=4, iter=15, r=8, zRatio=1.0, movRange=5.0, tol=1e-3)
+RAW_PENALTY = 0.08
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def make_template(seed, dev, zyx=FRAME, n_blobs=100_000):
+    """Noise-free template and its Poisson-noise uint16 frame."""
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    clean = synth.render_template_torch(zyx, synth.draw_blobs(rng, zyx, n_blobs), dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    tmpl = torch.poisson(clean, generator=g).round_().clamp_(0, 65535).to(torch.uint16).contiguous()
+    return clean, tmpl
+
+
+def make_moving(clean, seed, dev, n_modes=6, dmax=6.0):
+    """Moving frame (uint16) pull-warped by -d, and d."""
+    torch = _torch()
+    zyx = tuple(clean.shape)
+    d = synth.field_torch(zyx, synth.draw_modes(np.random.default_rng(seed), n_modes), dmax, dev)
+    warped = D.correct_motion(clean, (-d).contiguous())
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    mov = torch.poisson(warped.clamp_min(0), generator=g).round_().clamp_(0, 65535).to(torch.uint16).contiguous()
+    return mov, d
+
+
+def truth_field(d, n_iter=25):
+    """m with m(p) = d(p + m(p)) by fixed-point iteration (library warp)."""
+    torch = _torch()
+    comps = [d[..., c].contiguous() for c in range(3)]
+    m = d.clone()
+    for _ in range(n_iter):
+        m = torch.stack([D.correct_motion(comps[c], m) for c in range(3)], dim=-1).contiguous()
+    return m
+
+
+def to_level(field, level=Lr.LEVEL):
+    """A full-resolution (Z, Y, X, 3) field resampled (imresize, bicubic) to
+    level ``level``'s x-y size, values unchanged (float32)."""
+    torch = _torch()
+    nz, ny, nx, _ = field.shape
+    lx, ly = int(nx / 2 ** level), int(ny / 2 ** level)
+    comps = [D.imresize_xy(field[..., c].contiguous(), lx, ly).float() for c in range(3)]
+    return torch.stack(comps, dim=-1).contiguous()
+
+
+def penalty(tmpl):
+    return float(D.smooth_penalty_factor(tmpl.float().contiguous())) * RAW_PENALTY
+
+
+class PairSet:
+    """Network inputs and targets of generated pairs, resident on the device."""
+
+    def __init__(self, dev, templates, fields_per_template, seed0, target="lk", log=None):
+        torch = _torch()
+        self.x, self.t, self.meta = [], [], []
+        for ti in range(templates):
+            clean, tmpl = make_template(seed0 + 1000 * ti, dev)
+            ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
+            sp = penalty(tmpl)
+            for fi in range(fields_per_template):
+                seed = seed0 + 1000 * ti + 1 + fi
+                mov, d = make_moving(clean, seed, dev)
+                if target == "lk":
+                    res = D.get_motion(mov, tmpl, smoothPenalty=sp, **LK_PARAMS)
+                    full = res.motion
+                elif target == "truth":
+                    full = truth_field(d)
+                else:
+                    raise ValueError("target must be 'lk' or 'truth'")
+                mov_l = D.pyramid_level(mov, Lr.LEVEL)
+                x = Lr.net_input(mov_l, ref_l)
+                self.x.append(x)
+                self.t.append(to_level(full).unsqueeze(0))
+                self.meta.append(seed)
+                del mov, d, full
+            del clean, tmpl
+            torch.cuda.empty_cache()
+            if log:
+                log(f"[pairs] template {ti + 1}/{templates}: {len(self.x)} pairs")
+
+    def batch(self, idx):
+        torch = _torch()
+        return torch.cat([self.x[i] for i in idx]), torch.cat([self.t[i] for i in idx])
+
+
+def train(net, pairs, steps, batch, lr, log=None, log_every=50):
+    """Adam with cosine learning-rate decay over ``steps`` steps of ``batch``
+    random pairs; returns the loss curve (mean per logging window)."""
+    rng = np.random.default_rng(0)
+    curve, window = [], []
+    n = len(pairs.x)
+    for k in range(steps):
+        idx = rng.choice(n, size=min(batch, n), replace=False)
+        x, t = pairs.batch(idx)
+        net.zero_grad()
+        out = net.forward(x)
+        loss, dout = net.loss(out, t)
+        net.backward(dout)
+        net.step(lr=lr * 0.5 * (1.0 + math.cos(math.pi * k / steps)))
+        window.append(loss)
+        if (k + 1) % log_every == 0 or k + 1 == steps:
+            v = float(sum(float(w.item()) for w in window) / len(window))
+            curve.append((k + 1, v))
+            window = []
+            if log:
+                log(f"[train] step {k + 1}/{steps} loss {v:.5f}")
+    return curve
+
+
+def evaluate(net, pairs):
+    """Mean squared error of the network on ``pairs`` (and of the zero field)."""
+    torch = _torch()
+    tot, zero = 0.0, 0.0
+    for i in range(len(pairs.x)):
+        out = net.forward(pairs.x[i])
+        loss, _ = net.loss(out, pairs.t[i])
+        tot += float(loss.item())
+        zero += float((pairs.t[i] ** 2).mean().item())
+    torch.cuda.synchronize()
+    return tot / len(pairs.x), zero / len(pairs.x)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", required=True, help="safetensors file for the trained weights")
+    ap.add_argument("--templates", type=int, default=8)
+    ap.add_argument("--fields", type=int, default=16, help="moving frames per template")
+    ap.add_argument("--val-templates", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--target", default="lk", choices=("lk", "truth"))
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--report", default=None, help="JSON file for the loss curve and validation")
+    args = ap.parse_args(argv)
+    torch = _torch()
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda", 0)
+    t0 = time.time()
+    pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log)
+    val = PairSet(dev, args.val_templates, 4, 20000, args.target, log)
+    t1 = time.time()
+    net = Lr.InitNet(seed=args.seed, lr=args.lr)
+    torch.cuda.synchronize()
+    t2 = time.time()
+    curve = train(net, pairs, args.steps, args.batch, args.lr, log)
+    torch.cuda.synchronize()
+    t3 = time.time()
+    v_loss, v_zero = evaluate(net, val)
+    net.save(args.out)
+    rep = {"pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target, "steps": args.steps,
+           "batch": args.batch, "lr": args.lr, "data_s": t1 - t0, "train_s": t3 - t2,
+           "ms_per_step": 1e3 * (t3 - t2) / args.steps, "curve": curve,
+           "val_mse": v_loss, "val_mse_zero_field": v_zero}
+    log(json.dumps({k: v for k, v in rep.items() if k != "curve"}))
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(rep, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
