@@ -1,0 +1,90 @@
+"""What the trained network gains over the method without it, on held-out
+64 x 512 x 1024 uint16 frame pairs (template seed 90000 and moving frames
+90001..; training used template seeds 10000 + 1000 i, i < 32, with frames
++1..+16, and validation 20000, 21000).
+
+The plain pyramid LK (layer 4, r 8, 15 iterations, float32) is the judge:
+endpoint errors are measured against its field.  The network's field starts
+getMotion at full resolution (layer 0) with 5 iterations and ends within the
+0.1-voxel gate; the same schedule started from zero (the method without the
+network, at the same cost) ends outside it with about twice the error.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = (90001, 90002)
+
+
+@pytest.fixture(scope="module")
+def setup(cuda):
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    if not os.path.exists(Lr.WEIGHTS):
+        pytest.fail(f"trained weights missing: {Lr.WEIGHTS}")
+    net = Lr.InitNet.load()
+    clean, tmpl = T.make_template(90000, cuda)
+    params = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=T.penalty(tmpl), tol=1e-3)
+    ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
+    yield dict(net=net, clean=clean, tmpl=tmpl, params=params, ref_l=ref_l)
+
+
+def _epe(a, b):
+    import torch
+    return float(torch.sqrt(((a - b) ** 2).sum(-1)).mean())
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_learned_start_beats_method_without_it(setup, seed):
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    s = setup
+    mov, _ = T.make_moving(s["clean"], seed, s["tmpl"].device)
+    p = s["params"]
+    judge = D.get_motion(mov, s["tmpl"], layer=4, iter=15, **p).motion.clone()
+    learned, info = Lr.register_learned(mov, s["tmpl"], s["net"], ref_level=s["ref_l"], **Lr.SCHEDULE, **p)
+    assert info["used"] == "learned"
+    e_learned = _epe(learned.motion, judge)
+    zero = D.get_motion(mov, s["tmpl"], **Lr.SCHEDULE, **p)
+    e_zero = _epe(zero.motion, judge)
+    assert e_learned <= 0.1, e_learned               # the gate
+    assert e_zero > 0.1 and e_zero > 1.5 * e_learned, (e_zero, e_learned)
+    assert learned.final_error < zero.final_error
+
+
+def test_fallback_keeps_the_lower_residual(setup):
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    s = setup
+    mov, _ = T.make_moving(s["clean"], SEEDS[0], s["tmpl"].device)
+    p = s["params"]
+    res, info = Lr.register_learned(mov, s["tmpl"], s["net"], ref_level=s["ref_l"], fallback="plain", **Lr.SCHEDULE,
+                                    **p)
+    assert set(info) >= {"used", "learned_error", "plain_error"}
+    assert res.final_error == min(info["learned_error"], info["plain_error"])
+    assert info["used"] == ("plain" if info["plain_error"] < info["learned_error"] else "learned")
+
+
+def test_prediction_is_deterministic_and_finite(setup):
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    s = setup
+    mov, _ = T.make_moving(s["clean"], SEEDS[1], s["tmpl"].device)
+    m_l = D.pyramid_level(mov, Lr.LEVEL)
+    a = s["net"].predict(m_l, s["ref_l"]).clone()
+    b = s["net"].predict(m_l, s["ref_l"])
+    assert torch.equal(a, b)
+    assert torch.isfinite(a).all()
+    assert tuple(a.shape) == (64, 64, 128, 3)
+    assert float(a.abs().max()) < 20.0

@@ -875,6 +875,8 @@ int wr_net_conv3d(wr_ctx* ctx, const void* x, const void* w, const float* bias, 
     WR_ARG(cin > 0 && cin % 8 == 0 && cout > 0 && cout % 16 == 0 && cout / 16 <= 65535,
            "wr_net_conv3d: cin must be a multiple of 8 and cout of 16");
     WR_ARG(al16(x) && al16(w) && al16(y), "wr_net_conv3d: x, w and y must be 16-byte aligned");
+    // 32-bit element offsets within one batch item
+    WR_ARG(static_cast<double>(d) * h * wd * std::max(cin, cout) < 2147483647.0, "wr_net_conv3d: volume too large");
     NetGeom g;
     WR_TRY(net_geom(batch, d, h, wd, &g));
     WR_HIP_CHECK(hipSetDevice(ctx->c.device));
@@ -890,6 +892,8 @@ int wr_net_conv3d_bwd_data(wr_ctx* ctx, const void* dy, const void* w, void* dx,
     WR_ARG(cin > 0 && cin % 16 == 0 && cout > 0 && cout % 16 == 0 && cin / 16 <= 65535,
            "wr_net_conv3d_bwd_data: cin and cout must be multiples of 16");
     WR_ARG(al16(dy) && al16(dx), "wr_net_conv3d_bwd_data: dy and dx must be 16-byte aligned");
+    WR_ARG(static_cast<double>(d) * h * wd * std::max(cin, cout) < 2147483647.0,
+           "wr_net_conv3d_bwd_data: volume too large");
     NetGeom g;
     WR_TRY(net_geom(batch, d, h, wd, &g));
     Context& c = ctx->c;

@@ -62,45 +62,64 @@ __device__ __forceinline__ bf16x8 zero8() {
     return r.v;
 }
 
+// Bit t of the result is set when neighbour tap t of voxel (d, h, w) lies
+// inside the volume.
+__device__ __forceinline__ unsigned tap_mask(int d, int h, int w, const NetGeom& g) {
+    const unsigned mw = (w > 0 ? 1u : 0u) | 2u | (w + 1 < g.w ? 4u : 0u);
+    const unsigned mhw = (h > 0 ? mw : 0u) | (mw << 3) | (h + 1 < g.h ? mw << 6 : 0u);
+    return (d > 0 ? mhw : 0u) | (mhw << 9) | (d + 1 < g.d ? mhw << 18 : 0u);
+}
+
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
                                                   const float* __restrict__ bias, uint16_t* __restrict__ Y,
                                                   NetGeom g, int cin, int cout, int act, float slope) {
+    __shared__ int toff[27];          // element offset of tap t: voxel offset * cin
+    if (threadIdx.x < 27) {
+        const int t = threadIdx.x;
+        toff[t] = (((t / 9 - 1) * g.h + (t / 3) % 3 - 1) * g.w + t % 3 - 1) * cin;
+    }
+    __syncthreads();
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long nv = static_cast<long long>(g.d) * g.h * g.w;
     const int co0 = blockIdx.y * 16;
     const long long vb = static_cast<long long>(blockIdx.x) * 256 + wave * 64;
-    const size_t xb = static_cast<size_t>(blockIdx.z) * nv * cin;
+    const uint16_t* xb = X + static_cast<size_t>(blockIdx.z) * nv * cin;
     const size_t yb = static_cast<size_t>(blockIdx.z) * nv * cout;
     const int ksz = 27 * cin;
-    int vd[4], vh[4], vw[4];
-    bool vok[4];
+    int vo[4];            // element offset of the voxel's channel 0
+    unsigned vm[4];       // its in-volume taps (0: voxel past the end)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long v = vb + 16 * j + (l & 15);
-        vok[j] = v < nv;
-        const long long vv = vok[j] ? v : 0;
-        vw[j] = static_cast<int>(vv % g.w);
-        vh[j] = static_cast<int>((vv / g.w) % g.h);
-        vd[j] = static_cast<int>(vv / (static_cast<long long>(g.w) * g.h));
+        const bool ok = v < nv;
+        const long long vv = ok ? v : 0;
+        const int w = static_cast<int>(vv % g.w);
+        const int h = static_cast<int>((vv / g.w) % g.h);
+        const int d = static_cast<int>(vv / (static_cast<long long>(g.w) * g.h));
+        vo[j] = static_cast<int>(vv) * cin;
+        vm[j] = ok ? tap_mask(d, h, w, g) : 0u;
     }
     f4 acc[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
-    const uint16_t* wrow = Wt + static_cast<size_t>(co0 + (l & 15)) * ksz;
+    const uint16_t* wrow = Wt + static_cast<size_t>(co0 + (l & 15)) * ksz + 8 * (l >> 4);
+    // this lane's reduction index k0 + 8 (l / 16) = tap * cin + ci, advanced by 32 per step
+    int tap = (8 * (l >> 4)) / cin;
+    int ci = 8 * (l >> 4) - tap * cin;
     for (int k0 = 0; k0 < ksz; k0 += 32) {
-        const int kk = k0 + 8 * (l >> 4);
-        const bool kin = kk < ksz;
-        const bf16x8 a = kin ? load8(wrow + kk) : zero8();
-        const int tap = kin ? kk / cin : 0;
-        const int ci = kk - tap * cin;
-        const int dd = tap / 9 - 1, dh = (tap / 3) % 3 - 1, dw = tap % 3 - 1;
+        const bool kin = tap < 27;
+        const bf16x8 a = kin ? load8(wrow + k0) : zero8();
+        const int off = (kin ? toff[tap] : 0) + ci;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int zd = vd[j] + dd, zh = vh[j] + dh, zw = vw[j] + dw;
-            const bool in = kin && vok[j] && zd >= 0 && zd < g.d && zh >= 0 && zh < g.h && zw >= 0 && zw < g.w;
-            const bf16x8 b =
-                in ? load8(X + xb + ((static_cast<size_t>(zd) * g.h + zh) * g.w + zw) * cin + ci) : zero8();
+            const bool in = kin && ((vm[j] >> (tap & 31)) & 1u);
+            const bf16x8 b = in ? load8(xb + vo[j] + off) : zero8();
             acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        }
+        ci += 32;
+        while (ci >= cin) {
+            ci -= cin;
+            ++tap;
         }
     }
     const int m0 = co0 + 4 * (l >> 4);

@@ -44,6 +44,10 @@ LAYERS = (
     ("out", 16, 16, False),
 )
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "init_net.safetensors")
+# getMotion schedule after the network's start: the full-resolution level
+# only, 5 iterations (a start at the coarsest level is pulled back to that
+# level's own optimum within its first iterations and gains nothing)
+SCHEDULE = dict(layer=0, iter=5)
 
 
 def _torch():

@@ -123,9 +123,10 @@ class PairSet:
         return torch.cat([self.x[i] for i in idx]), torch.cat([self.t[i] for i in idx])
 
 
-def train(net, pairs, steps, batch, lr, log=None, log_every=50):
-    """Adam with cosine learning-rate decay over ``steps`` steps of ``batch``
-    random pairs; returns the loss curve (mean per logging window)."""
+def train(net, pairs, steps, batch, lr, log=None, log_every=50, warmup=200):
+    """Adam over ``steps`` steps of ``batch`` random pairs, the learning rate
+    rising linearly over ``warmup`` steps and then decaying on a cosine;
+    returns the loss curve (mean per logging window)."""
     rng = np.random.default_rng(0)
     curve, window = [], []
     n = len(pairs.x)
@@ -136,7 +137,8 @@ def train(net, pairs, steps, batch, lr, log=None, log_every=50):
         out = net.forward(x)
         loss, dout = net.loss(out, t)
         net.backward(dout)
-        net.step(lr=lr * 0.5 * (1.0 + math.cos(math.pi * k / steps)))
+        scale = (k + 1) / warmup if k < warmup else 0.5 * (1.0 + math.cos(math.pi * (k - warmup) / (steps - warmup)))
+        net.step(lr=lr * scale)
         window.append(loss)
         if (k + 1) % log_every == 0 or k + 1 == steps:
             v = float(sum(float(w.item()) for w in window) / len(window))
@@ -168,7 +170,8 @@ def main(argv=None):
     ap.add_argument("--val-templates", type=int, default=2)
     ap.add_argument("--steps", type=int, default=1500)
     ap.add_argument("--batch", type=int, default=4)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--target", default="lk", choices=("lk", "truth"))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--report", default=None, help="JSON file for the loss curve and validation")
@@ -186,7 +189,7 @@ def main(argv=None):
     net = Lr.InitNet(seed=args.seed, lr=args.lr)
     torch.cuda.synchronize()
     t2 = time.time()
-    curve = train(net, pairs, args.steps, args.batch, args.lr, log)
+    curve = train(net, pairs, args.steps, args.batch, args.lr, log, warmup=args.warmup)
     torch.cuda.synchronize()
     t3 = time.time()
     v_loss, v_zero = evaluate(net, val)
