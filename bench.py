@@ -331,6 +331,108 @@ def mask_measure(vol, crop, reps=3):
     }
 
 
+def learned_measure(n_frames=6, reps=5):
+    """Plain FP32 pyramid LK against the learned start (+ FP32 and + BF16
+    level images) on held-out frame pairs of the 64 x 512 x 1024 uint16
+    frame geometry (template seed 90000, frames 90001..; the network never
+    saw them).  Per frame: median wall time of getMotion + correctMotion over
+    ``reps`` runs (the learned path also computes the moving frame's level-3
+    image and runs the network; the template's level-3 image is computed once
+    per template), endpoint error of every field against the plain FP32 field
+    (layer 4, r 8, 15 iterations: the gate's judge, equal to the float64
+    oracle within 1e-3 voxel at oracle sizes) and against the generating
+    field, and whether the fallback rule (keep the plain field when its final
+    residual is lower) would fire."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    dev = torch.device("cuda", 0)
+    net = Lr.InitNet.load()
+    clean, tmpl = T.make_template(90000, dev)
+    sp = T.penalty(tmpl)
+    base = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=sp, tol=1e-3)
+    ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
+    nz, ny, nx = tmpl.shape
+    n0 = nz * ny * nx
+    warped = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    motion = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device=dev)
+
+    def plain(mov, layer, it, precision="fp32"):
+        r = D.get_motion(mov, tmpl, layer=layer, iter=it, precision=precision, motion_out=motion, **base)
+        D.correct_motion(mov, motion, out=warped)
+        return r
+
+    def learned(mov, precision):
+        r, _ = Lr.register_learned(mov, tmpl, net, ref_level=ref_l, precision=precision, motion_out=motion,
+                                   **Lr.SCHEDULE, **base)
+        D.correct_motion(mov, motion, out=warped)
+        return r
+
+    variants = {
+        "plain_fp32": lambda mov: plain(mov, 4, 15),
+        "learned_fp32": lambda mov: learned(mov, "fp32"),
+        "learned_bf16": lambda mov: learned(mov, "bf16"),
+        "plain_fp32_same_schedule": lambda mov: plain(mov, 0, Lr.SCHEDULE["iter"]),
+    }
+    rows = {k: {"ms": [], "epe": [], "epe_truth": [], "final_error": []} for k in variants}
+    fallback = 0
+    for i in range(n_frames):
+        mov, d = T.make_moving(clean, 90001 + i, dev)
+        truth = T.truth_field(d)
+        judge = None
+        for name, fn in variants.items():
+            fn(mov)                                           # warm (workspace sizes)
+            ts = []
+            for _ in range(reps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                res = fn(mov)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            if judge is None:
+                judge = motion.clone()
+            row = rows[name]
+            row["ms"].append(1e3 * float(np.median(ts)))
+            row["epe"].append(float(torch.sqrt(((motion - judge) ** 2).sum(-1)).mean()))
+            row["epe_truth"].append(float(torch.sqrt(((motion - truth) ** 2).sum(-1)).mean()))
+            row["final_error"].append(res.final_error)
+        if rows["learned_fp32"]["final_error"][-1] > rows["plain_fp32"]["final_error"][-1]:
+            fallback += 1
+        del mov, d, truth, judge
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m_l = D.pyramid_level(T.make_moving(clean, 90001, dev)[0], Lr.LEVEL)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        net.predict(m_l, ref_l)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        net.predict(m_l, ref_l)
+    torch.cuda.synchronize()
+    net_ms = 1e3 * (time.perf_counter() - t0) / 10
+    flops = Lr.forward_flops(*m_l.shape)
+    out = {"workload": f"{n_frames} held-out pairs of {nz}x{ny}x{nx} (z,y,x) uint16 frames (6-mode fields, "
+                       f"max |d| 6 voxels), getMotion + correctMotion per frame",
+           "judge": "plain FP32 field, layer 4, r 8, 15 iterations",
+           "learned_schedule": dict(Lr.SCHEDULE), "fallback_would_fire": fallback}
+    for name, row in rows.items():
+        ms = float(np.mean(row["ms"]))
+        out[name] = {"value": n0 / (ms * 1e-3), "unit": "registered voxels/s", "ms_per_frame": ms,
+                     "epe_vs_judge_mean": float(np.mean(row["epe"])), "epe_vs_judge_max": float(np.max(row["epe"])),
+                     "frames_within_gate": int(sum(e <= 0.1 for e in row["epe"])),
+                     "epe_vs_truth_mean": float(np.mean(row["epe_truth"])),
+                     "per_frame": {"ms": [round(v, 3) for v in row["ms"]], "epe": [round(v, 4) for v in row["epe"]]}}
+    out["speedup_learned_fp32"] = out["plain_fp32"]["ms_per_frame"] / out["learned_fp32"]["ms_per_frame"]
+    out["speedup_learned_bf16"] = out["plain_fp32"]["ms_per_frame"] / out["learned_bf16"]["ms_per_frame"]
+    out["network"] = {"forward_ms": net_ms, "flops": flops, "achieved_tflops": flops / (net_ms * 1e-3) / 1e12,
+                      "peak_tflops": 2500.0, "frac": flops / (net_ms * 1e-3) / 2.5e15,
+                      "input": list(m_l.shape) + [8], "note": "bf16 MFMA, dense peak; wall time of predict()"}
+    return out
+
+
 def truth_epe(motion, d, mask):
     """Endpoint error of the GPU field against the synthetic registration
     field m = d(p + m) (fixed point, evaluated with the library's warp)."""
@@ -368,6 +470,10 @@ def main():
                     help="frames of the time-series measurement (config 2 frame size; 0 to skip)")
     ap.add_argument("--no-mask-measure", action="store_true",
                     help="skip timing the harness's automatic mask kernels")
+    ap.add_argument("--learned-frames", type=int, default=6,
+                    help="held-out frame pairs of the plain vs learned-start comparison (0 to skip)")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only run the plain vs learned-start comparison (prints its JSON)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -391,6 +497,9 @@ def main():
     from wholistic_registration_amd import _lib
     from wholistic_registration_amd import device as D
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.learned_only:
+        print(json.dumps(learned_measure(max(args.learned_frames, 1))), flush=True)
+        return
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     w = WORKLOADS[args.workload]
@@ -476,6 +585,9 @@ def main():
     if args.batch_frames > 0:
         log(f"[bench] {args.batch_frames} frames against one template")
         out["batched_frames"] = batch_measure(args.batch_frames, (1, 2, 4, 8))
+    if args.learned_frames > 0:
+        log(f"[bench] plain vs learned start on {args.learned_frames} held-out frames")
+        out["learned_init"] = learned_measure(args.learned_frames)
     if not args.no_mask_measure:
         log("[bench] fixed-plane projections")
         out["plane_projections"] = splat_measure()

@@ -103,20 +103,35 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
     const uint16_t* wrow = Wt + static_cast<size_t>(co0 + (l & 15)) * ksz + 8 * (l >> 4);
-    // this lane's reduction index k0 + 8 (l / 16) = tap * cin + ci, advanced by 32 per step
+    // this lane's reduction index k0 + 8 (l / 16) = tap * cin + ci, advanced by 32 per step;
+    // two steps per trip so that the loads of both are in flight together
     int tap = (8 * (l >> 4)) / cin;
     int ci = 8 * (l >> 4) - tap * cin;
-    for (int k0 = 0; k0 < ksz; k0 += 32) {
-        const bool kin = tap < 27;
-        const bf16x8 a = kin ? load8(wrow + k0) : zero8();
-        const int off = (kin ? toff[tap] : 0) + ci;
+    for (int k0 = 0; k0 < ksz; k0 += 64) {
+        int tap1 = tap, ci1 = ci + 32;
+        while (ci1 >= cin) {
+            ci1 -= cin;
+            ++tap1;
+        }
+        const bool kin0 = tap < 27, kin1 = tap1 < 27;
+        const bf16x8 a0 = kin0 ? load8(wrow + k0) : zero8();
+        const bf16x8 a1 = kin1 ? load8(wrow + k0 + 32) : zero8();
+        const int off0 = (kin0 ? toff[tap] : 0) + ci;
+        const int off1 = (kin1 ? toff[tap1] : 0) + ci1;
+        bf16x8 b0[4], b1[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const bool in = kin && ((vm[j] >> (tap & 31)) & 1u);
-            const bf16x8 b = in ? load8(xb + vo[j] + off) : zero8();
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+            const bool in0 = kin0 && ((vm[j] >> (tap & 31)) & 1u);
+            const bool in1 = kin1 && ((vm[j] >> (tap1 & 31)) & 1u);
+            b0[j] = in0 ? load8(xb + vo[j] + off0) : zero8();
+            b1[j] = in1 ? load8(xb + vo[j] + off1) : zero8();
         }
-        ci += 32;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0[j], acc[j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[j], acc[j], 0, 0, 0);
+        tap = tap1;
+        ci = ci1 + 32;
         while (ci >= cin) {
             ci -= cin;
             ++tap;

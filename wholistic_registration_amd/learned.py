@@ -293,6 +293,18 @@ class InitNet:
         return cls(params, device=device)
 
 
+def forward_flops(d, h, w, batch=1):
+    """Multiply-add work of one forward pass (2 flops per MAC, the 8 padded
+    input channels included): what the matrix cores execute."""
+    dims = {"enc1a": 0, "enc1b": 0, "enc2a": 1, "enc2b": 1, "bota": 2, "botb": 2, "dec2a": 1, "dec2b": 1,
+            "dec1a": 0, "dec1b": 0, "out": 0}
+    total = 0
+    for name, cin, cout, _ in LAYERS:
+        k = 2 ** dims[name]
+        total += 2 * 27 * cin * cout * (d // k) * (h // k) * (w // k)
+    return batch * total
+
+
 def net_input(mov_l, ref_l, out=None):
     """(Z, Y, X) float32 level images -> (1, Z, Y, X, 8) bfloat16 network
     input: both normalised by the template's mean and standard deviation."""
