@@ -433,6 +433,80 @@ def learned_measure(n_frames=6, reps=5):
     return out
 
 
+def crops_measure(n_crops=16, seed=3, reps=3):
+    """Crops of uneven sizes (x in 128..1024, y in 128..768, z in 32..96,
+    multiples of 8, random positions; seed 3) of one 96 x 768 x 1024 uint16
+    frame pair (225k blobs, 6-mode field, max |d| = 6), r 8, 15 iterations,
+    up to 4 levels (fewer where a crop's coarsest level would hold no patch),
+    getMotion + correctMotion per crop.  Total time of all crops: on one
+    context with the built-in tile rule, with autotuned per-level tiles, and
+    with autotuned tiles on 4 concurrent contexts (FrameBatch); the tuning
+    itself (once per geometry) is timed separately."""
+    import torch
+
+    from wholistic_registration_amd import autotune
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import synth
+    from wholistic_registration_amd.batch import FrameBatch, max_layer
+    dev = torch.device("cuda", 0)
+    zyx = (96, 768, 1024)
+    rng = np.random.default_rng(seed)
+    clean = synth.render_template_torch(zyx, synth.draw_blobs(rng, zyx, 225_000), dev)
+    d = synth.field_torch(zyx, synth.draw_modes(rng, 6), 6.0, dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    tmpl = torch.poisson(clean, generator=g).round_().clamp_(0, 65535).to(torch.uint16)
+    mov = torch.poisson(D.correct_motion(clean, (-d).contiguous()).clamp_min(0), generator=g)
+    mov = mov.round_().clamp_(0, 65535).to(torch.uint16)
+    del clean, d
+    sp = float(D.smooth_penalty_factor(tmpl.float().contiguous())) * RAW_PENALTY
+    params = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=sp, tol=1e-3)
+    pairs, shapes = [], []
+    for _ in range(n_crops):
+        cz, cy, cx = (int(rng.integers(lo // 8, hi // 8 + 1)) * 8 for lo, hi in ((32, 96), (128, 768), (128, 1024)))
+        z0, y0, x0 = (int(rng.integers(0, n - c + 1)) for n, c in zip(zyx, (cz, cy, cx)))
+        sl = (slice(z0, z0 + cz), slice(y0, y0 + cy), slice(x0, x0 + cx))
+        pairs.append((mov[sl].contiguous(), tmpl[sl].contiguous()))
+        shapes.append((cz, cy, cx))
+    del mov, tmpl
+    n_vox = sum(int(np.prod(s)) for s in shapes)
+    cache = autotune.TileCache()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tiles = [cache.tiles(m, t, layer=max_layer(m.shape, 8), **params) for m, t in pairs]
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t0
+
+    def sequential(tuned):
+        for (m, t), tl in zip(pairs, tiles):
+            res = D.get_motion(m, t, layer=max_layer(m.shape, 8), iter=15, tile_patches_x=tl if tuned else 0, **params)
+            D.correct_motion(m, res.motion)
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    out = {"workload": f"{n_crops} crops of a 96x768x1024 uint16 pair, sizes (z,y,x) {shapes}, r 8, "
+                       f"15 iterations, up to 4 levels, getMotion + correctMotion each",
+           "voxels": n_vox, "tiles_per_level": [list(t) for t in tiles], "tuning_s": tune_s}
+    for name, fn in (("rule_1ctx", lambda: sequential(False)), ("tuned_1ctx", lambda: sequential(True))):
+        dt = timed(fn)
+        out[name] = {"value": n_vox / dt, "ms": 1e3 * dt}
+    with FrameBatch(workers=4, autotune=True) as fb:
+        fb.tiles = cache
+        dt = timed(lambda: fb.register_crops(pairs, iter=15, warp=True, **params))
+    out["tuned_4ctx"] = {"value": n_vox / dt, "ms": 1e3 * dt}
+    out["unit"] = "registered voxels/s"
+    return out
+
+
 def truth_epe(motion, d, mask):
     """Endpoint error of the GPU field against the synthetic registration
     field m = d(p + m) (fixed point, evaluated with the library's warp)."""
@@ -472,6 +546,10 @@ def main():
                     help="skip timing the harness's automatic mask kernels")
     ap.add_argument("--learned-frames", type=int, default=6,
                     help="held-out frame pairs of the plain vs learned-start comparison (0 to skip)")
+    ap.add_argument("--crops", type=int, default=16,
+                    help="crops of uneven sizes for the tile-autotuner measurement (0 to skip)")
+    ap.add_argument("--crops-only", action="store_true",
+                    help="only run the crops measurement (prints its JSON)")
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain vs learned-start comparison (prints its JSON)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -499,6 +577,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.learned_only:
         print(json.dumps(learned_measure(max(args.learned_frames, 1))), flush=True)
+        return
+    if args.crops_only:
+        print(json.dumps(crops_measure(max(args.crops, 1))), flush=True)
         return
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -585,6 +666,9 @@ def main():
     if args.batch_frames > 0:
         log(f"[bench] {args.batch_frames} frames against one template")
         out["batched_frames"] = batch_measure(args.batch_frames, (1, 2, 4, 8))
+    if args.crops > 0:
+        log(f"[bench] {args.crops} crops of uneven sizes")
+        out["crops"] = crops_measure(args.crops)
     if args.learned_frames > 0:
         log(f"[bench] plain vs learned start on {args.learned_frames} held-out frames")
         out["learned_init"] = learned_measure(args.learned_frames)

@@ -87,6 +87,11 @@ typedef struct wr_params {
                                the full-to-level size ratio
                                (calFlow3d_Wei_v1.py:362-377) */
     int init_ny;
+    const int* tile_patches_level;  /* NULL = tile_patches_x on every level; else
+                               [layer + 1] entries, the patches per
+                               workgroup along x of level L (0 = the
+                               built-in rule): what the tile autotuner
+                               picks per level */
 } wr_params;
 
 /* Per-level iteration log, filled on the host when wr_get_motion returns

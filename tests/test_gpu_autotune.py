@@ -1,0 +1,80 @@
+"""The LK tile autotuner and crops of uneven sizes.
+
+The tile shape changes only the float32 summation order of the per-patch
+sums, so a run with tuned (or any) per-level tile widths stays within the
+1e-3-voxel field tolerance of the built-in rule's run under the same forced
+schedule; crops registered together on several contexts equal their single
+registrations with the same tiles bit for bit (the kernels are
+deterministic and the crops independent).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(shape, seed, cuda):
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import synth
+    rng = np.random.default_rng(seed)
+    n = max(200, int(np.prod(shape) // 300))
+    clean = synth.render_template_torch(shape, synth.draw_blobs(rng, shape, n), cuda)
+    d = synth.field_torch(shape, synth.draw_modes(rng, 4), 3.0, cuda)
+    mov = D.correct_motion(clean, (-d).contiguous())
+    g = torch.Generator(device=cuda)
+    g.manual_seed(seed)
+    return torch.poisson(mov.clamp_min(0), generator=g).contiguous(), clean.contiguous()
+
+
+PARAMS = dict(r=4, zRatio=1.0, movRange=5.0, smoothPenalty=0.5, tol=1e-3)
+
+
+def test_per_level_tiles_match_the_rule(cuda):
+    from wholistic_registration_amd import autotune
+    from wholistic_registration_amd import device as D
+    mov, ref = _pair((12, 100, 150), 1, cuda)
+    layer = 2
+    base = D.get_motion(mov, ref, layer=layer, iter=6, **PARAMS)
+    sched = {L: (base.records[L], base.updates[L]) for L in range(layer + 1)}
+    for tiles in ([1, 1, 1], [3, 2, 1], [autotune.candidates(4)[-1]] * 3):
+        res = D.get_motion(mov, ref, layer=layer, iter=6, tile_patches_x=tiles, schedule=sched, **PARAMS)
+        assert float((res.motion - base.motion).abs().max()) <= 1e-3, tiles
+    with pytest.raises(ValueError):
+        D.get_motion(mov, ref, layer=layer, iter=6, tile_patches_x=[1, 1], **PARAMS)
+    with pytest.raises(ValueError):
+        D.get_motion(mov, ref, layer=layer, iter=6, tile_patches_x=[1, -1, 1], **PARAMS)
+
+
+def test_tune_tiles_picks_admissible_widths(cuda):
+    from wholistic_registration_amd import autotune
+    mov, ref = _pair((8, 96, 200), 2, cuda)
+    choice, table = autotune.tune_tiles(mov, ref, layer=2, iters=2, reps=1, **PARAMS)
+    assert len(choice) == 3
+    assert set(table) == {0, *autotune.candidates(4)}
+    for L, c in enumerate(choice):
+        assert c == 0 or c in autotune.candidates(4)
+        if c:
+            assert table[c][L] < table[0][L]
+    assert all(np.isfinite(v).all() and (np.asarray(v) > 0).all() for v in table.values())
+
+
+def test_crops_of_uneven_sizes_equal_single_runs(cuda):
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd.batch import FrameBatch, max_layer
+    shapes = [(8, 64, 96), (12, 100, 150), (6, 40, 200), (10, 72, 72)]
+    pairs = [_pair(s, 10 + i, cuda) for i, s in enumerate(shapes)]
+    with FrameBatch(workers=3, autotune=True) as fb:
+        results, warped = fb.register_crops(pairs, iter=5, warp=True, **PARAMS)
+        for (mov, ref), res, wv in zip(pairs, results, warped):
+            L = max_layer(mov.shape, PARAMS["r"])
+            assert res.layer == L
+            key = fb.tiles.key(mov, L, PARAMS["r"])
+            single = D.get_motion(mov, ref, layer=L, iter=5, tile_patches_x=fb.tiles.get(key), **PARAMS)
+            assert single.records == res.records
+            assert bool((single.motion == res.motion).all())
+            assert bool((D.correct_motion(mov, single.motion) == wv).all())

@@ -38,6 +38,7 @@ class WrParams(ctypes.Structure):
         ("reuse_template", ctypes.c_int),
         ("init_nx", ctypes.c_int),
         ("init_ny", ctypes.c_int),
+        ("tile_patches_level", ctypes.POINTER(ctypes.c_int)),
     ]
 
 

@@ -18,6 +18,7 @@ from __future__ import annotations
 import threading
 
 from . import _lib
+from . import autotune as _tune
 from . import device as _dev
 
 
@@ -36,7 +37,7 @@ class FrameBatch:
     ``movRange``, ``smoothPenalty``, ``tol``, ``mask_ref``, ``precision``).
     """
 
-    def __init__(self, workers=4, device=0):
+    def __init__(self, workers=4, device=0, autotune=False):
         torch = _torch()
         if workers < 1:
             raise ValueError("workers must be at least 1")
@@ -53,6 +54,7 @@ class FrameBatch:
         with torch.cuda.device(device):
             self.streams = [torch.cuda.Stream() for _ in range(workers)]
         self._ref_seen = [None] * workers
+        self.tiles = _tune.TileCache() if autotune else None
 
     def invalidate(self):
         """Forget the template pyramids (call after modifying the template
@@ -121,3 +123,72 @@ class FrameBatch:
         if errors:
             raise errors[0]
         return motion_out, results
+
+    def register_crops(self, pairs, layer=None, warp=False, **params):
+        """Register crops of uneven sizes: ``pairs`` is a sequence of ``(mov,
+        ref)`` or ``(mov, ref, mask_ref)`` device volumes, each pair of one
+        shape.  The crops are spread over the worker contexts, largest first,
+        each to the worker with the fewest voxels so far.  ``layer=None``
+        uses up to 4 levels, fewer where a crop's coarsest level would be
+        smaller than one patch (``max_layer``).  With ``autotune=True`` each
+        geometry runs with the tile widths tuned on its first crop.  Returns
+        ``(results, warped)``: ``results[i]`` is crop i's
+        ``device.MotionResult`` and ``warped[i]`` its ``correctMotion`` output
+        (None unless ``warp``)."""
+        torch = _torch()
+        pairs = [tuple(p) for p in pairs]
+        n = len(pairs)
+        if n == 0:
+            raise ValueError("no crops")
+        for p in pairs:
+            if tuple(p[0].shape) != tuple(p[1].shape):
+                raise ValueError("each crop's mov and ref must share one shape")
+        r = params["r"]
+        layers = [max_layer(p[0].shape, r) if layer is None else int(layer) for p in pairs]
+        w = len(self.contexts)
+        load, owner = [0] * w, [[] for _ in range(w)]
+        for i in sorted(range(n), key=lambda i: -pairs[i][0].numel()):
+            k = min(range(w), key=lambda k: load[k])
+            owner[k].append(i)
+            load[k] += pairs[i][0].numel()
+        torch.cuda.current_stream(pairs[0][0].device).synchronize()
+        results, warped, errors = [None] * n, [None] * n, []
+
+        def work(k):
+            try:
+                ctx, strm = self.contexts[k], self.streams[k]
+                with torch.cuda.stream(strm):
+                    for i in owner[k]:
+                        mov, ref = pairs[i][0], pairs[i][1]
+                        extra = dict(params)
+                        if len(pairs[i]) > 2 and pairs[i][2] is not None:
+                            extra["mask_ref"] = pairs[i][2]
+                        if self.tiles is not None:
+                            extra["tile_patches_x"] = self.tiles.tiles(mov, ref, layer=layers[i], context=ctx,
+                                                                       stream=strm, **extra)
+                        results[i] = _dev.get_motion(mov, ref, layer=layers[i], context=ctx, stream=strm, **extra)
+                        if warp:
+                            warped[i] = _dev.correct_motion(mov, results[i].motion, context=ctx, stream=strm)
+                    strm.synchronize()
+            except BaseException as e:          # re-raised on the calling thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(k,)) for k in range(w) if owner[k]]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        return results, warped
+
+
+def max_layer(shape_zyx, r, cap=4):
+    """Deepest pyramid level (at most ``cap``) whose x and y extents
+    int(n / 2^L) still hold one LK patch centre (n_L >= r + 2,
+    calFlow3d_Wei_v1.py:434-436)."""
+    _, ny, nx = shape_zyx
+    L = 0
+    while L < cap and int(nx / 2 ** (L + 1)) >= r + 2 and int(ny / 2 ** (L + 1)) >= r + 2:
+        L += 1
+    return L

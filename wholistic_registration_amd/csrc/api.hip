@@ -346,7 +346,9 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         g.z_ratio_d = prm->zRatio / std::ldexp(1.0, L);
         g.z_ratio = static_cast<float>(g.z_ratio_d);
         g.mov_range = prm->movRange;
-        tiles[L] = plan_tile(g, prm->tile_patches_x);
+        const int np_x = prm->tile_patches_level ? prm->tile_patches_level[L] : prm->tile_patches_x;
+        WR_ARG(np_x >= 0, "wr_get_motion: negative tile width");
+        tiles[L] = plan_tile(g, np_x);
         down[L] = make_plan(nx, g.nx, ny, g.ny, KERNEL_CUBIC, wa, ia);
         tmp_need = std::max(tmp_need, tmp_bytes(down[L], nz));
         if (L == layer && motion_init && prm->init_nx > 0) {

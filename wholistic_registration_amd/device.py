@@ -109,6 +109,11 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     context had the same ``ref`` tensor (unchanged), size, layer and precision.
     ``context`` (a handle from ``_lib``) and ``stream`` (a torch stream)
     default to the device's shared context and torch's current stream.
+    ``tile_patches_x``: patches per LK workgroup along x, one value for every
+    level or a list of ``layer + 1`` values (level 0 first; 0 = the built-in
+    rule), as ``autotune.tune_tiles`` picks them.  ``motion_init`` may have
+    another x-y size than the volume (full-resolution voxel units; the
+    reference's warm start resamples it to the coarsest level).
     """
     torch = _torch()
     _check_vol(mov, "mov")
@@ -132,7 +137,15 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     prm.layer, prm.iter, prm.r = int(layer), int(iter), int(r)
     prm.zRatio, prm.movRange = float(zRatio), float(movRange)
     prm.smoothPenalty, prm.tol = float(smoothPenalty), float(tol)
-    prm.tile_patches_x = int(tile_patches_x)
+    keep = []
+    if isinstance(tile_patches_x, (list, tuple)):
+        if len(tile_patches_x) != layer + 1:
+            raise ValueError(f"tile_patches_x needs one entry per level (layer + 1 = {layer + 1})")
+        arr = (ctypes.c_int * (layer + 1))(*[int(v) for v in tile_patches_x])
+        prm.tile_patches_level = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int))
+        keep.append(arr)
+    else:
+        prm.tile_patches_x = int(tile_patches_x)
     prm.stop_kernels = 1 if stop_kernels else 0
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
@@ -142,7 +155,6 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
         # a warm start of another x-y size (full-resolution units)
         prm.init_ny, prm.init_nx = int(motion_init.shape[1]), int(motion_init.shape[2])
     prm.timing = 1 if timing else 0
-    keep = []
     if schedule is not None:
         rec = (ctypes.c_int * (layer + 1))(*[schedule[L][0] for L in range(layer + 1)])
         upd = (ctypes.c_int * (layer + 1))(*[schedule[L][1] for L in range(layer + 1)])
@@ -175,11 +187,13 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
         max_diff_motion=bufs["md"], final_error=float(log.final_error), layer=layer)
 
 
-def stats(device=0):
-    """Launch statistics of the last get_motion on ``device`` (HIP-event
-    durations of the LK passes when it ran with ``timing=True``)."""
+def stats(device=0, context=None):
+    """Launch statistics of the last get_motion on ``device``'s shared
+    context, or on ``context`` (HIP-event durations of the LK passes when it
+    ran with ``timing=True``)."""
     st = _lib.WrStats()
-    _lib.check(_lib.load().wr_get_stats(_lib.context(device), ctypes.byref(st)), "wr_get_stats")
+    ctx = context if context is not None else _lib.context(device)
+    _lib.check(_lib.load().wr_get_stats(ctx, ctypes.byref(st)), "wr_get_stats")
     return st
 
 
