@@ -78,3 +78,20 @@ def test_crops_of_uneven_sizes_equal_single_runs(cuda):
             assert single.records == res.records
             assert bool((single.motion == res.motion).all())
             assert bool((D.correct_motion(mov, single.motion) == wv).all())
+
+
+def test_frames_with_tuned_tiles_equal_single_runs(cuda):
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd.batch import FrameBatch
+    mov0, ref = _pair((10, 96, 128), 20, cuda)
+    frames = [mov0] + [_pair((10, 96, 128), 21 + i, cuda)[0] for i in range(3)]
+    with FrameBatch(workers=2, autotune=True) as fb:
+        motion, results = fb.register(frames, ref, layer=2, iter=5, **PARAMS)
+        key = fb.tiles.key(frames[0], 2, PARAMS["r"])
+        tiles = fb.tiles.get(key)
+        assert tiles is not None and len(tiles) == 3
+    for f, m in zip(frames, motion):
+        single = D.get_motion(f, ref, layer=2, iter=5, tile_patches_x=tiles, **PARAMS)
+        assert torch.equal(single.motion, m)

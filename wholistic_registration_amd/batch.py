@@ -35,6 +35,8 @@ class FrameBatch:
     onto ``ref``; ``params`` are the keyword arguments of
     ``device.get_motion`` (``layer``, ``iter``, ``r``, ``zRatio``,
     ``movRange``, ``smoothPenalty``, ``tol``, ``mask_ref``, ``precision``).
+    With ``autotune=True`` every geometry runs with per-level LK tile widths
+    tuned on its first frame (``autotune.TileCache``).
     """
 
     def __init__(self, workers=4, device=0, autotune=False):
@@ -104,9 +106,15 @@ class FrameBatch:
                     key = (ref.data_ptr(), shape, params.get("layer"), params.get("precision", "fp32"),
                            None if params.get("mask_ref") is None else params["mask_ref"].data_ptr())
                     reuse = self._ref_seen[k] == key
+                    extra = dict(params)
+                    if self.tiles is not None:
+                        # tuning (first frame of a geometry) runs on the same template, size,
+                        # layer, precision and mask: the context's template levels stay valid
+                        extra["tile_patches_x"] = self.tiles.tiles(frames[i], ref, context=ctx, stream=strm,
+                                                                   **params)
                     results[i] = _dev.get_motion(
                         frames[i], ref, motion_out=motion_out[i], context=ctx, stream=strm,
-                        reuse_template=reuse, mask_mov=None if masks_mov is None else masks_mov[i], **params)
+                        reuse_template=reuse, mask_mov=None if masks_mov is None else masks_mov[i], **extra)
                     self._ref_seen[k] = key
                     if warped_out is not None:
                         _dev.correct_motion(frames[i], motion_out[i], out=warped_out[i], context=ctx, stream=strm)
