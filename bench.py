@@ -378,8 +378,10 @@ def learned_measure(n_frames=6, reps=5):
     }
     rows = {k: {"ms": [], "epe": [], "epe_truth": [], "final_error": []} for k in variants}
     fallback = 0
+    frames = []
     for i in range(n_frames):
         mov, d = T.make_moving(clean, 90001 + i, dev)
+        frames.append(mov)
         truth = T.truth_field(d)
         judge = None
         for name, fn in variants.items():
@@ -400,7 +402,28 @@ def learned_measure(n_frames=6, reps=5):
             row["final_error"].append(res.final_error)
         if rows["learned_fp32"]["final_error"][-1] > rows["plain_fp32"]["final_error"][-1]:
             fallback += 1
-        del mov, d, truth, judge
+        del d, truth, judge
+    # the same frames (each twice) on 4 concurrent contexts
+    from wholistic_registration_amd.batch import FrameBatch
+    batch = frames + frames
+    conc = {}
+    with FrameBatch(workers=4) as fb:
+        wo = torch.empty((len(batch), nz, ny, nx), dtype=torch.float32, device=dev)
+        mo = torch.empty((len(batch), nz, ny, nx, 3), dtype=torch.float32, device=dev)
+        for name, kw in (("plain_fp32_4ctx", dict(layer=4, iter=15)),
+                         ("learned_fp32_4ctx", dict(net=net, **Lr.SCHEDULE))):
+            fb.register(batch[:4], tmpl, motion_out=mo[:4], warped_out=wo[:4], **kw, **base)
+            ts = []
+            for _ in range(3):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fb.register(batch, tmpl, motion_out=mo, warped_out=wo, **kw, **base)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            ms = 1e3 * float(np.median(ts)) / len(batch)
+            conc[name] = {"value": n0 / (ms * 1e-3), "unit": "registered voxels/s", "ms_per_frame": ms,
+                          "frames": len(batch)}
+    del frames, batch, wo, mo
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     m_l = D.pyramid_level(T.make_moving(clean, 90001, dev)[0], Lr.LEVEL)
@@ -425,7 +448,9 @@ def learned_measure(n_frames=6, reps=5):
                      "frames_within_gate": int(sum(e <= 0.1 for e in row["epe"])),
                      "epe_vs_truth_mean": float(np.mean(row["epe_truth"])),
                      "per_frame": {"ms": [round(v, 3) for v in row["ms"]], "epe": [round(v, 4) for v in row["epe"]]}}
+    out.update(conc)
     out["speedup_learned_fp32"] = out["plain_fp32"]["ms_per_frame"] / out["learned_fp32"]["ms_per_frame"]
+    out["speedup_learned_fp32_4ctx"] = conc["plain_fp32_4ctx"]["ms_per_frame"] / conc["learned_fp32_4ctx"]["ms_per_frame"]
     out["speedup_learned_bf16"] = out["plain_fp32"]["ms_per_frame"] / out["learned_bf16"]["ms_per_frame"]
     out["network"] = {"forward_ms": net_ms, "flops": flops, "achieved_tflops": flops / (net_ms * 1e-3) / 1e12,
                       "peak_tflops": 2500.0, "frac": flops / (net_ms * 1e-3) / 2.5e15,

@@ -88,3 +88,19 @@ def test_prediction_is_deterministic_and_finite(setup):
     assert torch.isfinite(a).all()
     assert tuple(a.shape) == (64, 64, 128, 3)
     assert float(a.abs().max()) < 20.0
+
+
+def test_learned_frames_on_concurrent_contexts_equal_single_runs(setup):
+    import torch
+
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    from wholistic_registration_amd.batch import FrameBatch
+    s = setup
+    frames = [T.make_moving(s["clean"], seed, s["tmpl"].device)[0] for seed in (90003, 90004, 90005)]
+    with FrameBatch(workers=2) as fb:
+        motion, results = fb.register(frames, s["tmpl"], net=s["net"], **Lr.SCHEDULE, **s["params"])
+    for f, m, res in zip(frames, motion, results):
+        single, _ = Lr.register_learned(f, s["tmpl"], s["net"], ref_level=s["ref_l"], **Lr.SCHEDULE, **s["params"])
+        assert torch.equal(single.motion, m)
+        assert single.records == res.records

@@ -74,11 +74,14 @@ class FrameBatch:
     def __exit__(self, *exc):
         self.close()
 
-    def register(self, frames, ref, motion_out=None, warped_out=None, masks_mov=None, **params):
+    def register(self, frames, ref, motion_out=None, warped_out=None, masks_mov=None, net=None, **params):
         """Register every frame; returns ``(motion (T, Z, Y, X, 3), results)``
         where ``results[i]`` is frame i's ``device.MotionResult``.  With
         ``warped_out`` (a ``(T, Z, Y, X)`` float32 tensor) every frame is also
-        warped by its field (``correctMotion``)."""
+        warped by its field (``correctMotion``).  With ``net`` (a
+        ``learned.InitNet``) every frame starts from the network's field
+        (``learned.register_learned``; pass ``learned.SCHEDULE``'s layer and
+        iter), each worker on its own fork of the network."""
         torch = _torch()
         frames = list(frames)
         n = len(frames)
@@ -93,6 +96,13 @@ class FrameBatch:
             raise ValueError("masks_mov needs one mask per frame")
         if motion_out is None:
             motion_out = torch.empty((n,) + shape + (3,), dtype=torch.float32, device=ref.device)
+        ref_level, forks = None, None
+        if net is not None:
+            from . import learned as _learned
+            if masks_mov is not None:
+                raise ValueError("the learned start takes no mask_mov")
+            ref_level = _dev.pyramid_level(ref, _learned.LEVEL)
+            forks = [net.fork(ctx) for ctx in self.contexts]
         # inputs come from the caller's stream: let it finish before the workers start
         torch.cuda.current_stream(ref.device).synchronize()
         results = [None] * n
@@ -112,9 +122,14 @@ class FrameBatch:
                         # layer, precision and mask: the context's template levels stay valid
                         extra["tile_patches_x"] = self.tiles.tiles(frames[i], ref, context=ctx, stream=strm,
                                                                    **params)
-                    results[i] = _dev.get_motion(
-                        frames[i], ref, motion_out=motion_out[i], context=ctx, stream=strm,
-                        reuse_template=reuse, mask_mov=None if masks_mov is None else masks_mov[i], **extra)
+                    if forks is not None:
+                        results[i], _ = _learned.register_learned(
+                            frames[i], ref, forks[k], ref_level=ref_level, motion_out=motion_out[i],
+                            context=ctx, stream=strm, reuse_template=reuse, **extra)
+                    else:
+                        results[i] = _dev.get_motion(
+                            frames[i], ref, motion_out=motion_out[i], context=ctx, stream=strm,
+                            reuse_template=reuse, mask_mov=None if masks_mov is None else masks_mov[i], **extra)
                     self._ref_seen[k] = key
                     if warped_out is not None:
                         _dev.correct_motion(frames[i], motion_out[i], out=warped_out[i], context=ctx, stream=strm)

@@ -96,7 +96,11 @@ Plan2 make_plan(int nx_in, int nx_out, int ny_in, int ny_out, Kernel k, std::vec
     return p;
 }
 
-int upload_tables(Context& c, const std::vector<double>& wa, const std::vector<int>& ia) {
+// The tables are replaced with host-synchronous copies, which are not ordered
+// after work on a non-blocking stream: wait for the caller's stream first, so
+// that no kernel of an earlier call on it still reads the old tables.
+int upload_tables(Context& c, const std::vector<double>& wa, const std::vector<int>& ia, hipStream_t s) {
+    WR_HIP_CHECK(hipStreamSynchronize(s));
     if (int rc = grow(c.tab_w, std::max<size_t>(wa.size(), 1) * sizeof(double))) return rc;
     if (int rc = grow(c.tab_i, std::max<size_t>(ia.size(), 1) * sizeof(int))) return rc;
     if (!wa.empty()) WR_HIP_CHECK(hipMemcpy(c.tab_w.p, wa.data(), wa.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -207,7 +211,7 @@ int wr_imresize(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz, 
     std::vector<double> wa;
     std::vector<int> ia;
     const Plan2 pl = make_plan(nx, nx_out, ny, ny_out, method == 0 ? KERNEL_CUBIC : KERNEL_TENT, wa, ia);
-    WR_TRY(upload_tables(c, wa, ia));
+    WR_TRY(upload_tables(c, wa, ia, s));
     WR_TRY(grow(c.tmp, tmp_bytes(pl, nz)));
     resample2(in, elem_of(dtype), 1, 0, out, OUT_F64, 1.0, pl, c.tab_w.as<double>(), c.tab_i.as<int>(), nz,
               c.tmp.as<double>(), dense(nx_out), s);
@@ -230,7 +234,7 @@ int wr_pyramid_level(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int
     std::vector<double> wa;
     std::vector<int> ia;
     const Plan2 pl = make_plan(nx, lx, ny, ly, KERNEL_CUBIC, wa, ia);
-    WR_TRY(upload_tables(c, wa, ia));
+    WR_TRY(upload_tables(c, wa, ia, s));
     WR_TRY(grow(c.tmp, tmp_bytes(pl, nz)));
     resample2(in, elem_of(dtype), 1, 0, out, OUT_F32, 1.0, pl, c.tab_w.as<double>(), c.tab_i.as<int>(), nz,
               c.tmp.as<double>(), dense(lx), s);
@@ -378,7 +382,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
 
     // ---- workspace
     const size_t n0 = static_cast<size_t>(nx) * ny * nz;
-    WR_TRY(upload_tables(c, wa, ia));
+    WR_TRY(upload_tables(c, wa, ia, s));
     WR_TRY(grow(c.tmp, tmp_need));    // intermediate of the side stream (level images)
     WR_TRY(grow(c.tmp2, tmp_need));   // intermediate of the main stream (warm start, upsample)
     // the moving image (and mask_mov) of every level is stored padded in x, y

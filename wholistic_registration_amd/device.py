@@ -216,7 +216,7 @@ def correct_motion(vol, motion, out=None, context=None, stream=None):
     return out
 
 
-def pyramid_level(vol, level):
+def pyramid_level(vol, level, context=None, stream=None):
     """Level image ``imresize(vol, (x/2^L, y/2^L, z))`` as float32 (Z, Y_L, X_L)."""
     torch = _torch()
     _check_vol(vol, "volume")
@@ -224,9 +224,9 @@ def pyramid_level(vol, level):
     lx, ly = int(nx / 2 ** level), int(ny / 2 ** level)
     out = torch.empty((nz, max(ly, 1), max(lx, 1)), dtype=torch.float32, device=vol.device)
     lib = _lib.load()
-    ctx = _lib.context(vol.device.index or 0)
-    rc = lib.wr_pyramid_level(ctx, _ptr(vol), dtype_code(vol), nx, ny, nz, int(level), _ptr(out),
-                              _stream())
+    ctx = context if context is not None else _lib.context(vol.device.index or 0)
+    strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    rc = lib.wr_pyramid_level(ctx, _ptr(vol), dtype_code(vol), nx, ny, nz, int(level), _ptr(out), strm)
     _lib.check(rc, "wr_pyramid_level")
     return out
 

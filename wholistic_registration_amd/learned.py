@@ -261,11 +261,22 @@ class InitNet:
 
     # ---------------------------------------------------------------- inference
 
+    def fork(self, context=None):
+        """Another handle on the same weights with its own activations (and
+        library context): one per host thread that runs the network."""
+        other = object.__new__(InitNet)
+        other.__dict__.update(self.__dict__)
+        other.act, other.shape = {}, None
+        if context is not None:
+            other.ctx = context
+        return other
+
     def predict(self, mov_l, ref_l, field=None):
         """Warm-start field (Z, Y_L, X_L, 3) float32, full-resolution voxel
-        units, from the level-3 images of the moving and template volumes."""
+        units, from the level-3 images of the moving and template volumes
+        (on torch's current stream)."""
         torch = _torch()
-        out = self.forward(net_input(mov_l, ref_l))
+        out = self.forward(net_input(mov_l, ref_l, context=self.ctx))
         n = mov_l.numel()
         if field is None:
             field = torch.empty(tuple(mov_l.shape) + (3,), dtype=torch.float32, device=self.dev)
@@ -305,7 +316,7 @@ def forward_flops(d, h, w, batch=1):
     return batch * total
 
 
-def net_input(mov_l, ref_l, out=None):
+def net_input(mov_l, ref_l, out=None, context=None):
     """(Z, Y, X) float32 level images -> (1, Z, Y, X, 8) bfloat16 network
     input: both normalised by the template's mean and standard deviation."""
     torch = _torch()
@@ -315,20 +326,21 @@ def net_input(mov_l, ref_l, out=None):
         out = torch.empty((1,) + tuple(mov_l.shape) + (8,), dtype=torch.bfloat16, device=mov_l.device)
     lib = _lib.load()
     mov_c, ref_c = mov_l.contiguous(), ref_l.contiguous()      # held until the launch
-    _lib.check(lib.wr_net_input(_lib.context(mov_l.device.index or 0), _p(mov_c), _p(ref_c), mov_l.numel(),
-                                _p(out), _dev._stream()), "wr_net_input")
+    ctx = context if context is not None else _lib.context(mov_l.device.index or 0)
+    _lib.check(lib.wr_net_input(ctx, _p(mov_c), _p(ref_c), mov_l.numel(), _p(out), _dev._stream()),
+               "wr_net_input")
     return out
 
 
-def level_images(mov, ref, level=LEVEL, ref_level=None):
+def level_images(mov, ref, level=LEVEL, ref_level=None, context=None, stream=None):
     """The network's inputs: pyramid level images of mov and ref (float32)."""
-    m = _dev.pyramid_level(mov, level)
-    r = ref_level if ref_level is not None else _dev.pyramid_level(ref, level)
+    m = _dev.pyramid_level(mov, level, context=context, stream=stream)
+    r = ref_level if ref_level is not None else _dev.pyramid_level(ref, level, context=context, stream=stream)
     return m, r
 
 
 def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol, fallback=None,
-                     plain_iter=None, ref_level=None, **kw):
+                     plain_iter=None, ref_level=None, context=None, stream=None, **kw):
     """getMotion warm-started by the network's field.
 
     Returns ``(result, info)``: ``result`` is a ``device.MotionResult``;
@@ -336,10 +348,15 @@ def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothP
     ``fallback="plain"``, both runs' final residuals.  With ``fallback="plain"``
     the plain method (``plain_iter`` iterations, default ``iter``) also runs
     and its field is kept when the learned run's final residual
-    (``currentError`` of level 0) is higher."""
-    params = dict(layer=layer, r=r, zRatio=zRatio, movRange=movRange, smoothPenalty=smoothPenalty, tol=tol, **kw)
-    m_l, r_l = level_images(mov, ref, ref_level=ref_level)
-    init = net.predict(m_l, r_l)
+    (``currentError`` of level 0) is higher.  ``context`` / ``stream``: the
+    library context (``net`` must run on it: ``net.fork(context)``) and torch
+    stream to run on (default: the shared context, torch's current stream)."""
+    torch = _torch()
+    params = dict(layer=layer, r=r, zRatio=zRatio, movRange=movRange, smoothPenalty=smoothPenalty, tol=tol,
+                  context=context, stream=stream, **kw)
+    with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(mov.device)):
+        m_l, r_l = level_images(mov, ref, ref_level=ref_level, context=context, stream=stream)
+        init = net.predict(m_l, r_l)
     res = _dev.get_motion(mov, ref, iter=iter, motion_init=init, **params)
     info = {"used": "learned", "learned_error": res.final_error}
     if fallback == "plain":
