@@ -71,6 +71,27 @@ def test_fallback_keeps_the_lower_residual(setup):
     assert set(info) >= {"used", "learned_error", "plain_error"}
     assert res.final_error == min(info["learned_error"], info["plain_error"])
     assert info["used"] == ("plain" if info["plain_error"] < info["learned_error"] else "learned")
+    # the plain side is the full pyramid method (4 levels, 15 iterations from zero)
+    from wholistic_registration_amd import device as D
+    plain = D.get_motion(mov, s["tmpl"], layer=4, iter=15, **p)
+    assert info["plain_error"] == plain.final_error
+
+
+def test_fallback_fills_the_callers_buffer(setup):
+    """When the plain field is kept it is the one in motion_out (forced here
+    with a plain run of more iterations than the learned one can match)."""
+    import torch
+
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    s = setup
+    mov, _ = T.make_moving(s["clean"], SEEDS[0], s["tmpl"].device)
+    out = torch.full(tuple(mov.shape) + (3,), float("nan"), device=mov.device)
+    res, info = Lr.register_learned(mov, s["tmpl"], s["net"], ref_level=s["ref_l"], fallback="plain", layer=0, iter=1,
+                                    motion_out=out, **s["params"])
+    assert info["used"] == "plain", info
+    assert res.motion.data_ptr() == out.data_ptr()
+    assert bool(torch.isfinite(out).all())
 
 
 def test_prediction_is_deterministic_and_finite(setup):

@@ -340,15 +340,16 @@ def level_images(mov, ref, level=LEVEL, ref_level=None, context=None, stream=Non
 
 
 def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol, fallback=None,
-                     plain_iter=None, ref_level=None, context=None, stream=None, **kw):
+                     plain_layer=4, plain_iter=15, ref_level=None, context=None, stream=None, **kw):
     """getMotion warm-started by the network's field.
 
     Returns ``(result, info)``: ``result`` is a ``device.MotionResult``;
     ``info`` holds ``used`` ("learned" or "plain") and, with
     ``fallback="plain"``, both runs' final residuals.  With ``fallback="plain"``
-    the plain method (``plain_iter`` iterations, default ``iter``) also runs
-    and its field is kept when the learned run's final residual
-    (``currentError`` of level 0) is higher.  ``context`` / ``stream``: the
+    the plain pyramid method (from zero, ``plain_layer`` levels and
+    ``plain_iter`` iterations; default the 4 levels and 15 iterations the
+    network's targets were made with) also runs and its field is kept when
+    the learned run's final residual (``currentError`` of level 0) is higher.  ``context`` / ``stream``: the
     library context (``net`` must run on it: ``net.fork(context)``) and torch
     stream to run on (default: the shared context, torch's current stream)."""
     torch = _torch()
@@ -360,10 +361,17 @@ def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothP
     res = _dev.get_motion(mov, ref, iter=iter, motion_init=init, **params)
     info = {"used": "learned", "learned_error": res.final_error}
     if fallback == "plain":
-        kw_plain = {k: v for k, v in params.items() if k != "motion_out"}
-        plain = _dev.get_motion(mov, ref, iter=plain_iter or iter, **kw_plain)
+        kw_plain = {k: v for k, v in params.items() if k not in ("motion_out", "layer", "reuse_template")}
+        if isinstance(kw_plain.get("tile_patches_x"), (list, tuple)):    # per-level widths of the learned run
+            kw_plain.pop("tile_patches_x")
+        plain = _dev.get_motion(mov, ref, layer=plain_layer, iter=plain_iter, **kw_plain)
         info["plain_error"] = plain.final_error
         if plain.final_error < res.final_error:
+            out = params.get("motion_out")
+            if out is not None:                      # the caller's buffer holds the kept field
+                with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(mov.device)):
+                    out.copy_(plain.motion)
+                plain.motion = out
             res, info["used"] = plain, "plain"
     elif fallback is not None:
         raise ValueError("fallback must be None or 'plain'")
