@@ -34,6 +34,17 @@ FRAME = (64, 512, 1024)
 LK_PARAMS = dict(layer=4, iter=15, r=8, zRatio=1.0, movRange=5.0, tol=1e-3)
 RAW_PENALTY = 0.08
 
+# Pair recipes: the frame geometry of the time-series frames (the default,
+# 100k blobs, 6-mode fields up to 6 voxels, r 8, 15 iterations) and of the
+# large single volume (400k blobs, 8-mode fields up to 8 voxels, a 10 %
+# ellipsoid mask_ref, r 16, 20 iterations).
+RECIPES = {
+    "frame_64x512x1024": dict(zyx=(64, 512, 1024), blobs=100_000, modes=6, dmax=6.0, mask_fraction=0.0,
+                              lk=dict(LK_PARAMS)),
+    "volume_96x1024x2048": dict(zyx=(96, 1024, 2048), blobs=400_000, modes=8, dmax=8.0, mask_fraction=0.10,
+                                lk=dict(layer=4, iter=20, r=16, zRatio=1.0, movRange=5.0, tol=1e-3)),
+}
+
 
 def _torch():
     import torch
@@ -49,6 +60,11 @@ def make_template(seed, dev, zyx=FRAME, n_blobs=100_000):
     g.manual_seed(seed)
     tmpl = torch.poisson(clean, generator=g).round_().clamp_(0, 65535).to(torch.uint16).contiguous()
     return clean, tmpl
+
+
+def make_mask(seed, zyx, fraction, dev):
+    """A random ellipsoid covering ``fraction`` of the voxels (float32, 1 = excluded)."""
+    return synth.ellipsoid_mask_torch(np.random.default_rng(seed), zyx, fraction, dev).float().contiguous()
 
 
 def make_moving(clean, seed, dev, n_modes=6, dmax=6.0):
@@ -90,18 +106,21 @@ def penalty(tmpl):
 class PairSet:
     """Network inputs and targets of generated pairs, resident on the device."""
 
-    def __init__(self, dev, templates, fields_per_template, seed0, target="lk", log=None):
+    def __init__(self, dev, templates, fields_per_template, seed0, target="lk", log=None, recipe="frame_64x512x1024"):
         torch = _torch()
+        rc = RECIPES[recipe]
         self.x, self.t, self.meta = [], [], []
         for ti in range(templates):
-            clean, tmpl = make_template(seed0 + 1000 * ti, dev)
+            clean, tmpl = make_template(seed0 + 1000 * ti, dev, zyx=rc["zyx"], n_blobs=rc["blobs"])
             ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
-            sp = penalty(tmpl)
+            mask = make_mask(seed0 + 1000 * ti, rc["zyx"], rc["mask_fraction"], dev) if rc["mask_fraction"] else None
+            sp = penalty(tmpl) if mask is None else float(D.smooth_penalty_factor(
+                tmpl.float().contiguous(), mask)) * RAW_PENALTY
             for fi in range(fields_per_template):
                 seed = seed0 + 1000 * ti + 1 + fi
-                mov, d = make_moving(clean, seed, dev)
+                mov, d = make_moving(clean, seed, dev, rc["modes"], rc["dmax"])
                 if target == "lk":
-                    res = D.get_motion(mov, tmpl, smoothPenalty=sp, **LK_PARAMS)
+                    res = D.get_motion(mov, tmpl, smoothPenalty=sp, mask_ref=mask, **rc["lk"])
                     full = res.motion
                 elif target == "truth":
                     full = truth_field(d)
@@ -174,6 +193,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--target", default="lk", choices=("lk", "truth"))
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--recipe", default="frame_64x512x1024", choices=sorted(RECIPES))
     ap.add_argument("--report", default=None, help="JSON file for the loss curve and validation")
     args = ap.parse_args(argv)
     torch = _torch()
@@ -183,8 +203,8 @@ def main(argv=None):
 
     dev = torch.device("cuda", 0)
     t0 = time.time()
-    pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log)
-    val = PairSet(dev, args.val_templates, 4, 20000, args.target, log)
+    pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log, args.recipe)
+    val = PairSet(dev, args.val_templates, 4, 20000, args.target, log, args.recipe)
     t1 = time.time()
     net = Lr.InitNet(seed=args.seed, lr=args.lr)
     torch.cuda.synchronize()
@@ -194,7 +214,8 @@ def main(argv=None):
     t3 = time.time()
     v_loss, v_zero = evaluate(net, val)
     net.save(args.out)
-    rep = {"pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target, "steps": args.steps,
+    rep = {"recipe": args.recipe, "pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target,
+           "steps": args.steps,
            "batch": args.batch, "lr": args.lr, "data_s": t1 - t0, "train_s": t3 - t2,
            "ms_per_step": 1e3 * (t3 - t2) / args.steps, "curve": curve,
            "val_mse": v_loss, "val_mse_zero_field": v_zero}
