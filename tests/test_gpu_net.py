@@ -265,3 +265,27 @@ def test_low_resolution_warm_start_matches_oracle(cuda):
     got = res.motion.cpu().numpy().transpose(2, 1, 0, 3)
     assert np.abs(got - want).max() <= 1e-3
     assert res.final_error == pytest.approx(err, rel=2e-5)
+
+
+def test_entry_points_reject_bad_shapes(cuda):
+    """Shape and alignment rules are errors, not faults."""
+    import torch
+    L = _lib().load()
+    x = torch.zeros((1, 4, 4, 8, 16), dtype=torch.bfloat16, device=cuda)
+    w = torch.zeros((16, 27, 16), dtype=torch.bfloat16, device=cuda)
+    y = torch.zeros_like(x)
+    cases = [
+        L.wr_net_conv3d(_ctx(), _p(x), _p(w), None, _p(y), 1, 4, 4, 8, 12, 16, 1, 0.1, _s()),   # cin % 8
+        L.wr_net_conv3d(_ctx(), _p(x), _p(w), None, _p(y), 1, 4, 4, 8, 16, 24, 1, 0.1, _s()),   # cout % 16
+        L.wr_net_conv3d(_ctx(), ctypes.c_void_p(x.data_ptr() + 2), _p(w), None, _p(y), 1, 4, 4, 8, 16, 16, 1,
+                        0.1, _s()),                                                          # alignment
+        L.wr_net_conv3d(_ctx(), _p(x), _p(w), None, _p(y), 0, 4, 4, 8, 16, 16, 1, 0.1, _s()),    # empty batch
+        L.wr_net_conv3d_bwd_data(_ctx(), _p(x), _p(w), _p(y), 1, 4, 4, 8, 8, 16, _s()),          # cin % 16
+        L.wr_net_conv3d_bwd_weight(_ctx(), _p(x), _p(x), _p(y), None, 1, 4, 4, 12, 16, 16, _s()),  # w % 8
+        L.wr_net_down2(_ctx(), _p(x), _p(y), 1, 4, 4, 7, 16, 0.125, _s()),                        # odd extent
+        L.wr_net_up2(_ctx(), _p(x), _p(y), 1, 3, 4, 8, 16, 1.0, 0, _s()),                          # odd extent
+        L.wr_net_leaky_bwd(_ctx(), _p(x), _p(y), 12, 0.1, _s()),                                   # n % 8
+        L.wr_net_mse(_ctx(), _p(x), 2, _p(x), 3, 16, 4.0, 1.0, _p(y), _p(y), _s()),                # ct > cp
+        L.wr_net_adam(_ctx(), _p(y), _p(y), _p(y), _p(y), None, 16, 1e-3, 0.9, 0.999, 1e-8, 0, _s()),  # step 0
+    ]
+    assert all(rc == _lib().WR_E_ARG for rc in cases), cases

@@ -99,3 +99,27 @@ def test_smooth_penalty_null_arguments_are_rejected():
     out = ctypes.c_double()
     rc = lib.wr_smooth_penalty_factor(None, None, 0, 4, 4, 4, None, ctypes.byref(out), None)
     assert rc == _lib.WR_E_ARG
+
+
+@pytest.mark.parametrize("name,args", [
+    ("wr_net_conv3d", (None, None, None, None, None, 1, 4, 4, 8, 8, 16, 1, 0.1, None)),
+    ("wr_net_conv3d_bwd_data", (None, None, None, None, 1, 4, 4, 8, 16, 16, None)),
+    ("wr_net_conv3d_bwd_weight", (None, None, None, None, None, 1, 4, 4, 8, 8, 16, None)),
+    ("wr_net_leaky_bwd", (None, None, None, 8, 0.1, None)),
+    ("wr_net_down2", (None, None, None, 1, 4, 4, 8, 8, 0.125, None)),
+    ("wr_net_up2", (None, None, None, 1, 4, 4, 8, 8, 1.0, 0, None)),
+    ("wr_net_concat", (None, None, 8, None, 8, 16, None, None)),
+    ("wr_net_split", (None, None, 8, 8, 16, None, None, None)),
+    ("wr_net_mse", (None, None, 16, None, 3, 16, 4.0, 1.0, None, None, None)),
+    ("wr_net_adam", (None, None, None, None, None, None, 10, 1e-3, 0.9, 0.999, 1e-8, 1, None)),
+    ("wr_net_to_bf16", (None, None, 10, None, None)),
+    ("wr_net_input", (None, None, None, 10, None, None)),
+    ("wr_net_output", (None, None, 16, 10, 4.0, None, None)),
+])
+def test_network_entry_points_reject_null_arguments(name, args):
+    """The learned initialisation's entry points validate before touching a
+    device: a null context or buffer is WR_E_ARG with the function named."""
+    lib = _lib.load()
+    rc = getattr(lib, name)(*args)
+    assert rc == _lib.WR_E_ARG
+    assert name in _lib.last_error()
