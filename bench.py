@@ -201,8 +201,8 @@ def precision_measure(w, n0, step, motion_fp32, field, mask, steps, prec):
 
 
 def batch_measure(n_frames, worker_counts, seed=2):
-    """Frames of a time series against one template (BASELINE config 2's
-    frame geometry): a 64 x 512 x 1024 uint16 template with 100k blobs, frames
+    """Frames of a time series against one template (the time-series frame
+    geometry): a 64 x 512 x 1024 uint16 template with 100k blobs, frames
     pull-warped by their own 6-mode fields (max |d| = 6) with fresh Poisson
     noise; layer 4, r = 8, 15 iterations, no mask.  Registered voxels/s with
     one context (frames in sequence) and with several concurrent contexts
@@ -339,7 +339,7 @@ def learned_measure(n_frames=6, reps=5):
     ``reps`` runs (the learned path also computes the moving frame's level-3
     image and runs the network; the template's level-3 image is computed once
     per template), endpoint error of every field against the plain FP32 field
-    (layer 4, r 8, 15 iterations: the gate's judge, equal to the float64
+    (layer 4, r 8, 15 iterations: the gate's yardstick, equal to the float64
     oracle within 1e-3 voxel at oracle sizes) and against the generating
     field, and whether the fallback rule (keep the plain field when its final
     residual is lower) would fire."""
@@ -383,7 +383,7 @@ def learned_measure(n_frames=6, reps=5):
         mov, d = T.make_moving(clean, 90001 + i, dev)
         frames.append(mov)
         truth = T.truth_field(d)
-        judge = None
+        plain_field = None
         for name, fn in variants.items():
             fn(mov)                                           # warm (workspace sizes)
             ts = []
@@ -393,16 +393,16 @@ def learned_measure(n_frames=6, reps=5):
                 res = fn(mov)
                 torch.cuda.synchronize()
                 ts.append(time.perf_counter() - t0)
-            if judge is None:
-                judge = motion.clone()
+            if plain_field is None:
+                plain_field = motion.clone()
             row = rows[name]
             row["ms"].append(1e3 * float(np.median(ts)))
-            row["epe"].append(float(torch.sqrt(((motion - judge) ** 2).sum(-1)).mean()))
+            row["epe"].append(float(torch.sqrt(((motion - plain_field) ** 2).sum(-1)).mean()))
             row["epe_truth"].append(float(torch.sqrt(((motion - truth) ** 2).sum(-1)).mean()))
             row["final_error"].append(res.final_error)
         if rows["learned_fp32"]["final_error"][-1] > rows["plain_fp32"]["final_error"][-1]:
             fallback += 1
-        del d, truth, judge
+        del d, truth, plain_field
     # the same frames (each twice) on 4 concurrent contexts
     from wholistic_registration_amd.batch import FrameBatch
     batch = frames + frames
@@ -439,12 +439,12 @@ def learned_measure(n_frames=6, reps=5):
     flops = Lr.forward_flops(*m_l.shape)
     out = {"workload": f"{n_frames} held-out pairs of {nz}x{ny}x{nx} (z,y,x) uint16 frames (6-mode fields, "
                        f"max |d| 6 voxels), getMotion + correctMotion per frame",
-           "judge": "plain FP32 field, layer 4, r 8, 15 iterations",
+           "epe_against": "plain FP32 field, layer 4, r 8, 15 iterations",
            "learned_schedule": dict(Lr.SCHEDULE), "fallback_would_fire": fallback}
     for name, row in rows.items():
         ms = float(np.mean(row["ms"]))
         out[name] = {"value": n0 / (ms * 1e-3), "unit": "registered voxels/s", "ms_per_frame": ms,
-                     "epe_vs_judge_mean": float(np.mean(row["epe"])), "epe_vs_judge_max": float(np.max(row["epe"])),
+                     "epe_vs_plain_mean": float(np.mean(row["epe"])), "epe_vs_plain_max": float(np.max(row["epe"])),
                      "frames_within_gate": int(sum(e <= 0.1 for e in row["epe"])),
                      "epe_vs_truth_mean": float(np.mean(row["epe_truth"])),
                      "per_frame": {"ms": [round(v, 3) for v in row["ms"]], "epe": [round(v, 4) for v in row["epe"]]}}
@@ -566,7 +566,7 @@ def main():
                     help="reduced-precision level-image variants to time beside the float32 path "
                          "(comma list of bf16, fp16; 'none' to skip)")
     ap.add_argument("--batch-frames", type=int, default=32,
-                    help="frames of the time-series measurement (config 2 frame size; 0 to skip)")
+                    help="frames of the time-series measurement (64x512x1024 frames; 0 to skip)")
     ap.add_argument("--no-mask-measure", action="store_true",
                     help="skip timing the harness's automatic mask kernels")
     ap.add_argument("--learned-frames", type=int, default=6,

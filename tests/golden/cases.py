@@ -100,20 +100,20 @@ def small_inputs(name):
     return mov, ref, opt
 
 
-# config 0 of the workload set: 64x128x128 (z, y, x) float32, 2000 blobs,
-# 4 modes with max |d| = 3, layer 3, 16^3 patches (r = 8), 10 iterations
-CONFIG0 = dict(seed=0, zyx=(64, 128, 128), n_blobs=2000, n_modes=4, dmax=3.0,
-               layer=3, r=8, iter=10, zRatio=1.0, raw_penalty=0.08)
+# the small blob pair: 64x128x128 (z, y, x) float32, 2000 blobs,
+# 4 modes with max |d| = 3, layer 3, 17x17x1 patches (r = 8), 10 iterations
+BLOBS_64x128x128 = dict(seed=0, zyx=(64, 128, 128), n_blobs=2000, n_modes=4, dmax=3.0,
+                        layer=3, r=8, iter=10, zRatio=1.0, raw_penalty=0.08)
 
 
-def config0_pair():
-    c = CONFIG0
+def pair_64x128x128():
+    c = BLOBS_64x128x128
     return synth.make_pair_np(c["seed"], c["zyx"], c["n_blobs"], c["n_modes"], c["dmax"])
 
 
-def config0_inputs(pair=None):
-    c = CONFIG0
-    pair = pair if pair is not None else config0_pair()
+def inputs_64x128x128(pair=None):
+    c = BLOBS_64x128x128
+    pair = pair if pair is not None else pair_64x128x128()
     mov = pair.moving.transpose(2, 1, 0)
     ref = pair.template.transpose(2, 1, 0)
     opt = _base_option(c["layer"], c["r"], c["iter"], c["zRatio"])

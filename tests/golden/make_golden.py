@@ -3,7 +3,7 @@
 Run in the development container only (the reference tree is not present on
 GPU machines):
 
-    python tests/golden/make_golden.py [--reference /root/reference] [--skip-config0]
+    python tests/golden/make_golden.py [--reference /root/reference] [--skip-64x128x128]
 
 It imports the reference's hot-path modules
 (``src/wholistic_registration/utils/{calFlow3d_Wei_v1,imresize,interp,
@@ -38,7 +38,7 @@ sys.path.insert(0, REPO)
 from wholistic_registration_amd import synth  # noqa: E402
 
 from tests.golden.cases import (  # noqa: E402
-    CONFIG0, IMRESIZE_CASES, SMALL_CASES, config0_inputs, imresize_input, small_inputs,
+    BLOBS_64x128x128, IMRESIZE_CASES, SMALL_CASES, inputs_64x128x128, imresize_input, small_inputs,
 )
 
 
@@ -102,8 +102,8 @@ def write_small(cf):
                           for L in range(option["layer"], -1, -1)])
 
 
-def write_config0(cf):
-    mov, ref, option = config0_inputs()
+def write_64x128x128(cf):
+    mov, ref, option = inputs_64x128x128()
     motion, err, log = run_reference(cf, mov, ref, option)
     r = option["r"]
     nx, ny, nz = mov.shape
@@ -118,21 +118,21 @@ def write_config0(cf):
         sample_idx=pick, sample_motion=motion[pick[:, 0], pick[:, 1], pick[:, 2]],
         motion_absmax=np.float32(np.abs(motion).max()),
         current_error=np.float64(err), **log_arrays(log, option["layer"]))
-    np.savez_compressed(os.path.join(HERE, "getmotion_config0.npz"), **payload)
-    print("config0 err", err, "records",
+    np.savez_compressed(os.path.join(HERE, "getmotion_64x128x128.npz"), **payload)
+    print("64x128x128 err", err, "records",
           [len(log[f"layer_{L}"]["currentError"]) for L in range(option["layer"], -1, -1)])
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--skip-config0", action="store_true")
+    ap.add_argument("--skip-64x128x128", action="store_true")
     args = ap.parse_args()
     cf, im = import_reference(args.reference)
     write_imresize(im)
     write_small(cf)
-    if not args.skip_config0:
-        write_config0(cf)
+    if not args.skip_64x128x128:
+        write_64x128x128(cf)
     _ = synth  # the inputs come from the package's own generator
 
 

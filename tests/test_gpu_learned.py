@@ -3,7 +3,7 @@
 90001..; training used template seeds 10000 + 1000 i, i < 32, with frames
 +1..+16, and validation 20000, 21000).
 
-The plain pyramid LK (layer 4, r 8, 15 iterations, float32) is the judge:
+The plain pyramid LK (layer 4, r 8, 15 iterations, float32) is the yardstick:
 endpoint errors are measured against its field.  The network's field starts
 getMotion at full resolution (layer 0) with 5 iterations and ends within the
 0.1-voxel gate; the same schedule started from zero (the method without the
@@ -49,12 +49,12 @@ def test_learned_start_beats_method_without_it(setup, seed):
     s = setup
     mov, _ = T.make_moving(s["clean"], seed, s["tmpl"].device)
     p = s["params"]
-    judge = D.get_motion(mov, s["tmpl"], layer=4, iter=15, **p).motion.clone()
+    plain_field = D.get_motion(mov, s["tmpl"], layer=4, iter=15, **p).motion.clone()
     learned, info = Lr.register_learned(mov, s["tmpl"], s["net"], ref_level=s["ref_l"], **Lr.SCHEDULE, **p)
     assert info["used"] == "learned"
-    e_learned = _epe(learned.motion, judge)
+    e_learned = _epe(learned.motion, plain_field)
     zero = D.get_motion(mov, s["tmpl"], **Lr.SCHEDULE, **p)
-    e_zero = _epe(zero.motion, judge)
+    e_zero = _epe(zero.motion, plain_field)
     assert e_learned <= 0.1, e_learned               # the gate
     assert e_zero > 0.1 and e_zero > 1.5 * e_learned, (e_zero, e_learned)
     assert learned.final_error < zero.final_error

@@ -17,7 +17,7 @@ import pytest
 
 from oracle import lk_oracle as O
 from oracle.metrics import epe_stats, rel_l2
-from tests.golden.cases import (CONFIG0, IMRESIZE_CASES, SMALL_CASES, config0_inputs,
+from tests.golden.cases import (BLOBS_64x128x128, IMRESIZE_CASES, SMALL_CASES, inputs_64x128x128,
                                 imresize_input, small_inputs)
 
 pytestmark = pytest.mark.gpu
@@ -134,13 +134,13 @@ def test_get_motion_free_running_matches_reference(cuda, name, stop_kernels):
 
 
 @pytest.mark.parametrize("stop_kernels", [False, True])
-def test_get_motion_config0_matches_reference(cuda, stop_kernels):
-    g = np.load(os.path.join(GOLDEN, "getmotion_config0.npz"))
-    mov, ref, opt = config0_inputs()
+def test_get_motion_64x128x128_matches_reference(cuda, stop_kernels):
+    g = np.load(os.path.join(GOLDEN, "getmotion_64x128x128.npz"))
+    mov, ref, opt = inputs_64x128x128()
     res, got = _run_gpu(mov, ref, opt, cuda, stop_kernels=stop_kernels)
-    for L in range(CONFIG0["layer"], -1, -1):
+    for L in range(BLOBS_64x128x128["layer"], -1, -1):
         assert res.records[L] == len(g[f"L{L}_currentError"]), L
-    r = CONFIG0["r"]
+    r = BLOBS_64x128x128["r"]
     nx, ny, nz = mov.shape
     xg = np.arange(r, nx - 1, 2 * r + 1)
     yg = np.arange(r, ny - 1, 2 * r + 1)
@@ -181,7 +181,7 @@ def test_identity_pair_gives_exact_zero(cuda):
 
 def test_deterministic(cuda):
     import torch
-    mov, ref, opt = config0_inputs()
+    mov, ref, opt = inputs_64x128x128()
     a, ma = _run_gpu(mov, ref, opt, cuda)
     b, mb = _run_gpu(mov, ref, opt, cuda)
     np.testing.assert_array_equal(ma, mb)

@@ -13,7 +13,7 @@ import pytest
 
 from oracle import lk_oracle as O
 from oracle.metrics import epe_stats
-from tests.golden.cases import config0_inputs, small_inputs
+from tests.golden.cases import inputs_64x128x128, small_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -38,10 +38,10 @@ def _run(mov, ref, opt, cuda, precision):
     return res.motion.cpu().numpy().transpose(2, 1, 0, 3)
 
 
-@pytest.mark.parametrize("case", ["plain", "masked", "odd", "config0"])
+@pytest.mark.parametrize("case", ["plain", "masked", "odd", "64x128x128"])
 @pytest.mark.parametrize("precision", ["bf16", "fp16"])
 def test_reduced_precision_epe_gate(cuda, case, precision):
-    mov, ref, opt = config0_inputs() if case == "config0" else small_inputs(case)
+    mov, ref, opt = inputs_64x128x128() if case == "64x128x128" else small_inputs(case)
     m64, _, _, _ = O.get_motion(mov, ref, dict(opt), mode="fp64", keep_images=False)
     got = _run(mov, ref, opt, cuda, precision)
     f32 = _run(mov, ref, opt, cuda, "fp32")

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import lk_oracle as O
-from tests.golden.cases import (CONFIG0, IMRESIZE_CASES, SMALL_CASES, config0_inputs,
+from tests.golden.cases import (BLOBS_64x128x128, IMRESIZE_CASES, SMALL_CASES, inputs_64x128x128,
                                 imresize_input, small_inputs)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -66,16 +66,16 @@ def test_get_motion_small_cases(name, sums):
                                   motion[..., 0] + np.arange(nx, dtype=np.float32)[:, None, None])
 
 
-def test_get_motion_config0():
-    """Config 0 (64x128x128, layer 3, r 8, 10 iterations) against the
+def test_get_motion_64x128x128():
+    """The 64x128x128 blob pair (, layer 3, r 8, 10 iterations) against the
     reference's run on the same bytes."""
-    g = _load("getmotion_config0.npz")
-    mov, ref, opt = config0_inputs()
+    g = _load("getmotion_64x128x128.npz")
+    mov, ref, opt = inputs_64x128x128()
     assert hashlib.sha256(np.ascontiguousarray(mov).tobytes()).hexdigest() == str(g["mov_sha"])
     assert hashlib.sha256(np.ascontiguousarray(ref).tobytes()).hexdigest() == str(g["ref_sha"])
     assert opt["smoothPenalty"] == float(g["smooth_penalty"])
     motion, err, _, log = O.get_motion(mov, ref, dict(opt), keep_images=False)
-    r = CONFIG0["r"]
+    r = BLOBS_64x128x128["r"]
     nx, ny, nz = mov.shape
     xg = np.arange(r, nx - 1, 2 * r + 1)
     yg = np.arange(r, ny - 1, 2 * r + 1)
@@ -84,5 +84,5 @@ def test_get_motion_config0():
     np.testing.assert_allclose(motion[idx[:, 0], idx[:, 1], idx[:, 2]], g["sample_motion"], rtol=0,
                                atol=1e-5)
     assert err == pytest.approx(float(g["current_error"]), rel=1e-9)
-    for L in range(CONFIG0["layer"], -1, -1):
+    for L in range(BLOBS_64x128x128["layer"], -1, -1):
         assert len(log[f"layer_{L}"]["currentError"]) == len(g[f"L{L}_currentError"])

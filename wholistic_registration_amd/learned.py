@@ -5,9 +5,10 @@ and predicts the registration field there (in full-resolution voxel units).
 That field is the warm start the reference already accepts,
 ``option["motion"]`` (calFlow3d_Wei_v1.py:362-377): getMotion resamples it
 to the coarsest level and the pyramid LK refines it, with fewer iterations
-per level than from zero.  The plain pyramid LK stays the judge and the
-fallback: ``register_learned(..., fallback="plain")`` also runs the plain
-method and keeps whichever field leaves the lower final residual.
+per level than from zero.  The plain pyramid LK stays the yardstick of its
+endpoint error and the fallback: ``register_learned(..., fallback="plain")``
+also runs the plain method and keeps whichever field leaves the lower final
+residual.
 
 Every layer runs in libwrhip.so (``wr_net_*``): 3x3x3 convolutions as
 bfloat16 MFMA implicit GEMMs with float32 accumulation (forward, data and
