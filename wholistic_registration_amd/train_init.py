@@ -12,8 +12,9 @@ that level, in full-resolution voxel units:
   result, which the learned start has to reproduce with fewer iterations;
 * ``target="truth"``: the generating registration field m = d(p + m).
 
-Run ``python -m wholistic_registration_amd.train_init --out weights.safetensors``
-on the GPU.
+``python -m wholistic_registration_amd.train_init --out weights.safetensors``
+reproduces the shipped weights' recipe on one GPU (1024 pairs, 20k Adam steps
+of batch 4, about 3 minutes).
 """
 
 from __future__ import annotations
@@ -184,10 +185,11 @@ def evaluate(net, pairs):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", required=True, help="safetensors file for the trained weights")
-    ap.add_argument("--templates", type=int, default=8)
+    # defaults: the shipped weights (weights/init_net.safetensors)
+    ap.add_argument("--templates", type=int, default=64)
     ap.add_argument("--fields", type=int, default=16, help="moving frames per template")
     ap.add_argument("--val-templates", type=int, default=2)
-    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--warmup", type=int, default=200)
