@@ -92,6 +92,16 @@ typedef struct wr_params {
                                workgroup along x of level L (0 = the
                                built-in rule): what the tile autotuner
                                picks per level */
+    int init_cp;            /* 1 = motion_init is the coarsest level's control-
+                               point field itself: 3 planes (x, y, z
+                               components) of nz * ncy * ncx float32 in that
+                               level's voxel units, ncx = len(range(r, nx_L - 1,
+                               2r + 1)) (calFlow3d_Wei_v1.py:434-436); the level
+                               starts from its bilinear interpolant with no
+                               dense start field.  No counterpart in the
+                               reference: the start of the learned
+                               initialisation, which skips resampling a dense
+                               field that the first update replaces anyway */
 } wr_params;
 
 /* Per-level iteration log, filled on the host when wr_get_motion returns
@@ -345,6 +355,13 @@ int wr_net_to_bf16(wr_ctx* ctx, const float* in, long long n, void* out, void* s
 int wr_net_input(wr_ctx* ctx, const float* mov, const float* ref, long long n, void* out, void* stream);
 /* field (n, 3) float32 = scale * p[:, 0:3] of p (n, cp). */
 int wr_net_output(wr_ctx* ctx, const void* p, int cp, long long n, double scale, float* field, void* stream);
+/* The network's field (p: nz x h x w voxels of cp channels, on pyramid level
+ * q of the full volume, scale * channels 0..2 in full-resolution units)
+ * sampled bilinearly at the control points of pyramid level `level` (ncx x
+ * ncy per plane, half-width r), in that level's units: cp_out is the 3-plane
+ * field wr_params.init_cp takes. */
+int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int q, int ncx, int ncy, int r,
+                     int level, double scale, float* cp_out, void* stream);
 
 #ifdef __cplusplus
 }

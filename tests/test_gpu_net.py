@@ -289,3 +289,33 @@ def test_entry_points_reject_bad_shapes(cuda):
         L.wr_net_adam(_ctx(), _p(y), _p(y), _p(y), _p(y), None, 16, 1e-3, 0.9, 0.999, 1e-8, 0, _s()),  # step 0
     ]
     assert all(rc == _lib().WR_E_ARG for rc in cases), cases
+
+
+def test_cp_start_equals_dense_start_of_its_interpolant(cuda):
+    """motion_init_cp (wr_params.init_cp): a level started from a CP field
+    behaves as the reference's dense warm start with that field's bilinear
+    interpolant; the interpolant equals the CP values at the control points."""
+    import torch
+
+    from tests.golden.cases import small_inputs
+    from wholistic_registration_amd import device as D
+    mov, ref, opt = small_inputs("plain")
+    zyx = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32).transpose(2, 1, 0))).to(cuda)
+    mt, rt = zyx(mov), zyx(ref)
+    nz, ny, nx = mt.shape
+    r = opt["r"]
+    P = dict(r=r, zRatio=opt["zRatio"], movRange=opt["movRange"], smoothPenalty=opt["smoothPenalty"], tol=opt["tol"])
+    ncx, ncy = D.cp_grid((nz, ny, nx), r)
+    rng = np.random.default_rng(13)
+    cp = torch.from_numpy((0.6 * rng.standard_normal((3, nz, ncy, ncx))).astype(np.float32)).to(cuda)
+    start = D.get_motion(mt, rt, layer=0, iter=1, motion_init_cp=cp, **P)
+    assert start.updates[0] == 0
+    p = 2 * r + 1
+    at_cp = start.motion[:, r:r + p * ncy:p, r:r + p * ncx:p, :].permute(3, 0, 1, 2)
+    assert torch.equal(at_cp, cp)
+    dense = D.get_motion(mt, rt, layer=0, iter=4, motion_init=start.motion.clone(), **P)
+    via_cp = D.get_motion(mt, rt, layer=0, iter=4, motion_init_cp=cp, **P)
+    assert dense.records == via_cp.records
+    assert float((dense.motion - via_cp.motion).abs().max()) <= 1e-4
+    with pytest.raises(ValueError):
+        D.get_motion(mt, rt, layer=0, iter=2, motion_init_cp=cp[:, :, :-1].contiguous(), **P)

@@ -39,6 +39,7 @@ class WrParams(ctypes.Structure):
         ("init_nx", ctypes.c_int),
         ("init_ny", ctypes.c_int),
         ("tile_patches_level", ctypes.POINTER(ctypes.c_int)),
+        ("init_cp", ctypes.c_int),
     ]
 
 
@@ -110,6 +111,7 @@ _SIGNATURES = {
     "wr_net_to_bf16": ([_VP, _VP, ctypes.c_longlong, _VP, _VP], _I),
     "wr_net_input": ([_VP, _VP, _VP, ctypes.c_longlong, _VP, _VP], _I),
     "wr_net_output": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, _VP, _VP], _I),
+    "wr_net_output_cp": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double, _VP, _VP], _I),
 }
 
 _lib = None

@@ -330,6 +330,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     std::vector<Plan2> down(layer + 1), up(layer + 1);
     Plan2 init_plan{};
     WR_ARG((prm->init_nx > 0) == (prm->init_ny > 0), "wr_get_motion: init_nx and init_ny must both be set or both 0");
+    WR_ARG(!prm->init_cp || motion_init, "wr_get_motion: init_cp needs motion_init");
     std::vector<double> wa;
     std::vector<int> ia;
     size_t tmp_need = 0, ncp_max = 0, nblk_max = 0, ncpb_max = 0, nvox_prev = 0;
@@ -355,7 +356,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         tiles[L] = plan_tile(g, np_x);
         down[L] = make_plan(nx, g.nx, ny, g.ny, KERNEL_CUBIC, wa, ia);
         tmp_need = std::max(tmp_need, tmp_bytes(down[L], nz));
-        if (L == layer && motion_init && prm->init_nx > 0) {
+        if (L == layer && motion_init && !prm->init_cp && prm->init_nx > 0) {
             init_plan = make_plan(prm->init_nx, g.nx, prm->init_ny, g.ny, KERNEL_CUBIC, wa, ia);
             tmp_need = std::max(tmp_need, tmp_bytes(init_plan, nz));
         }
@@ -483,8 +484,12 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         const LkTile& t = tiles[L];
         const size_t nv = static_cast<size_t>(g.nx) * g.ny * nz;
 
-        // initial dense field of the level, 3 planes
-        if (L == layer) {
+        // initial dense field of the level, 3 planes (or, with init_cp, the CP field itself)
+        const bool cp_start = L == layer && motion_init && prm->init_cp;
+        if (cp_start) {
+            WR_HIP_CHECK(hipMemcpyAsync(c.cp.p, motion_init, 3 * static_cast<size_t>(g.ncx) * g.ncy * nz * sizeof(float),
+                                        hipMemcpyDeviceToDevice, s));
+        } else if (L == layer) {
             if (motion_init) {
                 const double div[3] = {1.0 * nx / g.nx, 1.0 * ny / g.ny, 1.0 * nz / g.nz};
                 for (int q = 0; q < 3; ++q)
@@ -500,7 +505,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
                 resample2(mprev + q * np, E_F32, 1, 0, m0 + q * nv, OUT_F32, q < 2 ? 0.5 : 1.0, up[L], tw, ti, nz, tmp2,
                           dense(g.nx), s);
         }
-        launch_cp_init(g, m0, c.cp.as<float>(), s);
+        if (!cp_start) launch_cp_init(g, m0, c.cp.as<float>(), s);
         const int frec = prm->forced_records ? prm->forced_records[L] : -1;
         const int fupd = prm->forced_updates ? prm->forced_updates[L] : -1;
         launch_ctl_reset(c.ctl.as<Ctl>(), iter, frec, fupd, prm->tol, prm->smoothPenalty * p * p,
@@ -525,6 +530,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         b.counts = c.counts.as<int>() + L * 3;
         b.mdpart = c.mdpart.as<float>();
         b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
+        b.cp_start = cp_start ? 1 : 0;
         for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, k, prm->stop_kernels != 0, s));
         c.stats.lk_launches[L] = iter;
         c.stats.lk_blocks[L] = static_cast<int>(t.blocks);
@@ -1034,6 +1040,18 @@ int wr_net_input(wr_ctx* ctx, const float* mov, const float* ref, long long n, v
     WR_TRY(harness_small(c, &part, &ms));
     launch_mean_std(ref, n, part, ms, s);
     launch_net_input(mov, ref, n, ms, static_cast<uint16_t*>(out), s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int q, int ncx, int ncy, int r,
+                     int level, double scale, float* cp_out, void* stream) {
+    WR_ARG(ctx && p && cp_out, "wr_net_output_cp: null argument");
+    WR_ARG(cp >= 3 && nz > 0 && h > 0 && w > 0 && ncx > 0 && ncy > 0 && r >= 0, "wr_net_output_cp: bad sizes");
+    WR_ARG(q >= 0 && q < 16 && level >= 0 && level < WR_MAX_LEVELS, "wr_net_output_cp: bad level");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_net_output_cp(static_cast<const uint16_t*>(p), cp, h, w, q, nz, ncx, ncy, r, level,
+                         static_cast<float>(scale), cp_out, static_cast<hipStream_t>(stream));
     WR_HIP_CHECK(hipGetLastError());
     return WR_OK;
 }

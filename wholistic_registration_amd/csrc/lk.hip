@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
                                                     const float* __restrict__ cp, int nx, int ny, int nz, int r,
                                                     int p, int ncx, int ncy, int xblocks, float* __restrict__ soa,
                                                     float* __restrict__ aos, float* __restrict__ coords,
-                                                    int* __restrict__ counts) {
+                                                    int* __restrict__ counts, int cp_start) {
     const unsigned b = blockIdx.x;
     const unsigned row = b / static_cast<unsigned>(xblocks);
     const int x = static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x);
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
     const size_t ncp = static_cast<size_t>(ncx) * ncy * nz;
     const size_t v = static_cast<size_t>(row) * nx + x;
     float m[3];
-    if (c->upd > 0) {
+    if (c->upd > 0 || cp_start) {   // the CP interpolant (a level without updates keeps its start)
         const Axis1 ax = cp_axis(x, r, p, ncx), ay = cp_axis(y, r, p, ncy);
         const float* base = cp + static_cast<size_t>(z) * ncy * ncx;
         for (int q = 0; q < 3; ++q) m[q] = cp_bilinear(base + q * ncp, ncx, ax, ay);
@@ -315,18 +315,20 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     const long long nerr = t.blocks;
     const long long npen = (ncp + 63) / 64;                  // upper bound of the LK launch's |D|^2 partials
     const int nd = dense_blocks(g);
+    // the level's first iteration reads the dense start field unless it starts from the CP field
+    const bool dense = k == 0 && !b.cp_start;
     // maxima behind the |delta motion| test of update j
-    auto parts_of = [&](int j) { return static_cast<long long>(cpb) + (j == 0 ? nd : 0); };
+    auto parts_of = [&](int j) { return static_cast<long long>(cpb) + (j == 0 && !b.cp_start ? nd : 0); };
     auto fuse_check = [&](int j) { return !separate_stop && (t.blocks + npen) * parts_of(j) <= (1LL << 22); };
     const bool fuse_err = !separate_stop && static_cast<long long>(cpb) * (nerr + npen) <= (1LL << 22);
     const int check_now = (k > 0 && fuse_check(k - 1)) ? static_cast<int>(parts_of(k - 1)) : 0;
     if (b.events) (void)hipEventRecord(b.events[2 * k], s);
-    const long long np = launch_lk_pass(g, t, b, k, check_now, s);
+    const long long np = launch_lk_pass(g, t, b, k, dense, check_now, s);
     if (b.events) (void)hipEventRecord(b.events[2 * k + 1], s);
     if (!fuse_err) k_err_finish<<<1, 1024, 0, s>>>(b.ctl, k, b.errpart, nerr, b.penpart, np, b.log, b.counts);
     k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d, b.mdpart, k,
                                 fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts);
-    if (k == 0)
+    if (dense)
         k_dense_maxdiff<<<nd, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, b.mdpart + cpb);
     if (!fuse_check(k))
         k_check<<<1, 1024, 0, s>>>(b.ctl, k, b.mdpart, static_cast<int>(parts_of(k)), b.log, b.counts);
@@ -340,7 +342,7 @@ void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int ite
     const int xblocks = (g.nx + 255) / 256;
     const unsigned blocks = static_cast<unsigned>(static_cast<long long>(xblocks) * g.ny * g.nz);
     k_level_end<<<blocks, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, xblocks,
-                                       dense_soa, motion_aos, coords_aos, b.counts);
+                                       dense_soa, motion_aos, coords_aos, b.counts, b.cp_start);
 }
 
 }  // namespace wr

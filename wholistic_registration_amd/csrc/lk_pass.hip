@@ -539,8 +539,8 @@ PVol make_pvol(const void* base, unsigned es, int nx, int ny, int nz) {
 
 }  // namespace
 
-long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, int check_parts,
-                         hipStream_t s) {
+long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool dense,
+                         int check_parts, hipStream_t s) {
     PassArgs a;
     a.mov = make_pvol(b.mov, static_cast<unsigned>(store_bytes(b.store)), g.nx, g.ny, g.nz);
     a.mmov = make_pvol(b.mmov ? static_cast<const void*>(b.mmov) : b.mov, 4u, g.nx, g.ny, g.nz);
@@ -586,7 +586,7 @@ long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b
      k_lk_pass<true, true, false, S>,   k_lk_pass<true, true, true, S>}
     static const Fn table[3][8] = {WR_LK_TABLE(ST_F32), WR_LK_TABLE(ST_BF16), WR_LK_TABLE(ST_F16)};
 #undef WR_LK_TABLE
-    const int sel = (k == 0 ? 4 : 0) | (mr ? 2 : 0) | (mm ? 1 : 0);
+    const int sel = (dense ? 4 : 0) | (mr ? 2 : 0) | (mm ? 1 : 0);
     hipLaunchKernelGGL(table[b.store][sel], grid, dim3(threads), 0, s, a);
     return a.npen;
 }

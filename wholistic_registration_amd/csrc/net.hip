@@ -422,6 +422,43 @@ __global__ void k_net_output(const uint16_t* __restrict__ P, int cp, long long n
     field[i] = bf2f(P[static_cast<size_t>(v) * cp + c]) * scale;
 }
 
+// The network's field at the control points of pyramid level L: CP (i, j)
+// of plane z sits at level voxel (r + p i, r + p j), i.e. full-resolution
+// X = (x_L + 0.5) 2^L - 0.5, and at (X + 0.5) / 2^q - 0.5 in the network's
+// level-q grid (q = log2 of full / network size); bilinear there with edge
+// clamping, times scale, in level-L units (x, y divided by 2^L).  out: 3
+// planes of nz * ncy * ncx.  A thread per control point.
+__global__ void k_net_output_cp(const uint16_t* __restrict__ P, int cp, int h3, int w3, float inv_q, int nz,
+                                int ncx, int ncy, int r, int pstep, float lscale, float scale,
+                                float* __restrict__ out) {
+    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    const long long ncp = static_cast<long long>(ncx) * ncy * nz;
+    if (i >= ncp) return;
+    const int ix = static_cast<int>(i % ncx);
+    const int iy = static_cast<int>((i / ncx) % ncy);
+    const int z = static_cast<int>(i / (static_cast<long long>(ncx) * ncy));
+    // full-resolution position of the CP, then its coordinate in the network's grid
+    const float u = fminf(fmaxf((static_cast<float>(r + pstep * ix) + 0.5f) * lscale * inv_q - 0.5f, 0.0f),
+                          static_cast<float>(w3 - 1));
+    const float v = fminf(fmaxf((static_cast<float>(r + pstep * iy) + 0.5f) * lscale * inv_q - 0.5f, 0.0f),
+                          static_cast<float>(h3 - 1));
+    const int u0 = static_cast<int>(floorf(u)), v0 = static_cast<int>(floorf(v));
+    const int u1 = min(u0 + 1, w3 - 1), v1 = min(v0 + 1, h3 - 1);
+    const float tu = u - static_cast<float>(u0), tv = v - static_cast<float>(v0);
+    const size_t pz = static_cast<size_t>(z) * h3;
+    const uint16_t* p00 = P + ((pz + v0) * w3 + u0) * cp;
+    const uint16_t* p01 = P + ((pz + v0) * w3 + u1) * cp;
+    const uint16_t* p10 = P + ((pz + v1) * w3 + u0) * cp;
+    const uint16_t* p11 = P + ((pz + v1) * w3 + u1) * cp;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float a = bf2f(p00[c]) + tu * (bf2f(p01[c]) - bf2f(p00[c]));
+        const float b = bf2f(p10[c]) + tu * (bf2f(p11[c]) - bf2f(p10[c]));
+        const float val = (a + tv * (b - a)) * scale;
+        out[c * ncp + i] = c < 2 ? val / lscale : val;
+    }
+}
+
 inline unsigned blocks_of(long long n, int per = 256) { return static_cast<unsigned>((n + per - 1) / per); }
 
 }  // namespace
@@ -494,6 +531,14 @@ void launch_to_bf16(const float* in, long long n, uint16_t* out, hipStream_t s) 
 void launch_net_input(const float* mov, const float* ref, long long n, const float* ms, uint16_t* out,
                       hipStream_t s) {
     hipLaunchKernelGGL(k_net_input, dim3(blocks_of(n)), dim3(256), 0, s, mov, ref, n, ms, out);
+}
+
+void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int q, int nz, int ncx, int ncy, int r,
+                          int level, float scale, float* out, hipStream_t s) {
+    const long long ncp = static_cast<long long>(ncx) * ncy * nz;
+    hipLaunchKernelGGL(k_net_output_cp, dim3(blocks_of(ncp)), dim3(256), 0, s, p, cp, h3, w3,
+                       1.0f / static_cast<float>(1 << q), nz, ncx, ncy, r, 2 * r + 1,
+                       static_cast<float>(1 << level), scale, out);
 }
 
 void launch_net_output(const uint16_t* p, int cp, long long n, float scale, float* field, hipStream_t s) {

@@ -204,17 +204,20 @@ struct LkBuffers {
     int* counts;             // [level][3]: rec, upd, nan
     hipEvent_t* events;      // null, or 2*iter events of this level bracketing the LK passes
     float* mdpart;           // per-workgroup max |delta motion| partials
+    int cp_start = 0;        // 1: the level starts from the CP field itself (no dense field m0):
+                             // iteration 0 is a CP pass and its |delta motion| test uses CP maxima
 };
 
 // Partial-maximum slots needed by one iteration with `ncp` control points.
 size_t mdpart_slots(long long ncp);
 
 // The fused per-voxel pass of iteration k plus the neighbour differences of
-// the CP field (lk_pass.hip); with check_parts > 0 every workgroup first
+// the CP field (lk_pass.hip), reading the motion from the dense field m0
+// (dense) or the CP interpolant; with check_parts > 0 every workgroup first
 // applies the |delta motion| test of update k - 1 to that many maxima in
 // b.mdpart (workgroup 0 records it).  Returns the number of |D|^2 partials.
-long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, int check_parts,
-                         hipStream_t s);
+long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool dense,
+                         int check_parts, hipStream_t s);
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
                       double inv_n, hipStream_t s);
@@ -359,6 +362,11 @@ void launch_net_input(const float* mov, const float* ref, long long n, const flo
                       hipStream_t s);
 // field (n, 3) float32 = scale * p[:, 0:3] of p (n, cp) bfloat16.
 void launch_net_output(const uint16_t* p, int cp, long long n, float scale, float* field, hipStream_t s);
+// The field of p (nz, h3, w3, cp) (level q of the full volume) at the control
+// points (ncx x ncy per plane, half-width r) of pyramid level `level`, in that
+// level's units: out 3 planes of nz * ncy * ncx float32.
+void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int q, int nz, int ncx, int ncy, int r,
+                          int level, float scale, float* out, hipStream_t s);
 
 // ---------------------------------------------------------------- context
 

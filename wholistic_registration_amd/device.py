@@ -94,7 +94,8 @@ PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
 def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
                schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
-               stop_kernels=False, precision="fp32", reuse_template=False, context=None, stream=None):
+               stop_kernels=False, precision="fp32", reuse_template=False, context=None, stream=None,
+               motion_init_cp=None):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
@@ -114,6 +115,9 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     rule), as ``autotune.tune_tiles`` picks them.  ``motion_init`` may have
     another x-y size than the volume (full-resolution voxel units; the
     reference's warm start resamples it to the coarsest level).
+    ``motion_init_cp`` (instead of ``motion_init``): the coarsest level's
+    control-point field itself, (3, Z, ncy, ncx) float32 in that level's
+    voxel units (``cp_grid``); the level starts from its interpolant.
     """
     torch = _torch()
     _check_vol(mov, "mov")
@@ -126,6 +130,13 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
             raise ValueError(f"{name} must be float32 with the volume's shape")
     _check_vol(motion_init, "motion_init", 4, 3)
     nz, ny, nx = mov.shape
+    if motion_init_cp is not None:
+        if motion_init is not None:
+            raise ValueError("give motion_init or motion_init_cp, not both")
+        want = (3, nz) + cp_grid((nz, ny, nx), r, layer)[::-1]
+        if tuple(motion_init_cp.shape) != want or motion_init_cp.dtype != torch.float32 or \
+                not motion_init_cp.is_contiguous():
+            raise ValueError(f"motion_init_cp must be contiguous float32 {want}")
     if motion_init is not None and (motion_init.shape[0] != nz or motion_init.dtype != torch.float32):
         raise ValueError("motion_init must be float32 (Z, Y', X', 3) with the volume's Z")
     dev = mov.device if device is None else device
@@ -151,7 +162,10 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
     prm.precision = PRECISIONS[precision]
     prm.reuse_template = 1 if reuse_template else 0
-    if motion_init is not None and tuple(motion_init.shape[1:3]) != (ny, nx):
+    if motion_init_cp is not None:
+        prm.init_cp = 1
+        motion_init = motion_init_cp
+    elif motion_init is not None and tuple(motion_init.shape[1:3]) != (ny, nx):
         # a warm start of another x-y size (full-resolution units)
         prm.init_ny, prm.init_nx = int(motion_init.shape[1]), int(motion_init.shape[2])
     prm.timing = 1 if timing else 0
@@ -185,6 +199,14 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
         nans=[log.nans[L] for L in range(nlev)],
         diff_error=bufs["diff"], penalty_error=bufs["pen"], current_error=bufs["cur"],
         max_diff_motion=bufs["md"], final_error=float(log.final_error), layer=layer)
+
+
+def cp_grid(shape_zyx, r, level=0):
+    """(ncx, ncy): control points per row and column of pyramid level
+    ``level`` (calFlow3d_Wei_v1.py:434-436: range(r, n_L - 1, 2r + 1))."""
+    _, ny, nx = shape_zyx
+    lx, ly = int(nx / 2 ** level), int(ny / 2 ** level)
+    return len(range(r, lx - 1, 2 * r + 1)), len(range(r, ly - 1, 2 * r + 1))
 
 
 def stats(device=0, context=None):

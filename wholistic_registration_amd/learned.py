@@ -284,6 +284,24 @@ class InitNet:
         self._check(self.lib.wr_net_output(self.ctx, _p(out), 16, n, PSCALE, _p(field), self._s()), "wr_net_output")
         return field
 
+    def predict_cp(self, mov_l, ref_l, full_shape, r, level=0, out=None):
+        """The predicted field at the control points of pyramid level
+        ``level`` of a ``full_shape`` (Z, Y, X) volume, in that level's units:
+        (3, Z, ncy, ncx) float32, the ``motion_init_cp`` of getMotion."""
+        torch = _torch()
+        d, h, w = mov_l.shape
+        nz, ny, nx = full_shape
+        q = int(round(np.log2(nx / w)))
+        if d != nz or nx != w * 2 ** q or ny != h * 2 ** q:
+            raise ValueError("level images must be the full volume's level q (x, y / 2^q, same z)")
+        ncx, ncy = _dev.cp_grid(full_shape, r, level)
+        p = self.forward(net_input(mov_l, ref_l, context=self.ctx))
+        if out is None:
+            out = torch.empty((3, nz, ncy, ncx), dtype=torch.float32, device=self.dev)
+        self._check(self.lib.wr_net_output_cp(self.ctx, _p(p), 16, d, h, w, q, ncx, ncy, int(r), int(level), PSCALE,
+                                              _p(out), self._s()), "wr_net_output_cp")
+        return out
+
     # ---------------------------------------------------------------- storage
 
     def params(self):
@@ -341,7 +359,7 @@ def level_images(mov, ref, level=LEVEL, ref_level=None, context=None, stream=Non
 
 
 def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol, fallback=None,
-                     plain_layer=4, plain_iter=15, ref_level=None, context=None, stream=None, **kw):
+                     plain_layer=4, plain_iter=15, ref_level=None, context=None, stream=None, start="cp", **kw):
     """getMotion warm-started by the network's field.
 
     Returns ``(result, info)``: ``result`` is a ``device.MotionResult``;
@@ -352,14 +370,23 @@ def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothP
     network's targets were made with) also runs and its field is kept when
     the learned run's final residual (``currentError`` of level 0) is higher.  ``context`` / ``stream``: the
     library context (``net`` must run on it: ``net.fork(context)``) and torch
-    stream to run on (default: the shared context, torch's current stream)."""
+    stream to run on (default: the shared context, torch's current stream).
+    ``start``: ``"cp"`` hands the network's field to getMotion as the
+    coarsest level's control-point field (``motion_init_cp``: no dense start
+    field is formed); ``"dense"`` as the reference's warm start
+    ``option["motion"]`` at the network's x-y size."""
     torch = _torch()
     params = dict(layer=layer, r=r, zRatio=zRatio, movRange=movRange, smoothPenalty=smoothPenalty, tol=tol,
                   context=context, stream=stream, **kw)
+    if start not in ("cp", "dense"):
+        raise ValueError("start must be 'cp' or 'dense'")
     with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(mov.device)):
         m_l, r_l = level_images(mov, ref, ref_level=ref_level, context=context, stream=stream)
-        init = net.predict(m_l, r_l)
-    res = _dev.get_motion(mov, ref, iter=iter, motion_init=init, **params)
+        if start == "cp":
+            init = {"motion_init_cp": net.predict_cp(m_l, r_l, tuple(mov.shape), r, level=layer)}
+        else:
+            init = {"motion_init": net.predict(m_l, r_l)}
+    res = _dev.get_motion(mov, ref, iter=iter, **init, **params)
     info = {"used": "learned", "learned_error": res.final_error}
     if fallback == "plain":
         kw_plain = {k: v for k, v in params.items() if k not in ("motion_out", "layer", "reuse_template")}
