@@ -196,6 +196,7 @@ def main(argv=None):
     ap.add_argument("--target", default="lk", choices=("lk", "truth"))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--recipe", default="frame_64x512x1024", choices=sorted(RECIPES))
+    ap.add_argument("--init", default=None, help="safetensors weights to start from (fine-tuning)")
     ap.add_argument("--report", default=None, help="JSON file for the loss curve and validation")
     args = ap.parse_args(argv)
     torch = _torch()
@@ -208,7 +209,7 @@ def main(argv=None):
     pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log, args.recipe)
     val = PairSet(dev, args.val_templates, 4, 20000, args.target, log, args.recipe)
     t1 = time.time()
-    net = Lr.InitNet(seed=args.seed, lr=args.lr)
+    net = Lr.InitNet.load(args.init) if args.init else Lr.InitNet(seed=args.seed, lr=args.lr)
     torch.cuda.synchronize()
     t2 = time.time()
     curve = train(net, pairs, args.steps, args.batch, args.lr, log, warmup=args.warmup)
@@ -216,7 +217,7 @@ def main(argv=None):
     t3 = time.time()
     v_loss, v_zero = evaluate(net, val)
     net.save(args.out)
-    rep = {"recipe": args.recipe, "pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target,
+    rep = {"recipe": args.recipe, "init": args.init, "pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target,
            "steps": args.steps,
            "batch": args.batch, "lr": args.lr, "data_s": t1 - t0, "train_s": t3 - t2,
            "ms_per_step": 1e3 * (t3 - t2) / args.steps, "curve": curve,
