@@ -1,7 +1,7 @@
 """What the trained network gains over the method without it, on held-out
-64 x 512 x 1024 uint16 frame pairs (template seed 90000 and moving frames
-90001..; training used template seeds 10000 + 1000 i, i < 32, with frames
-+1..+16, and validation 20000, 21000).
+64 x 512 x 1024 uint16 frame pairs (template seeds 90000 and 92000 and their
+moving frames; training used template seeds 10000 + 1000 i, i < 64, with
+frames +1..+16, and validation 20000, 21000).
 
 The plain pyramid LK (layer 4, r 8, 15 iterations, float32) is the yardstick:
 endpoint errors are measured against its field.  The network's field starts
@@ -41,12 +41,21 @@ def _epe(a, b):
     return float(torch.sqrt(((a - b) ** 2).sum(-1)).mean())
 
 
-@pytest.mark.parametrize("seed", SEEDS)
-def test_learned_start_beats_method_without_it(setup, seed):
+def _other_template(seed, cuda):
     from wholistic_registration_amd import device as D
     from wholistic_registration_amd import learned as Lr
     from wholistic_registration_amd import train_init as T
-    s = setup
+    clean, tmpl = T.make_template(seed, cuda)
+    params = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=T.penalty(tmpl), tol=1e-3)
+    return dict(clean=clean, tmpl=tmpl, params=params, ref_l=D.pyramid_level(tmpl, Lr.LEVEL))
+
+
+@pytest.mark.parametrize("template,seed", [(90000, SEEDS[0]), (90000, SEEDS[1]), (92000, 92001)])
+def test_learned_start_beats_method_without_it(setup, cuda, template, seed):
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    s = setup if template == 90000 else dict(setup, **_other_template(template, cuda))
     mov, _ = T.make_moving(s["clean"], seed, s["tmpl"].device)
     p = s["params"]
     plain_field = D.get_motion(mov, s["tmpl"], layer=4, iter=15, **p).motion.clone()
