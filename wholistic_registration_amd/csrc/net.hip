@@ -70,6 +70,9 @@ __device__ __forceinline__ unsigned tap_mask(int d, int h, int w, const NetGeom&
     return (d > 0 ? mhw : 0u) | (mhw << 9) | (d + 1 < g.d ? mhw << 18 : 0u);
 }
 
+// NT MFMA tiles of 16 voxels per wave (4 on large volumes, 1 on small ones
+// for 4x the waves).
+template <int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
                                                   const float* __restrict__ bias, uint16_t* __restrict__ Y,
                                                   NetGeom g, int cin, int cout, int act, float slope) {
@@ -82,14 +85,14 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long nv = static_cast<long long>(g.d) * g.h * g.w;
     const int co0 = blockIdx.y * 16;
-    const long long vb = static_cast<long long>(blockIdx.x) * 256 + wave * 64;
+    const long long vb = static_cast<long long>(blockIdx.x) * (64 * NT) + wave * (16 * NT);
     const uint16_t* xb = X + static_cast<size_t>(blockIdx.z) * nv * cin;
     const size_t yb = static_cast<size_t>(blockIdx.z) * nv * cout;
     const int ksz = 27 * cin;
-    int vo[4];            // element offset of the voxel's channel 0
-    unsigned vm[4];       // its in-volume taps (0: voxel past the end)
+    int vo[NT];           // element offset of the voxel's channel 0
+    unsigned vm[NT];      // its in-volume taps (0: voxel past the end)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NT; ++j) {
         const long long v = vb + 16 * j + (l & 15);
         const bool ok = v < nv;
         const long long vv = ok ? v : 0;
@@ -99,9 +102,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
         vo[j] = static_cast<int>(vv) * cin;
         vm[j] = ok ? tap_mask(d, h, w, g) : 0u;
     }
-    f4 acc[4];
+    f4 acc[NT];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < NT; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
     const uint16_t* wrow = Wt + static_cast<size_t>(co0 + (l & 15)) * ksz + 8 * (l >> 4);
     // this lane's reduction index k0 + 8 (l / 16) = tap * cin + ci, advanced by 32 per step;
     // two steps per trip so that the loads of both are in flight together
@@ -118,18 +121,18 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
         const bf16x8 a1 = kin1 ? load8(wrow + k0 + 32) : zero8();
         const int off0 = (kin0 ? toff[tap] : 0) + ci;
         const int off1 = (kin1 ? toff[tap1] : 0) + ci1;
-        bf16x8 b0[4], b1[4];
+        bf16x8 b0[NT], b1[NT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NT; ++j) {
             const bool in0 = kin0 && ((vm[j] >> (tap & 31)) & 1u);
             const bool in1 = kin1 && ((vm[j] >> (tap1 & 31)) & 1u);
             b0[j] = in0 ? load8(xb + vo[j] + off0) : zero8();
             b1[j] = in1 ? load8(xb + vo[j] + off1) : zero8();
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0[j], acc[j], 0, 0, 0);
+        for (int j = 0; j < NT; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0[j], acc[j], 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[j], acc[j], 0, 0, 0);
+        for (int j = 0; j < NT; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[j], acc[j], 0, 0, 0);
         tap = tap1;
         ci = ci1 + 32;
         while (ci >= cin) {
@@ -142,7 +145,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
 #pragma unroll
     for (int i = 0; i < 4; ++i) bs[i] = bias ? bias[m0 + i] : 0.0f;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NT; ++j) {
         const long long v = vb + 16 * j + (l & 15);
         if (v >= nv) continue;
         uint16_t o[4];
@@ -466,8 +469,15 @@ inline unsigned blocks_of(long long n, int per = 256) { return static_cast<unsig
 void launch_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, NetGeom g, int batch,
                      int cin, int cout, int act, float slope, hipStream_t s) {
     const long long nv = static_cast<long long>(g.d) * g.h * g.w;
-    dim3 grid(blocks_of(nv), cout / 16, batch);
-    hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, s, x, w, bias, y, g, cin, cout, act, slope);
+    // four tiles per wave while that still gives 512 workgroups (measured:
+    // 65k-voxel layers are faster with four tiles, 8k-voxel ones with one)
+    if (static_cast<long long>(blocks_of(nv, 256)) * (cout / 16) * batch >= 512) {
+        dim3 grid(blocks_of(nv, 256), cout / 16, batch);
+        hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, s, x, w, bias, y, g, cin, cout, act, slope);
+    } else {
+        dim3 grid(blocks_of(nv, 64), cout / 16, batch);
+        hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, x, w, bias, y, g, cin, cout, act, slope);
+    }
 }
 
 void launch_flip_weights(const uint16_t* w, int cin, int cout, uint16_t* wf, hipStream_t s) {
