@@ -353,15 +353,21 @@ int wr_net_to_bf16(wr_ctx* ctx, const float* in, long long n, void* out, void* s
 /* out (n, 8) = ((mov - mean) / std, (ref - mean) / std, 0, ...) with the
  * mean and standard deviation of ref's n float32 values. */
 int wr_net_input(wr_ctx* ctx, const float* mov, const float* ref, long long n, void* out, void* stream);
+/* The same input on a padded (dp, hp, wp) grid >= the level images' (d, h,
+ * w): voxels past the volume repeat its nearest edge voxel; the statistics
+ * are those of the d * h * w template voxels. */
+int wr_net_input_padded(wr_ctx* ctx, const float* mov, const float* ref, int d, int h, int w, int dp, int hp, int wp,
+                        void* out, void* stream);
 /* field (n, 3) float32 = scale * p[:, 0:3] of p (n, cp). */
 int wr_net_output(wr_ctx* ctx, const void* p, int cp, long long n, double scale, float* field, void* stream);
-/* The network's field (p: nz x h x w voxels of cp channels, on pyramid level
- * q of the full volume, scale * channels 0..2 in full-resolution units)
- * sampled bilinearly at the control points of pyramid level `level` (ncx x
- * ncy per plane, half-width r), in that level's units: cp_out is the 3-plane
- * field wr_params.init_cp takes. */
-int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int q, int ncx, int ncy, int r,
-                     int level, double scale, float* cp_out, void* stream);
+/* The network's field (p: nz planes of hs x ws voxels of cp channels, of
+ * which h x w is pyramid level q of the full volume and the rest edge
+ * padding; scale * channels 0..2 in full-resolution units) sampled
+ * bilinearly at the control points of pyramid level `level` (ncx x ncy per
+ * plane, half-width r), in that level's units: cp_out is the 3-plane field
+ * wr_params.init_cp takes. */
+int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int hs, int ws, int q, int ncx,
+                     int ncy, int r, int level, double scale, float* cp_out, void* stream);
 
 #ifdef __cplusplus
 }

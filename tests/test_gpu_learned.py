@@ -134,3 +134,32 @@ def test_learned_frames_on_concurrent_contexts_equal_single_runs(setup):
         single, _ = Lr.register_learned(f, s["tmpl"], s["net"], ref_level=s["ref_l"], **Lr.SCHEDULE, **s["params"])
         assert torch.equal(single.motion, m)
         assert single.records == res.records
+
+
+def test_learned_start_on_uneven_crops(setup):
+    """Crops whose level-3 image is not a multiple of the network's (4, 4, 32)
+    grid run on an edge-padded grid; crops on concurrent contexts equal their
+    single runs, and the learned start beats the zero start on a deep crop."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    from wholistic_registration_amd.batch import FrameBatch
+    s = setup
+    mov, _ = T.make_moving(s["clean"], SEEDS[0], s["tmpl"].device)
+    boxes = [(slice(4, 60), slice(40, 424), slice(100, 388)),      # 56 x 384 x 288 (level 3: 56 x 48 x 36)
+             (slice(0, 40), slice(300, 500), slice(17, 377))]      # 40 x 200 x 360 (level 3: 40 x 25 x 45)
+    pairs = [(mov[b].contiguous(), s["tmpl"][b].contiguous()) for b in boxes]
+    p = s["params"]
+    with FrameBatch(workers=2) as fb:
+        results, _ = fb.register_crops(pairs, net=s["net"], iter=Lr.SCHEDULE["iter"], **p)
+    for (m, t), res in zip(pairs, results):
+        single, _ = Lr.register_learned(m, t, s["net"], **Lr.SCHEDULE, **p)
+        assert torch.equal(single.motion, res.motion)
+    m, t = pairs[0]
+    plain = D.get_motion(m, t, layer=4, iter=15, **p).motion
+    zero = D.get_motion(m, t, **Lr.SCHEDULE, **p).motion
+    e_learned = float(torch.sqrt(((results[0].motion - plain) ** 2).sum(-1)).mean())
+    e_zero = float(torch.sqrt(((zero - plain) ** 2).sum(-1)).mean())
+    assert e_learned < e_zero, (e_learned, e_zero)

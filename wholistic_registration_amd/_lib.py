@@ -110,8 +110,9 @@ _SIGNATURES = {
                      ctypes.c_double, ctypes.c_double, _I, _VP], _I),
     "wr_net_to_bf16": ([_VP, _VP, ctypes.c_longlong, _VP, _VP], _I),
     "wr_net_input": ([_VP, _VP, _VP, ctypes.c_longlong, _VP, _VP], _I),
+    "wr_net_input_padded": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_net_output": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, _VP, _VP], _I),
-    "wr_net_output_cp": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double, _VP, _VP], _I),
+    "wr_net_output_cp": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double, _VP, _VP], _I),
 }
 
 _lib = None

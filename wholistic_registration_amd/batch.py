@@ -147,7 +147,7 @@ class FrameBatch:
             raise errors[0]
         return motion_out, results
 
-    def register_crops(self, pairs, layer=None, warp=False, **params):
+    def register_crops(self, pairs, layer=None, warp=False, net=None, **params):
         """Register crops of uneven sizes: ``pairs`` is a sequence of ``(mov,
         ref)`` or ``(mov, ref, mask_ref)`` device volumes, each pair of one
         shape.  The crops are spread over the worker contexts, largest first,
@@ -157,7 +157,10 @@ class FrameBatch:
         geometry runs with the tile widths tuned on its first crop.  Returns
         ``(results, warped)``: ``results[i]`` is crop i's
         ``device.MotionResult`` and ``warped[i]`` its ``correctMotion`` output
-        (None unless ``warp``)."""
+        (None unless ``warp``).  With ``net`` (a ``learned.InitNet``) every
+        crop starts from the network's field at full resolution
+        (``learned.register_learned``, ``layer`` 0 unless given; pass
+        ``learned.SCHEDULE``'s iter), each worker on its own fork."""
         torch = _torch()
         pairs = [tuple(p) for p in pairs]
         n = len(pairs)
@@ -167,7 +170,13 @@ class FrameBatch:
             if tuple(p[0].shape) != tuple(p[1].shape):
                 raise ValueError("each crop's mov and ref must share one shape")
         r = params["r"]
-        layers = [max_layer(p[0].shape, r) if layer is None else int(layer) for p in pairs]
+        forks = None
+        if net is not None:
+            from . import learned as _learned
+            forks = [net.fork(ctx) for ctx in self.contexts]
+            layers = [0 if layer is None else int(layer) for _ in pairs]
+        else:
+            layers = [max_layer(p[0].shape, r) if layer is None else int(layer) for p in pairs]
         w = len(self.contexts)
         load, owner = [0] * w, [[] for _ in range(w)]
         for i in sorted(range(n), key=lambda i: -pairs[i][0].numel()):
@@ -189,7 +198,11 @@ class FrameBatch:
                         if self.tiles is not None:
                             extra["tile_patches_x"] = self.tiles.tiles(mov, ref, layer=layers[i], context=ctx,
                                                                        stream=strm, **extra)
-                        results[i] = _dev.get_motion(mov, ref, layer=layers[i], context=ctx, stream=strm, **extra)
+                        if forks is not None:
+                            results[i], _ = _learned.register_learned(mov, ref, forks[k], layer=layers[i], context=ctx,
+                                                                      stream=strm, **extra)
+                        else:
+                            results[i] = _dev.get_motion(mov, ref, layer=layers[i], context=ctx, stream=strm, **extra)
                         if warp:
                             warped[i] = _dev.correct_motion(mov, results[i].motion, context=ctx, stream=strm)
                     strm.synchronize()

@@ -1044,14 +1044,32 @@ int wr_net_input(wr_ctx* ctx, const float* mov, const float* ref, long long n, v
     return WR_OK;
 }
 
-int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int q, int ncx, int ncy, int r,
-                     int level, double scale, float* cp_out, void* stream) {
+int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int hs, int ws, int q, int ncx,
+                     int ncy, int r, int level, double scale, float* cp_out, void* stream) {
     WR_ARG(ctx && p && cp_out, "wr_net_output_cp: null argument");
-    WR_ARG(cp >= 3 && nz > 0 && h > 0 && w > 0 && ncx > 0 && ncy > 0 && r >= 0, "wr_net_output_cp: bad sizes");
+    WR_ARG(cp >= 3 && nz > 0 && h > 0 && w > 0 && hs >= h && ws >= w && ncx > 0 && ncy > 0 && r >= 0,
+           "wr_net_output_cp: bad sizes");
     WR_ARG(q >= 0 && q < 16 && level >= 0 && level < WR_MAX_LEVELS, "wr_net_output_cp: bad level");
     WR_HIP_CHECK(hipSetDevice(ctx->c.device));
-    launch_net_output_cp(static_cast<const uint16_t*>(p), cp, h, w, q, nz, ncx, ncy, r, level,
+    launch_net_output_cp(static_cast<const uint16_t*>(p), cp, h, w, hs, ws, q, nz, ncx, ncy, r, level,
                          static_cast<float>(scale), cp_out, static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_input_padded(wr_ctx* ctx, const float* mov, const float* ref, int d, int h, int w, int dp, int hp, int wp,
+                        void* out, void* stream) {
+    WR_ARG(ctx && mov && ref && out, "wr_net_input_padded: null argument");
+    WR_ARG(d > 0 && h > 0 && w > 0 && dp >= d && hp >= h && wp >= w, "wr_net_input_padded: bad sizes");
+    WR_ARG(al16(out), "wr_net_input_padded: out must be 16-byte aligned");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* part;
+    float* ms;
+    WR_TRY(harness_small(c, &part, &ms));
+    launch_mean_std(ref, static_cast<long long>(d) * h * w, part, ms, s);
+    launch_net_input_padded(mov, ref, d, h, w, dp, hp, wp, ms, static_cast<uint16_t*>(out), s);
     WR_HIP_CHECK(hipGetLastError());
     return WR_OK;
 }

@@ -415,6 +415,26 @@ __global__ void k_net_input(const float* __restrict__ mov, const float* __restri
     reinterpret_cast<uint4*>(out)[v] = o.q;
 }
 
+// The same on a padded grid (dp, hp, wp) >= (d, h, w): voxels past the
+// volume repeat its nearest edge voxel (a thread per padded voxel).
+__global__ void k_net_input_padded(const float* __restrict__ mov, const float* __restrict__ ref, int d, int h,
+                                   int w, int dp, int hp, int wp, const float* __restrict__ ms,
+                                   uint16_t* __restrict__ out) {
+    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    const long long n = static_cast<long long>(dp) * hp * wp;
+    if (i >= n) return;
+    const int x = min(static_cast<int>(i % wp), w - 1);
+    const int y = min(static_cast<int>((i / wp) % hp), h - 1);
+    const int z = min(static_cast<int>(i / (static_cast<long long>(wp) * hp)), d - 1);
+    const size_t v = (static_cast<size_t>(z) * h + y) * w + x;
+    const float mu = ms[0], inv = ms[1] > 0.0f ? 1.0f / ms[1] : 1.0f;
+    Pack8 o;
+    o.q = make_uint4(0u, 0u, 0u, 0u);
+    o.h[0] = f2bf((mov[v] - mu) * inv);
+    o.h[1] = f2bf((ref[v] - mu) * inv);
+    reinterpret_cast<uint4*>(out)[i] = o.q;
+}
+
 // field[v][c] = scale * P[v][c], c < 3 (float32, the warm-start field)
 __global__ void k_net_output(const uint16_t* __restrict__ P, int cp, long long n, float scale,
                              float* __restrict__ field) {
@@ -425,14 +445,16 @@ __global__ void k_net_output(const uint16_t* __restrict__ P, int cp, long long n
     field[i] = bf2f(P[static_cast<size_t>(v) * cp + c]) * scale;
 }
 
-// The network's field at the control points of pyramid level L: CP (i, j)
-// of plane z sits at level voxel (r + p i, r + p j), i.e. full-resolution
+// The network's field at the control points of pyramid level L (P stored on
+// an hs x ws grid per plane, of which h3 x w3 is the volume's level q, the
+// rest edge padding): CP (i, j) of plane z sits at level voxel
+// (r + p i, r + p j), i.e. full-resolution
 // X = (x_L + 0.5) 2^L - 0.5, and at (X + 0.5) / 2^q - 0.5 in the network's
 // level-q grid (q = log2 of full / network size); bilinear there with edge
 // clamping, times scale, in level-L units (x, y divided by 2^L).  out: 3
 // planes of nz * ncy * ncx.  A thread per control point.
-__global__ void k_net_output_cp(const uint16_t* __restrict__ P, int cp, int h3, int w3, float inv_q, int nz,
-                                int ncx, int ncy, int r, int pstep, float lscale, float scale,
+__global__ void k_net_output_cp(const uint16_t* __restrict__ P, int cp, int h3, int w3, int hs, int ws, float inv_q,
+                                int nz, int ncx, int ncy, int r, int pstep, float lscale, float scale,
                                 float* __restrict__ out) {
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     const long long ncp = static_cast<long long>(ncx) * ncy * nz;
@@ -448,11 +470,11 @@ __global__ void k_net_output_cp(const uint16_t* __restrict__ P, int cp, int h3, 
     const int u0 = static_cast<int>(floorf(u)), v0 = static_cast<int>(floorf(v));
     const int u1 = min(u0 + 1, w3 - 1), v1 = min(v0 + 1, h3 - 1);
     const float tu = u - static_cast<float>(u0), tv = v - static_cast<float>(v0);
-    const size_t pz = static_cast<size_t>(z) * h3;
-    const uint16_t* p00 = P + ((pz + v0) * w3 + u0) * cp;
-    const uint16_t* p01 = P + ((pz + v0) * w3 + u1) * cp;
-    const uint16_t* p10 = P + ((pz + v1) * w3 + u0) * cp;
-    const uint16_t* p11 = P + ((pz + v1) * w3 + u1) * cp;
+    const size_t pz = static_cast<size_t>(z) * hs;      // stored grid hs x ws >= h3 x w3
+    const uint16_t* p00 = P + ((pz + v0) * ws + u0) * cp;
+    const uint16_t* p01 = P + ((pz + v0) * ws + u1) * cp;
+    const uint16_t* p10 = P + ((pz + v1) * ws + u0) * cp;
+    const uint16_t* p11 = P + ((pz + v1) * ws + u1) * cp;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float a = bf2f(p00[c]) + tu * (bf2f(p01[c]) - bf2f(p00[c]));
@@ -543,12 +565,18 @@ void launch_net_input(const float* mov, const float* ref, long long n, const flo
     hipLaunchKernelGGL(k_net_input, dim3(blocks_of(n)), dim3(256), 0, s, mov, ref, n, ms, out);
 }
 
-void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int q, int nz, int ncx, int ncy, int r,
-                          int level, float scale, float* out, hipStream_t s) {
+void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int hs, int ws, int q, int nz, int ncx, int ncy,
+                          int r, int level, float scale, float* out, hipStream_t s) {
     const long long ncp = static_cast<long long>(ncx) * ncy * nz;
-    hipLaunchKernelGGL(k_net_output_cp, dim3(blocks_of(ncp)), dim3(256), 0, s, p, cp, h3, w3,
+    hipLaunchKernelGGL(k_net_output_cp, dim3(blocks_of(ncp)), dim3(256), 0, s, p, cp, h3, w3, hs, ws,
                        1.0f / static_cast<float>(1 << q), nz, ncx, ncy, r, 2 * r + 1,
                        static_cast<float>(1 << level), scale, out);
+}
+
+void launch_net_input_padded(const float* mov, const float* ref, int d, int h, int w, int dp, int hp, int wp,
+                             const float* ms, uint16_t* out, hipStream_t s) {
+    const long long n = static_cast<long long>(dp) * hp * wp;
+    hipLaunchKernelGGL(k_net_input_padded, dim3(blocks_of(n)), dim3(256), 0, s, mov, ref, d, h, w, dp, hp, wp, ms, out);
 }
 
 void launch_net_output(const uint16_t* p, int cp, long long n, float scale, float* field, hipStream_t s) {

@@ -291,15 +291,22 @@ class InitNet:
         torch = _torch()
         d, h, w = mov_l.shape
         nz, ny, nx = full_shape
-        q = int(round(np.log2(nx / w)))
-        if d != nz or nx != w * 2 ** q or ny != h * 2 ** q:
-            raise ValueError("level images must be the full volume's level q (x, y / 2^q, same z)")
+        if d != nz or int(nx / 2 ** LEVEL) != w or int(ny / 2 ** LEVEL) != h:
+            raise ValueError(f"level images must be the full volume's level {LEVEL} (x, y / 2^{LEVEL}, same z)")
+        if mov_l.shape != ref_l.shape or mov_l.dtype != torch.float32 or ref_l.dtype != torch.float32:
+            raise ValueError("level images must be float32 volumes of one shape")
         ncx, ncy = _dev.cp_grid(full_shape, r, level)
-        p = self.forward(net_input(mov_l, ref_l, context=self.ctx))
+        # the network's grid: edge-padded up to multiples of (4, 4, 32)
+        dp, hp, wp = -(-d // 4) * 4, -(-h // 4) * 4, -(-w // 32) * 32
+        x = torch.empty((1, dp, hp, wp, 8), dtype=torch.bfloat16, device=self.dev)
+        mov_c, ref_c = mov_l.contiguous(), ref_l.contiguous()
+        self._check(self.lib.wr_net_input_padded(self.ctx, _p(mov_c), _p(ref_c), d, h, w, dp, hp, wp, _p(x),
+                                                 self._s()), "wr_net_input_padded")
+        p = self.forward(x)
         if out is None:
             out = torch.empty((3, nz, ncy, ncx), dtype=torch.float32, device=self.dev)
-        self._check(self.lib.wr_net_output_cp(self.ctx, _p(p), 16, d, h, w, q, ncx, ncy, int(r), int(level), PSCALE,
-                                              _p(out), self._s()), "wr_net_output_cp")
+        self._check(self.lib.wr_net_output_cp(self.ctx, _p(p), 16, d, h, w, hp, wp, LEVEL, ncx, ncy, int(r),
+                                              int(level), PSCALE, _p(out), self._s()), "wr_net_output_cp")
         return out
 
     # ---------------------------------------------------------------- storage
