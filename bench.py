@@ -466,7 +466,10 @@ def crops_measure(n_crops=16, seed=3, reps=3):
     getMotion + correctMotion per crop.  Total time of all crops: on one
     context with the built-in tile rule, with autotuned per-level tiles, and
     with autotuned tiles on 4 concurrent contexts (FrameBatch); the tuning
-    itself (once per geometry) is timed separately."""
+    itself (once per geometry) is timed separately.  Then the learned start (the frame
+    network, control-point start, layer 0, 5 iterations) on the same crops,
+    on one and on 4 contexts, with each crop's endpoint error against its
+    plain field."""
     import torch
 
     from wholistic_registration_amd import autotune
@@ -527,7 +530,18 @@ def crops_measure(n_crops=16, seed=3, reps=3):
     with FrameBatch(workers=4, autotune=True) as fb:
         fb.tiles = cache
         dt = timed(lambda: fb.register_crops(pairs, iter=15, warp=True, **params))
+        plain = [r.motion.clone() for r in fb.register_crops(pairs, iter=15, **params)[0]]
     out["tuned_4ctx"] = {"value": n_vox / dt, "ms": 1e3 * dt}
+    # the learned start on the same crops (network trained on 64 x 512 x 1024 frames)
+    from wholistic_registration_amd import learned as Lr
+    net = Lr.InitNet.load()
+    for workers, name in ((1, "learned_1ctx"), (4, "learned_4ctx")):
+        with FrameBatch(workers=workers) as fb:
+            dt = timed(lambda: fb.register_crops(pairs, iter=Lr.SCHEDULE["iter"], net=net, warp=True, **params))
+            res = fb.register_crops(pairs, iter=Lr.SCHEDULE["iter"], net=net, **params)[0]
+        epe = [float(torch.sqrt(((r.motion - pm) ** 2).sum(-1)).mean()) for r, pm in zip(res, plain)]
+        out[name] = {"value": n_vox / dt, "ms": 1e3 * dt, "epe_vs_plain": [round(e, 4) for e in epe],
+                     "crops_within_gate": int(sum(e <= 0.1 for e in epe))}
     out["unit"] = "registered voxels/s"
     return out
 
