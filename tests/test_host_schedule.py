@@ -53,3 +53,25 @@ def test_network_flops_and_recipes():
         d, h, w = z, y // 2 ** learned.LEVEL, x // 2 ** learned.LEVEL
         assert d % 4 == 0 and h % 4 == 0 and w % 32 == 0, name      # the network's input rule
         assert max_layer(rc["zyx"], rc["lk"]["r"]) >= rc["lk"]["layer"], name
+
+
+def test_pair_batches_share_one_shape():
+    import numpy as np
+
+    class X:
+        def __init__(self, shape):
+            self.shape = shape
+    ps = train_init.PairSet.__new__(train_init.PairSet)
+    ps.x = [X((1, 64, 64, 128, 8))] * 6 + [X((1, 32, 64, 128, 8))] * 3
+    ps.groups = {}
+    for i, x in enumerate(ps.x):
+        ps.groups.setdefault(tuple(x.shape), []).append(i)
+    rng = np.random.default_rng(0)
+    seen = set()
+    for _ in range(200):
+        idx = ps.sample(rng, 4)
+        shapes = {ps.x[i].shape for i in idx}
+        assert len(shapes) == 1 and len(set(idx)) == len(idx)
+        seen |= shapes
+        assert len(idx) == (4 if shapes == {(1, 64, 64, 128, 8)} else 3)
+    assert len(seen) == 2

@@ -107,18 +107,25 @@ def penalty(tmpl):
 class PairSet:
     """Network inputs and targets of generated pairs, resident on the device."""
 
-    def __init__(self, dev, templates, fields_per_template, seed0, target="lk", log=None, recipe="frame_64x512x1024"):
+    def __init__(self, dev, templates, fields_per_template, seed0, target="lk", log=None, recipe="frame_64x512x1024",
+                 depths=None):
         torch = _torch()
-        rc = RECIPES[recipe]
+        rc0 = RECIPES[recipe]
         self.x, self.t, self.meta = [], [], []
-        for ti in range(templates):
-            clean, tmpl = make_template(seed0 + 1000 * ti, dev, zyx=rc["zyx"], n_blobs=rc["blobs"])
+        # frames of several depths (z extents): each template is made at every depth, blobs
+        # in proportion; a batch is drawn from pairs of one shape
+        depths = [rc0["zyx"][0]] if not depths else list(depths)
+        jobs = [(ti, dz) for ti in range(templates) for dz in depths]
+        for ti, dz in jobs:
+            rc = dict(rc0, zyx=(dz,) + tuple(rc0["zyx"][1:]), blobs=int(rc0["blobs"] * dz / rc0["zyx"][0]))
+            clean, tmpl = make_template(seed0 + 1000 * ti + 100 * depths.index(dz), dev, zyx=rc["zyx"],
+                                        n_blobs=rc["blobs"])
             ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
             mask = make_mask(seed0 + 1000 * ti, rc["zyx"], rc["mask_fraction"], dev) if rc["mask_fraction"] else None
             sp = penalty(tmpl) if mask is None else float(D.smooth_penalty_factor(
                 tmpl.float().contiguous(), mask)) * RAW_PENALTY
             for fi in range(fields_per_template):
-                seed = seed0 + 1000 * ti + 1 + fi
+                seed = seed0 + 1000 * ti + 100 * depths.index(dz) + 1 + fi
                 mov, d = make_moving(clean, seed, dev, rc["modes"], rc["dmax"])
                 if target == "lk":
                     res = D.get_motion(mov, tmpl, smoothPenalty=sp, mask_ref=mask, **rc["lk"])
@@ -136,11 +143,22 @@ class PairSet:
             del clean, tmpl
             torch.cuda.empty_cache()
             if log:
-                log(f"[pairs] template {ti + 1}/{templates}: {len(self.x)} pairs")
+                log(f"[pairs] template {ti + 1}/{templates} depth {dz}: {len(self.x)} pairs")
+        self.groups = {}
+        for i, x in enumerate(self.x):
+            self.groups.setdefault(tuple(x.shape), []).append(i)
 
     def batch(self, idx):
         torch = _torch()
         return torch.cat([self.x[i] for i in idx]), torch.cat([self.t[i] for i in idx])
+
+    def sample(self, rng, batch):
+        """Indices of ``batch`` pairs of one shape (shapes drawn in proportion
+        to their pairs)."""
+        keys = list(self.groups)
+        sizes = np.array([len(self.groups[k]) for k in keys], dtype=np.float64)
+        g = self.groups[keys[int(rng.choice(len(keys), p=sizes / sizes.sum()))]]
+        return [g[int(i)] for i in rng.choice(len(g), size=min(batch, len(g)), replace=False)]
 
 
 def train(net, pairs, steps, batch, lr, log=None, log_every=50, warmup=200):
@@ -149,10 +167,8 @@ def train(net, pairs, steps, batch, lr, log=None, log_every=50, warmup=200):
     returns the loss curve (mean per logging window)."""
     rng = np.random.default_rng(0)
     curve, window = [], []
-    n = len(pairs.x)
     for k in range(steps):
-        idx = rng.choice(n, size=min(batch, n), replace=False)
-        x, t = pairs.batch(idx)
+        x, t = pairs.batch(pairs.sample(rng, batch))
         net.zero_grad()
         out = net.forward(x)
         loss, dout = net.loss(out, t)
@@ -197,6 +213,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--recipe", default="frame_64x512x1024", choices=sorted(RECIPES))
     ap.add_argument("--init", default=None, help="safetensors weights to start from (fine-tuning)")
+    ap.add_argument("--depths", default=None, help="comma list of frame depths (z extents) to mix")
     ap.add_argument("--report", default=None, help="JSON file for the loss curve and validation")
     args = ap.parse_args(argv)
     torch = _torch()
@@ -206,8 +223,9 @@ def main(argv=None):
 
     dev = torch.device("cuda", 0)
     t0 = time.time()
-    pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log, args.recipe)
-    val = PairSet(dev, args.val_templates, 4, 20000, args.target, log, args.recipe)
+    depths = [int(v) for v in args.depths.split(",")] if args.depths else None
+    pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log, args.recipe, depths)
+    val = PairSet(dev, args.val_templates, 4, 20000, args.target, log, args.recipe, depths)
     t1 = time.time()
     net = Lr.InitNet.load(args.init) if args.init else Lr.InitNet(seed=args.seed, lr=args.lr)
     torch.cuda.synchronize()
@@ -217,7 +235,7 @@ def main(argv=None):
     t3 = time.time()
     v_loss, v_zero = evaluate(net, val)
     net.save(args.out)
-    rep = {"recipe": args.recipe, "init": args.init, "pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target,
+    rep = {"recipe": args.recipe, "init": args.init, "depths": depths, "pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target,
            "steps": args.steps,
            "batch": args.batch, "lr": args.lr, "data_s": t1 - t0, "train_s": t3 - t2,
            "ms_per_step": 1e3 * (t3 - t2) / args.steps, "curve": curve,
