@@ -139,7 +139,8 @@ def test_learned_frames_on_concurrent_contexts_equal_single_runs(setup):
 def test_learned_start_on_uneven_crops(setup):
     """Crops whose level-3 image is not a multiple of the network's (4, 4, 32)
     grid run on an edge-padded grid; crops on concurrent contexts equal their
-    single runs, and the learned start beats the zero start on a deep crop."""
+    single runs, the learned start beats the zero start on a deep crop, and by
+    default a crop shallower than 48 planes runs the plain pyramid method."""
     import torch
 
     from wholistic_registration_amd import device as D
@@ -153,10 +154,17 @@ def test_learned_start_on_uneven_crops(setup):
     pairs = [(mov[b].contiguous(), s["tmpl"][b].contiguous()) for b in boxes]
     p = s["params"]
     with FrameBatch(workers=2) as fb:
-        results, _ = fb.register_crops(pairs, net=s["net"], iter=Lr.SCHEDULE["iter"], **p)
+        results, _ = fb.register_crops(pairs, net=s["net"], iter=Lr.SCHEDULE["iter"], learned_min_depth=0, **p)
+        policy, _ = fb.register_crops(pairs, net=s["net"], iter=Lr.SCHEDULE["iter"], **p)
     for (m, t), res in zip(pairs, results):
         single, _ = Lr.register_learned(m, t, s["net"], **Lr.SCHEDULE, **p)
         assert torch.equal(single.motion, res.motion)
+    # default policy: the 40-plane crop (< 48) runs the plain pyramid method
+    from wholistic_registration_amd.batch import max_layer
+    assert torch.equal(policy[0].motion, results[0].motion)
+    m, t = pairs[1]
+    plain40 = D.get_motion(m, t, layer=max_layer(m.shape, p["r"]), iter=15, **p)
+    assert torch.equal(policy[1].motion, plain40.motion) and policy[1].records == plain40.records
     m, t = pairs[0]
     plain = D.get_motion(m, t, layer=4, iter=15, **p).motion
     zero = D.get_motion(m, t, **Lr.SCHEDULE, **p).motion

@@ -147,7 +147,7 @@ class FrameBatch:
             raise errors[0]
         return motion_out, results
 
-    def register_crops(self, pairs, layer=None, warp=False, net=None, **params):
+    def register_crops(self, pairs, layer=None, warp=False, net=None, learned_min_depth=48, plain_iter=15, **params):
         """Register crops of uneven sizes: ``pairs`` is a sequence of ``(mov,
         ref)`` or ``(mov, ref, mask_ref)`` device volumes, each pair of one
         shape.  The crops are spread over the worker contexts, largest first,
@@ -160,7 +160,11 @@ class FrameBatch:
         (None unless ``warp``).  With ``net`` (a ``learned.InitNet``) every
         crop starts from the network's field at full resolution
         (``learned.register_learned``, ``layer`` 0 unless given; pass
-        ``learned.SCHEDULE``'s iter), each worker on its own fork."""
+        ``learned.SCHEDULE``'s iter), each worker on its own fork; crops
+        shallower than ``learned_min_depth`` planes run the plain pyramid
+        method instead (``max_layer`` levels, ``plain_iter`` iterations): on
+        32-plane crops the full-resolution schedule stops outside the
+        0.1-voxel gate from any start."""
         torch = _torch()
         pairs = [tuple(p) for p in pairs]
         n = len(pairs)
@@ -171,12 +175,19 @@ class FrameBatch:
                 raise ValueError("each crop's mov and ref must share one shape")
         r = params["r"]
         forks = None
+        use_net = [net is not None and p[0].shape[0] >= learned_min_depth for p in pairs]
         if net is not None:
             from . import learned as _learned
             forks = [net.fork(ctx) for ctx in self.contexts]
-            layers = [0 if layer is None else int(layer) for _ in pairs]
-        else:
-            layers = [max_layer(p[0].shape, r) if layer is None else int(layer) for p in pairs]
+
+        def crop_layer(shape, learned_start):
+            if learned_start:
+                return 0 if layer is None else int(layer)
+            if net is not None or layer is None:          # the plain method: as deep as the crop allows
+                return max_layer(shape, r)
+            return int(layer)
+
+        layers = [crop_layer(p[0].shape, u) for p, u in zip(pairs, use_net)]
         w = len(self.contexts)
         load, owner = [0] * w, [[] for _ in range(w)]
         for i in sorted(range(n), key=lambda i: -pairs[i][0].numel()):
@@ -198,10 +209,12 @@ class FrameBatch:
                         if self.tiles is not None:
                             extra["tile_patches_x"] = self.tiles.tiles(mov, ref, layer=layers[i], context=ctx,
                                                                        stream=strm, **extra)
-                        if forks is not None:
+                        if use_net[i]:
                             results[i], _ = _learned.register_learned(mov, ref, forks[k], layer=layers[i], context=ctx,
                                                                       stream=strm, **extra)
                         else:
+                            if net is not None:          # the plain method for a crop too shallow
+                                extra["iter"] = plain_iter
                             results[i] = _dev.get_motion(mov, ref, layer=layers[i], context=ctx, stream=strm, **extra)
                         if warp:
                             warped[i] = _dev.correct_motion(mov, results[i].motion, context=ctx, stream=strm)
