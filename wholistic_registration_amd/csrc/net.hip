@@ -85,7 +85,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long nv = static_cast<long long>(g.d) * g.h * g.w;
     const int co0 = blockIdx.y * 16;
-    const long long vb = static_cast<long long>(blockIdx.x) * (64 * NT) + wave * (16 * NT);
+    // workgroups go round-robin to the 8 XCDs; give each XCD a contiguous run
+    // of voxel chunks so the neighbour planes its taps share stay in its L2
+    // (gridDim.x is a multiple of 8 on the network's volumes, so the XCD of a
+    // workgroup is blockIdx.x % 8 whatever blockIdx.y, z)
+    const unsigned per = gridDim.x / 8;
+    const unsigned bx = blockIdx.x < per * 8 ? (blockIdx.x % 8) * per + blockIdx.x / 8 : blockIdx.x;
+    const long long vb = static_cast<long long>(bx) * (64 * NT) + wave * (16 * NT);
     const uint16_t* xb = X + static_cast<size_t>(blockIdx.z) * nv * cin;
     const size_t yb = static_cast<size_t>(blockIdx.z) * nv * cout;
     const int ksz = 27 * cin;
