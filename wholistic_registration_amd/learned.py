@@ -44,6 +44,9 @@ LAYERS = (
     ("dec1a", 48, 16, True), ("dec1b", 16, 16, True),
     ("out", 16, 16, False),
 )
+# halvings of the input grid each layer runs at (0 = the input's own size)
+LAYER_DEPTH = {"enc1a": 0, "enc1b": 0, "enc2a": 1, "enc2b": 1, "bota": 2, "botb": 2, "dec2a": 1, "dec2b": 1,
+               "dec1a": 0, "dec1b": 0, "out": 0}
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "init_net.safetensors")
 # getMotion schedule after the network's start: the full-resolution level
 # only, 5 iterations (a start at the coarsest level is pulled back to that
@@ -333,11 +336,9 @@ class InitNet:
 def forward_flops(d, h, w, batch=1):
     """Multiply-add work of one forward pass (2 flops per MAC, the 8 padded
     input channels included): what the matrix cores execute."""
-    dims = {"enc1a": 0, "enc1b": 0, "enc2a": 1, "enc2b": 1, "bota": 2, "botb": 2, "dec2a": 1, "dec2b": 1,
-            "dec1a": 0, "dec1b": 0, "out": 0}
     total = 0
     for name, cin, cout, _ in LAYERS:
-        k = 2 ** dims[name]
+        k = 2 ** LAYER_DEPTH[name]
         total += 2 * 27 * cin * cout * (d // k) * (h // k) * (w // k)
     return batch * total
 
