@@ -24,6 +24,12 @@ def test_library_exports_every_header_symbol():
     assert set(names) == set(_lib._SIGNATURES), "ctypes signatures out of sync with the header"
 
 
+def test_integration_guide_names_every_header_symbol():
+    guide = open(os.path.join(os.path.dirname(_lib.PKG), "INTEGRATION.md")).read()
+    missing = [n for n in _lib.header_symbols() if f"`{n}`" not in guide]
+    assert not missing, missing
+
+
 def _c_struct_fields(struct_name):
     text = open(_lib.HEADER).read()
     body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct_name, struct_name), text, re.S).group(1)
