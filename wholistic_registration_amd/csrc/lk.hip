@@ -191,6 +191,66 @@ __global__ __launch_bounds__(256) void k_dense_maxdiff(const Ctl* __restrict__ c
     block_max_to(md, mdpart + blockIdx.x);
 }
 
+// The same maximum with the level's column weights (one per x, formed once
+// per workgroup) and each row's two control-point rows staged in LDS: a
+// voxel then costs LDS reads instead of twelve cached gathers and a
+// correctly rounded division.  Same arithmetic and operands as
+// cp_bilinear; LDS holds 2*nx + 6*ncx floats (see dense_lds_bytes).
+inline size_t dense_lds_bytes(const LevelGeom& g) {
+    return (2 * static_cast<size_t>(g.nx) + 6 * static_cast<size_t>(g.ncx)) * sizeof(float);
+}
+constexpr size_t kDenseLdsMax = 48 * 1024;
+
+__global__ __launch_bounds__(256) void k_dense_maxdiff_lds(const Ctl* __restrict__ c, const float* __restrict__ m0,
+                                                            const float* __restrict__ cp, int nx, int ny, int nz,
+                                                            int r, int p, int ncx, int ncy,
+                                                            float* __restrict__ mdpart) {
+    extern __shared__ __align__(16) unsigned char dense_smem[];
+    if (c->done) return;   // the same value for every thread of the launch
+    int* sx0 = reinterpret_cast<int*>(dense_smem);
+    float* stx = reinterpret_cast<float*>(sx0 + nx);
+    float* srow = stx + nx;   // [component][row i0 | row i1][ncx]
+    for (int x = threadIdx.x; x < nx; x += 256) {
+        const Axis1 ax = cp_axis(x, r, p, ncx);
+        sx0[x] = ax.i0;
+        stx[x] = ax.t;
+    }
+    const size_t nvox = static_cast<size_t>(nx) * ny * nz;
+    const size_t ncp = static_cast<size_t>(ncx) * ncy * nz;
+    const unsigned nrows = static_cast<unsigned>(ny) * static_cast<unsigned>(nz);
+    const int nst = 6 * ncx;
+    float md = 0.0f;
+    for (unsigned row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const int y = static_cast<int>(row % static_cast<unsigned>(ny));
+        const int z = static_cast<int>(row / static_cast<unsigned>(ny));
+        const Axis1 ay = cp_axis(y, r, p, ncy);
+        const float* base = cp + static_cast<size_t>(z) * ncy * ncx;
+        __syncthreads();   // the previous row's reads (and the column weights) are done
+        for (int i = threadIdx.x; i < nst; i += 256) {
+            const int q = i / (2 * ncx);
+            const int rem = i - q * 2 * ncx;
+            const int h = rem >= ncx ? 1 : 0;
+            const int ix = rem - h * ncx;
+            srow[i] = base[q * ncp + static_cast<size_t>(h ? ay.i1 : ay.i0) * ncx + ix];
+        }
+        __syncthreads();
+        const float* mrow = m0 + static_cast<size_t>(row) * nx;
+        for (int x = threadIdx.x; x < nx; x += 256) {
+            const int i0 = sx0[x];
+            const int i1 = min(i0 + 1, ncx - 1);
+            const float t = stx[x];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float* a0 = srow + q * 2 * ncx;
+                const float* a1 = a0 + ncx;
+                const float m1 = lerpf(lerpf(a0[i0], a0[i1], t), lerpf(a1[i0], a1[i1], t), ay.t);
+                md = fmaxf(md, fabsf(m1 - mrow[x + q * nvox]));
+            }
+        }
+    }
+    block_max_to(md, mdpart + blockIdx.x);
+}
+
 // Error terms, log and the error-based stop rule of iteration k.
 __global__ __launch_bounds__(1024) void k_err_finish(Ctl* __restrict__ c, int k, const double* __restrict__ errpart,
                                                       long long nerr, const double* __restrict__ penpart,
@@ -328,8 +388,15 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     if (!fuse_err) k_err_finish<<<1, 1024, 0, s>>>(b.ctl, k, b.errpart, nerr, b.penpart, np, b.log, b.counts);
     k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d, b.mdpart, k,
                                 fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts);
-    if (dense)
-        k_dense_maxdiff<<<nd, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, b.mdpart + cpb);
+    if (dense) {
+        const size_t lds = dense_lds_bytes(g);
+        if (lds <= kDenseLdsMax)
+            k_dense_maxdiff_lds<<<nd, 256, lds, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
+                                                     b.mdpart + cpb);
+        else
+            k_dense_maxdiff<<<nd, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
+                                               b.mdpart + cpb);
+    }
     if (!fuse_check(k))
         k_check<<<1, 1024, 0, s>>>(b.ctl, k, b.mdpart, static_cast<int>(parts_of(k)), b.log, b.counts);
     return WR_OK;
