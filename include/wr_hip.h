@@ -73,12 +73,12 @@ typedef struct wr_params {
                                iterations read: 0 = float32 (the reference's
                                values), 1 = bfloat16, 2 = float16; every sum,
                                interpolation and solve stays float32/float64 */
-    int reuse_template;     /* 1 = `ref` (and mask_ref) hold the same data as in
+    int reuse_template;     /* 1 = `ref` and mask_ref hold the same data as in
                                the previous wr_get_motion on this context, with
-                               the same pointer, size, dtype, layer and
-                               precision: its pyramid levels are not rebuilt
-                               (frames of a time series against one template);
-                               a mismatch of those is an error */
+                               the same ref and mask_ref pointers, size, dtype,
+                               layer and precision: their pyramid levels are not
+                               rebuilt (frames of a time series against one
+                               template); a mismatch of those is an error */
     int init_nx;            /* with init_ny: 0, 0 = motion_init is (nz, ny, nx, 3); else
                                motion_init is (nz, init_ny, init_nx, 3), still
                                in full-resolution voxel units: the reference's

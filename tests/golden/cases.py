@@ -74,8 +74,21 @@ SMALL_CASES = {
 }
 
 
+# The parameter set of the 96x1024x2048 whole-body workload (33x33x1 patches,
+# r = 16; 20 iterations per level; a mask_ref ellipsoid over 10 % of the
+# voxels; zRatio 1; blob density 2e-3 per voxel) on volumes small enough for
+# the reference's full-grid correlates, with every level still holding a
+# patch centre (x_L - 1 > r).  The fixtures store input hashes, the level-0
+# control-point field, sampled voxels and the per-level error logs.
+R16_CASES = {
+    "r16_6x80x96": (107, (6, 80, 96), 90, 2, 3.0, 2, 16, 20, 1.0, {"mask_ref": 0.10}),
+    "r16_4x288x320": (108, (4, 288, 320), 740, 3, 4.0, 4, 16, 20, 1.0, {"mask_ref": 0.10}),
+}
+
+
 def small_inputs(name):
-    seed, zyx, nb, nm, dmax, layer, r, n_iter, zr, flags = SMALL_CASES[name]
+    spec = SMALL_CASES[name] if name in SMALL_CASES else R16_CASES[name]
+    seed, zyx, nb, nm, dmax, layer, r, n_iter, zr, flags = spec
     pair = synth.make_pair_np(seed, zyx, nb, nm, dmax,
                               mask_fraction=flags.get("mask_ref", 0.0))
     mov = pair.moving.transpose(2, 1, 0)

@@ -3,7 +3,7 @@
 Run in the development container only (the reference tree is not present on
 GPU machines):
 
-    python tests/golden/make_golden.py [--reference /root/reference] [--skip-64x128x128]
+    python tests/golden/make_golden.py [--reference /root/reference] [--skip-64x128x128] [--only-r16]
 
 It imports the reference's hot-path modules
 (``src/wholistic_registration/utils/{calFlow3d_Wei_v1,imresize,interp,
@@ -38,7 +38,7 @@ sys.path.insert(0, REPO)
 from wholistic_registration_amd import synth  # noqa: E402
 
 from tests.golden.cases import (  # noqa: E402
-    BLOBS_64x128x128, IMRESIZE_CASES, SMALL_CASES, inputs_64x128x128, imresize_input, small_inputs,
+    BLOBS_64x128x128, IMRESIZE_CASES, R16_CASES, SMALL_CASES, inputs_64x128x128, imresize_input, small_inputs,
 )
 
 
@@ -123,14 +123,44 @@ def write_64x128x128(cf):
           [len(log[f"layer_{L}"]["currentError"]) for L in range(option["layer"], -1, -1)])
 
 
+def write_r16(cf):
+    """The whole-body workload's parameter set (r 16, 20 iterations, mask_ref)
+    on small volumes: input hashes, the level-0 field at the control points
+    and at 4096 sampled voxels, the per-level error logs."""
+    for name in R16_CASES:
+        mov, ref, option = small_inputs(name)
+        motion, err, log = run_reference(cf, mov, ref, option)
+        r = option["r"]
+        nx, ny, nz = mov.shape
+        xg = np.arange(r, nx - 1, 2 * r + 1)
+        yg = np.arange(r, ny - 1, 2 * r + 1)
+        pick = np.random.default_rng(54321).integers(0, [nx, ny, nz], size=(4096, 3))
+        payload = dict(
+            mov_sha=sha(mov), ref_sha=sha(ref), mask_ref_sha=sha(np.asarray(option["mask_ref"])),
+            smooth_penalty=np.float64(option["smoothPenalty"]),
+            motion_cp=motion[np.ix_(xg, yg, np.arange(nz))],
+            sample_idx=pick, sample_motion=motion[pick[:, 0], pick[:, 1], pick[:, 2]],
+            motion_absmax=np.float32(np.abs(motion).max()),
+            current_error=np.float64(err), **log_arrays(log, option["layer"]))
+        np.savez_compressed(os.path.join(HERE, f"getmotion_{name}.npz"), **payload)
+        print(name, "err", err, "records",
+              [len(log[f"layer_{L}"]["currentError"]) for L in range(option["layer"], -1, -1)])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
     ap.add_argument("--skip-64x128x128", action="store_true")
+    ap.add_argument("--only-r16", action="store_true",
+                    help="write only the fixtures of the r = 16 parameter set")
     args = ap.parse_args()
     cf, im = import_reference(args.reference)
+    if args.only_r16:
+        write_r16(cf)
+        return
     write_imresize(im)
     write_small(cf)
+    write_r16(cf)
     if not args.skip_64x128x128:
         write_64x128x128(cf)
     _ = synth  # the inputs come from the package's own generator

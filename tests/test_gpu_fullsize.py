@@ -127,3 +127,53 @@ def test_fullsize_reduced_precision_epe(full, precision):
     b = D.get_motion(full["mov"], full["tmpl"], mask_ref=full["mask"], precision=precision, **p).motion
     e = torch.sqrt(((a - b) ** 2).sum(-1)).mean().item()
     assert e <= 0.1
+
+
+CROP = (12, 320, 640)    # (z, y, x): level 4 is 40 x 20, one patch centre
+
+
+def test_fullsize_parameter_set_on_masked_crop_matches_oracle(full):
+    """The workload's own parameters (layer 4, r 16, 20 iterations, mask_ref)
+    on a crop of the workload's pair centred on the mask ellipsoid, against
+    the oracle on the same bytes (calFlow3d_Wei_v1.py:430-439,515-530,546-588,
+    641-654): free-running schedule equal to the oracle's; with that schedule
+    forced, field max |d| <= 1e-3 voxel and the warp by that field within
+    relative L2 1e-5 of the oracle's warp; mean
+    endpoint error against the all-float64 oracle <= 0.1 voxel."""
+    import torch
+
+    from oracle.metrics import epe_stats, rel_l2
+    from wholistic_registration_amd import device as D
+    mask = full["mask"]
+    nz, ny, nx = mask.shape
+    cen = torch.nonzero(mask > 0).float().mean(0).tolist()
+    lo = [int(min(max(round(c - s / 2), 0), n - s)) for c, s, n in zip(cen, CROP, (nz, ny, nx))]
+    sl = tuple(slice(a, a + s) for a, s in zip(lo, CROP))
+    tmpl = full["tmpl"][sl].contiguous()
+    mov = full["mov"][sl].contiguous()
+    mk = mask[sl].contiguous()
+    frac = float((mk > 0).float().mean())
+    print(f"crop origin (z,y,x) {lo}, size {CROP}, masked fraction {frac:.3f}", flush=True)
+    assert 0.0 < frac < 1.0
+    ref_x = tmpl.float().cpu().numpy().transpose(2, 1, 0)
+    mov_x = mov.float().cpu().numpy().transpose(2, 1, 0)
+    mk_x = (mk.cpu().numpy() > 0).transpose(2, 1, 0)
+    p = dict(full["params"])
+    p["smoothPenalty"] = O.smooth_penalty_factor(ref_x, mk_x) * 0.08
+    opt = dict(layer=p["layer"], iter=p["iter"], r=p["r"], zRatio=1.0, movRange=5.0, tol=1e-3,
+               save_ite=10 ** 9, mask_ref=mk_x, mask_mov=np.zeros(mk_x.shape, bool),
+               smoothPenalty=p["smoothPenalty"])
+    ex = {}
+    want, _, _, _ = O.get_motion(mov_x, ref_x, dict(opt), extras=ex, keep_images=False)
+    free = D.get_motion(mov, tmpl, mask_ref=mk, **p)
+    print(f"schedule oracle {ex['schedule']}, GPU {free.schedule()}", flush=True)
+    assert free.schedule() == ex["schedule"]
+    forced = D.get_motion(mov, tmpl, mask_ref=mk, schedule=ex["schedule"], **p)
+    got = forced.motion.cpu().numpy().transpose(2, 1, 0, 3)
+    assert np.abs(got - want).max() <= 1e-3
+    warped = D.correct_motion(mov, forced.motion).cpu().numpy().transpose(2, 1, 0)
+    assert rel_l2(warped, O.correct_motion(mov_x, got)) <= 1e-5
+    m64, _, _, _ = O.get_motion(mov_x, ref_x, dict(opt), mode="fp64", keep_images=False)
+    e = epe_stats(free.motion.cpu().numpy().transpose(2, 1, 0, 3), m64)
+    print(f"EPE vs float64 oracle: {e}", flush=True)
+    assert e["epe_mean"] <= 0.1

@@ -171,3 +171,24 @@ def test_learned_start_on_uneven_crops(setup):
     e_learned = float(torch.sqrt(((results[0].motion - plain) ** 2).sum(-1)).mean())
     e_zero = float(torch.sqrt(((zero - plain) ** 2).sum(-1)).mean())
     assert e_learned < e_zero, (e_learned, e_zero)
+
+
+def test_learned_frames_with_fallback_on_concurrent_contexts(setup):
+    """FrameBatch(net=..., fallback="plain"): the plain fallback run rebuilds a
+    context's template levels for its own layer, so the next frame on that
+    context must not reuse them; every frame equals its single run."""
+    import torch
+
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    from wholistic_registration_amd.batch import FrameBatch
+    s = setup
+    frames = [T.make_moving(s["clean"], seed, s["tmpl"].device)[0] for seed in (90003, 90004, 90005, 90006)]
+    with FrameBatch(workers=2) as fb:
+        motion, results = fb.register(frames, s["tmpl"], net=s["net"], fallback="plain", **Lr.SCHEDULE,
+                                      **s["params"])
+    for f, m, res in zip(frames, motion, results):
+        single, _ = Lr.register_learned(f, s["tmpl"], s["net"], ref_level=s["ref_l"], fallback="plain",
+                                        **Lr.SCHEDULE, **s["params"])
+        assert torch.equal(single.motion, m)
+        assert single.records == res.records

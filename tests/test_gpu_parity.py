@@ -17,8 +17,8 @@ import pytest
 
 from oracle import lk_oracle as O
 from oracle.metrics import epe_stats, rel_l2
-from tests.golden.cases import (BLOBS_64x128x128, IMRESIZE_CASES, SMALL_CASES, inputs_64x128x128,
-                                imresize_input, small_inputs)
+from tests.golden.cases import (BLOBS_64x128x128, IMRESIZE_CASES, R16_CASES, SMALL_CASES,
+                                inputs_64x128x128, imresize_input, small_inputs)
 
 pytestmark = pytest.mark.gpu
 
@@ -148,6 +148,45 @@ def test_get_motion_64x128x128_matches_reference(cuda, stop_kernels):
     idx = g["sample_idx"]
     assert np.abs(got[idx[:, 0], idx[:, 1], idx[:, 2]] - g["sample_motion"]).max() <= FIELD_TOL
     assert res.final_error == pytest.approx(float(g["current_error"]), rel=1e-5)
+
+
+@pytest.mark.parametrize("stop_kernels", [False, True])
+@pytest.mark.parametrize("name", sorted(R16_CASES))
+def test_get_motion_r16_matches_reference(cuda, name, stop_kernels):
+    """The whole-body workload's parameter set (r 16, 20 iterations, mask_ref)
+    on the reference's own runs: free-running schedule equal, level-0 field
+    within 1e-3 voxel at the control points and sampled voxels."""
+    g = np.load(os.path.join(GOLDEN, f"getmotion_{name}.npz"))
+    mov, ref, opt = small_inputs(name)
+    res, got = _run_gpu(mov, ref, opt, cuda, stop_kernels=stop_kernels)
+    for L in range(opt["layer"], -1, -1):
+        assert res.records[L] == len(g[f"L{L}_currentError"]), L
+        assert res.updates[L] == len(g[f"L{L}_max_diff_motion"]), L
+        np.testing.assert_allclose(res.current_error[L, :res.records[L]], g[f"L{L}_currentError"], rtol=2e-5)
+    r = opt["r"]
+    nx, ny, nz = mov.shape
+    xg = np.arange(r, nx - 1, 2 * r + 1)
+    yg = np.arange(r, ny - 1, 2 * r + 1)
+    assert np.abs(got[np.ix_(xg, yg, np.arange(nz))] - g["motion_cp"]).max() <= FIELD_TOL
+    idx = g["sample_idx"]
+    assert np.abs(got[idx[:, 0], idx[:, 1], idx[:, 2]] - g["sample_motion"]).max() <= FIELD_TOL
+    assert res.final_error == pytest.approx(float(g["current_error"]), rel=1e-5)
+
+
+@pytest.mark.parametrize("name", sorted(R16_CASES))
+def test_get_motion_r16_forced_schedule_matches_oracle(cuda, name):
+    """Same cases with the oracle's schedule forced: the whole field within
+    1e-3 voxel of the oracle, its warp within relative L2 1e-5."""
+    mov, ref, opt = small_inputs(name)
+    ex = {}
+    want, err, _, _ = O.get_motion(mov, ref, dict(opt), extras=ex, keep_images=False)
+    res, got = _run_gpu(mov, ref, opt, cuda, schedule=ex["schedule"])
+    assert res.schedule() == ex["schedule"]
+    assert np.abs(got - want).max() <= FIELD_TOL
+    import torch
+    warped = _dev().correct_motion(_to_zyx(mov, cuda), torch.from_numpy(
+        np.ascontiguousarray(got.transpose(2, 1, 0, 3))).to(cuda)).cpu().numpy().transpose(2, 1, 0)
+    assert rel_l2(warped, O.correct_motion(mov, got)) <= WARP_TOL
 
 
 def test_drop_in_module_round_trip(cuda):

@@ -10,8 +10,8 @@ import numpy as np
 import pytest
 
 from oracle import lk_oracle as O
-from tests.golden.cases import (BLOBS_64x128x128, IMRESIZE_CASES, SMALL_CASES, inputs_64x128x128,
-                                imresize_input, small_inputs)
+from tests.golden.cases import (BLOBS_64x128x128, IMRESIZE_CASES, R16_CASES, SMALL_CASES,
+                                inputs_64x128x128, imresize_input, small_inputs)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -86,3 +86,31 @@ def test_get_motion_64x128x128():
     assert err == pytest.approx(float(g["current_error"]), rel=1e-9)
     for L in range(BLOBS_64x128x128["layer"], -1, -1):
         assert len(log[f"layer_{L}"]["currentError"]) == len(g[f"L{L}_currentError"])
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("name", sorted(R16_CASES))
+def test_get_motion_r16_cases(name):
+    """The whole-body workload's parameter set (33x33x1 patches, 20
+    iterations per level, a mask_ref ellipsoid) against the reference's runs
+    on the same bytes: schedule, error logs, and the level-0 field at the
+    control points and at sampled voxels."""
+    g = _load(f"getmotion_{name}.npz")
+    mov, ref, opt = small_inputs(name)
+    assert _sha(mov) == str(g["mov_sha"]) and _sha(ref) == str(g["ref_sha"])
+    assert _sha(np.asarray(opt["mask_ref"])) == str(g["mask_ref_sha"])
+    assert opt["smoothPenalty"] == float(g["smooth_penalty"])
+    motion, err, _, log = O.get_motion(mov, ref, dict(opt), keep_images=False)
+    _check_log(log, g, opt["layer"], exact=False)
+    r = opt["r"]
+    nx, ny, nz = mov.shape
+    xg = np.arange(r, nx - 1, 2 * r + 1)
+    yg = np.arange(r, ny - 1, 2 * r + 1)
+    np.testing.assert_allclose(motion[np.ix_(xg, yg, np.arange(nz))], g["motion_cp"], rtol=0, atol=1e-5)
+    idx = g["sample_idx"]
+    np.testing.assert_allclose(motion[idx[:, 0], idx[:, 1], idx[:, 2]], g["sample_motion"], rtol=0,
+                               atol=1e-5)
+    assert err == pytest.approx(float(g["current_error"]), rel=1e-9)

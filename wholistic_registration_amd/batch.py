@@ -105,6 +105,10 @@ class FrameBatch:
             forks = [net.fork(ctx) for ctx in self.contexts]
         # inputs come from the caller's stream: let it finish before the workers start
         torch.cuda.current_stream(ref.device).synchronize()
+        # template pyramids are reused within one call only: between calls the
+        # caller may free the template and the allocator hand its address to a
+        # new one
+        self.invalidate()
         results = [None] * n
         errors = []
         w = len(self.contexts)
@@ -123,14 +127,16 @@ class FrameBatch:
                         extra["tile_patches_x"] = self.tiles.tiles(frames[i], ref, context=ctx, stream=strm,
                                                                    **params)
                     if forks is not None:
-                        results[i], _ = _learned.register_learned(
+                        results[i], info = _learned.register_learned(
                             frames[i], ref, forks[k], ref_level=ref_level, motion_out=motion_out[i],
                             context=ctx, stream=strm, reuse_template=reuse, **extra)
+                        # a fallback run rebuilt the context's template levels for its own layer
+                        self._ref_seen[k] = key if "plain_error" not in info else None
                     else:
                         results[i] = _dev.get_motion(
                             frames[i], ref, motion_out=motion_out[i], context=ctx, stream=strm,
                             reuse_template=reuse, mask_mov=None if masks_mov is None else masks_mov[i], **extra)
-                    self._ref_seen[k] = key
+                        self._ref_seen[k] = key
                     if warped_out is not None:
                         _dev.correct_motion(frames[i], motion_out[i], out=warped_out[i], context=ctx, stream=strm)
                 strm.synchronize()

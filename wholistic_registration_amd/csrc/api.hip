@@ -309,13 +309,14 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     key.precision = st;
     key.masked = mask_ref != nullptr;
     key.ref = ref;
+    key.mref = mask_ref;
     const bool reuse = prm->reuse_template != 0;
     if (reuse) {
         const Context::TemplateKey& k = ctx->c.tmpl;
         WR_ARG(k.valid && k.nx == nx && k.ny == ny && k.nz == nz && k.layer == layer && k.dtype == dtype &&
-                   k.precision == st && k.masked == key.masked && k.ref == ref,
-               "wr_get_motion: reuse_template without a previous call on the same template, size, dtype, "
-               "layer and precision");
+                   k.precision == st && k.masked == key.masked && k.ref == ref && k.mref == mask_ref,
+               "wr_get_motion: reuse_template without a previous call on the same template, mask_ref, size, "
+               "dtype, layer and precision");
     }
     ctx->c.tmpl.valid = 0;   // set again once this call's template levels are enqueued
 

@@ -392,6 +392,7 @@ struct Context {
     struct TemplateKey {
         int valid = 0, nx = 0, ny = 0, nz = 0, layer = 0, dtype = 0, precision = 0, masked = 0;
         const void* ref = nullptr;
+        const float* mref = nullptr;
     } tmpl;
 };
 
