@@ -161,6 +161,21 @@ int wr_correct_motion(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, in
 int wr_smooth_penalty_factor(wr_ctx* ctx, const void* ref, int dtype, int nx, int ny, int nz,
                              const float* mask_ref, double* factor, void* stream);
 
+/* Allocation-free variant of wr_smooth_penalty_factor.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and arithmetic, except that the factor goes to
+ * DEVICE memory (`factor`, one double) and the call is asynchronous on
+ * `stream` (no host synchronisation).  Its row partials live in a fixed
+ * per-GPU scratch array of the library (WR_NOALLOC_MAX_ROWS rows of 4
+ * doubles, 8 MiB, held until the process ends), resolved by wr_create:
+ * ny * nz above WR_NOALLOC_MAX_ROWS is rejected with WR_E_ARG.  Every call on
+ * one GPU shares that array, so calls that overlap on one GPU (several
+ * streams or threads) race; calls in stream order do not. */
+#define WR_NOALLOC_MAX_ROWS 262144
+int wr_smooth_penalty_factor_noalloc(wr_ctx* ctx, const void* ref, int dtype, int nx, int ny, int nz,
+                                     const float* mask_ref, double* factor, void* stream);
+
 /* Pyramid level image as the reference builds it: imresize(vol,
  * output_shape=(int(nx/2^level), int(ny/2^level), nz)) (bicubic,
  * antialiased, imresize.py:148-191; calFlow3d_Wei_v1.py:341-351), stored as

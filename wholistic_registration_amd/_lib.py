@@ -80,6 +80,7 @@ _SIGNATURES = {
     "wr_imresize": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_get_stats": ([_VP, ctypes.POINTER(WrStats)], _I),
     "wr_smooth_penalty_factor": ([_VP, _VP, _I, _I, _I, _I, _VP, ctypes.POINTER(ctypes.c_double), _VP], _I),
+    "wr_smooth_penalty_factor_noalloc": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP], _I),
     "wr_robust_mean_std": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, ctypes.POINTER(ctypes.c_float),
                             _VP], _I),
     "wr_get_mask": ([_VP, _VP, _I, _I, _I, _I, ctypes.c_double, ctypes.POINTER(ctypes.c_float), _VP, _VP], _I),

@@ -10,8 +10,17 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "wr_internal.h"
+
+// Device scratch of the allocation-free entry point
+// (wr_smooth_penalty_factor_noalloc): the four row partials of the smoothness
+// normaliser for up to WR_NOALLOC_MAX_ROWS rows, then its three results.
+// Each GPU holds its own copy, at its own address, until the process ends;
+// host code never passes the array's name to a kernel, only the address
+// resolved for the device (scratch_on).
+static __device__ __align__(256) double scratch[4 * static_cast<size_t>(WR_NOALLOC_MAX_ROWS) + 4];
 
 namespace wr {
 
@@ -114,6 +123,38 @@ Elem elem_of(int dtype) { return dtype == WR_U16 ? E_U16 : (dtype == WR_F64 ? E_
 
 void set_error(const std::string& msg) { g_error = msg; }
 
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), resolved once per
+// device; null (with the error set) when the runtime cannot give it.
+double* resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device index above the scratch table");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch[device] = static_cast<double*>(ptr);
+    }
+    return g_scratch[device];
+}
+}  // namespace
+
+double* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch[device];
+}
+
 int grow(DevBuf& b, size_t bytes) {
     if (bytes <= b.bytes && b.p) return WR_OK;
     if (b.p) {
@@ -171,6 +212,8 @@ wr_ctx* wr_create(int device) {
         set_error("wr_create: hipSetDevice failed");
         return nullptr;
     }
+    // set-up of the allocation-free entry point: this device's copy of the scratch array
+    if (!resolve_scratch(device)) return nullptr;
     auto* ctx = new wr_ctx;
     ctx->c.device = device;
     return ctx;
@@ -256,12 +299,33 @@ int wr_smooth_penalty_factor(wr_ctx* ctx, const void* ref, int dtype, int nx, in
     WR_TRY(grow(c.pen, (4 * rows + 3) * sizeof(double)));
     double* part = c.pen.as<double>();
     double* out = part + 4 * rows;
-    launch_penalty(ref, elem_of(dtype), mask_ref, nx, ny, nz, part, out, s);
+    launch_penalty(ref, elem_of(dtype), mask_ref, nx, ny, nz, part, out, nullptr, s);
     WR_HIP_CHECK(hipGetLastError());
     double h[3];
     WR_HIP_CHECK(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
     WR_HIP_CHECK(hipStreamSynchronize(s));
     *factor = h[2];
+    return WR_OK;
+}
+
+int wr_smooth_penalty_factor_noalloc(wr_ctx* ctx, const void* ref, int dtype, int nx, int ny, int nz,
+                                     const float* mask_ref, double* factor, void* stream) {
+    WR_ARG(ctx && ref && factor, "wr_smooth_penalty_factor_noalloc: null argument");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_smooth_penalty_factor_noalloc: empty volume");
+    WR_ARG(dtype == WR_F32 || dtype == WR_U16 || dtype == WR_F64,
+           "wr_smooth_penalty_factor_noalloc: dtype must be WR_F32, WR_U16 or WR_F64");
+    WR_ARG(static_cast<long long>(ny) * nz <= WR_NOALLOC_MAX_ROWS,
+           "wr_smooth_penalty_factor_noalloc: " + std::to_string(static_cast<long long>(ny) * nz) +
+               " rows (ny * nz) exceed the scratch maximum of " + std::to_string(WR_NOALLOC_MAX_ROWS));
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    double* part = scratch_on(c.device);
+    WR_ARG(part != nullptr, "wr_smooth_penalty_factor_noalloc: scratch of this device not resolved");
+    const size_t rows = static_cast<size_t>(ny) * nz;
+    // every row partial is written before it is read: nothing to clear
+    launch_penalty(ref, elem_of(dtype), mask_ref, nx, ny, nz, part, part + 4 * rows, factor,
+                   static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
     return WR_OK;
 }
 

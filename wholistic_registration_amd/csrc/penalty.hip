@@ -92,9 +92,10 @@ __global__ __launch_bounds__(kThreads) void k_penalty_rows(const void* __restric
     }
 }
 
-// Fixed-order reduction of the row partials; out = {mean_x, mean_y, factor}.
+// Fixed-order reduction of the row partials; out = {mean_x, mean_y, factor},
+// and the factor also to `factor` when it is not null.
 __global__ __launch_bounds__(1024) void k_penalty_finish(const double* __restrict__ part, long long nrows,
-                                                         double* __restrict__ out) {
+                                                         double* __restrict__ out, double* __restrict__ factor) {
     __shared__ double s[4][1024];
     double a[4] = {0.0, 0.0, 0.0, 0.0};
     for (long long r = threadIdx.x; r < nrows; r += 1024)
@@ -113,13 +114,14 @@ __global__ __launch_bounds__(1024) void k_penalty_finish(const double* __restric
         out[0] = mx;
         out[1] = my;
         out[2] = (mx + my) / 2.0;
+        if (factor) *factor = out[2];
     }
 }
 
 }  // namespace
 
 void launch_penalty(const void* ref, Elem t, const float* mask, int nx, int ny, int nz, double* part,
-                    double* out, hipStream_t s) {
+                    double* out, double* factor, hipStream_t s) {
     const unsigned rows = static_cast<unsigned>(static_cast<long long>(ny) * nz);
     switch (t) {
         case E_U16:
@@ -131,7 +133,7 @@ void launch_penalty(const void* ref, Elem t, const float* mask, int nx, int ny, 
         default:
             k_penalty_rows<E_F64><<<rows, kThreads, 0, s>>>(ref, mask, nx, ny, part);
     }
-    k_penalty_finish<<<1, 1024, 0, s>>>(part, rows, out);
+    k_penalty_finish<<<1, 1024, 0, s>>>(part, rows, out, factor);
 }
 
 }  // namespace wr

@@ -15,6 +15,10 @@ namespace wr {
 
 void set_error(const std::string& msg);
 
+// The device address of the allocation-free entry point's scratch array on
+// `device`, resolved by wr_create; null before that.
+double* scratch_on(int device);
+
 #define WR_HIP_CHECK(expr)                                                       \
     do {                                                                         \
         hipError_t e_ = (expr);                                                  \
@@ -235,9 +239,10 @@ void launch_warp(const void* in, Elem in_t, int nx, int ny, int nz, const float*
 // ---------------------------------------------------------------- penalty
 
 // getSmPnltNormFctr on a (nz, ny, nx) volume: `part` holds 4 doubles per row
-// (ny * nz rows), `out` receives {mean_x, mean_y, factor} on the device.
+// (ny * nz rows), `out` receives {mean_x, mean_y, factor} on the device, and
+// `factor` (device, nullable) the factor alone.
 void launch_penalty(const void* ref, Elem t, const float* mask, int nx, int ny, int nz, double* part,
-                    double* out, hipStream_t s);
+                    double* out, double* factor, hipStream_t s);
 
 // ---------------------------------------------------------------- masks
 
