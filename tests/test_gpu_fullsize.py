@@ -134,7 +134,7 @@ CROP = (12, 320, 640)    # (z, y, x): level 4 is 40 x 20, one patch centre
 
 def test_fullsize_parameter_set_on_masked_crop_matches_oracle(full):
     """The workload's own parameters (layer 4, r 16, 20 iterations, mask_ref)
-    on a crop of the workload's pair centred on the mask ellipsoid, against
+    on a crop of the workload's pair across the mask ellipsoid's edge, against
     the oracle on the same bytes (calFlow3d_Wei_v1.py:430-439,515-530,546-588,
     641-654): free-running schedule equal to the oracle's; with that schedule
     forced, field max |d| <= 1e-3 voxel and the warp by that field within
@@ -146,8 +146,12 @@ def test_fullsize_parameter_set_on_masked_crop_matches_oracle(full):
     from wholistic_registration_amd import device as D
     mask = full["mask"]
     nz, ny, nx = mask.shape
-    cen = torch.nonzero(mask > 0).float().mean(0).tolist()
-    lo = [int(min(max(round(c - s / 2), 0), n - s)) for c, s, n in zip(cen, CROP, (nz, ny, nx))]
+    # centred on the ellipsoid's edge: the row through its centroid, where
+    # the mask begins along x
+    cen = [int(round(c)) for c in torch.nonzero(mask > 0).float().mean(0).tolist()]
+    row = torch.nonzero(mask[cen[0], cen[1]] > 0).flatten()
+    cen[2] = int(row[0]) if row.numel() else cen[2]
+    lo = [int(min(max(c - s // 2, 0), n - s)) for c, s, n in zip(cen, CROP, (nz, ny, nx))]
     sl = tuple(slice(a, a + s) for a, s in zip(lo, CROP))
     tmpl = full["tmpl"][sl].contiguous()
     mov = full["mov"][sl].contiguous()
