@@ -223,7 +223,7 @@ void wr_destroy(wr_ctx* ctx) {
     if (!ctx) return;
     Context& c = ctx->c;
     (void)hipSetDevice(c.device);
-    for (DevBuf* b : {&c.mov_l, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
+    for (DevBuf* b : {&c.mov_l, &c.movq, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
                       &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen,
                       &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig, &c.splat, &c.netw})
         release(*b);
@@ -460,15 +460,19 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
            "wr_get_motion: more than 2^24 padded rows");
     // every level keeps its own images: they are built on the context's side
     // stream, ahead of the iterations that consume them
-    std::vector<size_t> mov_off(layer + 1), ref_off(layer + 1);
-    size_t mov_tot = 0, ref_tot = 0;
+    std::vector<size_t> mov_off(layer + 1), ref_off(layer + 1), q_off(layer + 1);
+    size_t mov_tot = 0, ref_tot = 0, q_tot = 0;
     for (int L = layer; L >= 0; --L) {
         mov_off[L] = mov_tot;
         mov_tot += (padded_floats(geo[L].nx, geo[L].ny, nz) + 63) & ~static_cast<size_t>(63);
+        q_off[L] = q_tot;
+        if (st != ST_F32)
+            q_tot += (moving_records_bytes(geo[L].nx, geo[L].ny, nz, st) + 255) & ~static_cast<size_t>(255);
         ref_off[L] = ref_tot;
         ref_tot += (static_cast<size_t>(geo[L].nx) * geo[L].ny * nz + 255) & ~static_cast<size_t>(255);
     }
-    WR_TRY(grow(c.mov_l, mov_tot * es));
+    WR_TRY(grow(c.mov_l, mov_tot * sizeof(float)));
+    if (q_tot) WR_TRY(grow(c.movq, q_tot));
     WR_TRY(grow(c.ref_l, ref_tot * es));
     if (mask_ref) WR_TRY(grow(c.mref_l, ref_tot));
     if (mask_mov) WR_TRY(grow(c.mmov_l, mov_tot * sizeof(float)));
@@ -521,9 +525,17 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     auto build_level = [&](int L) -> int {
         const LevelGeom& g = geo[L];
         const OutLayout padded = padded_layout(g.nx, g.ny);
-        void* mov_l = c.mov_l.as<char>() + mov_off[L] * es;
-        resample2(mov, et, 1, 0, mov_l, om_level, 1.0, down[L], tw, ti, nz, tmp, padded, side);
-        launch_pad_volume(mov_l, static_cast<int>(es), g.nx, g.ny, nz, side);
+        // the moving level image as float32, padded: what the float32 LK pass
+        // gathers from; with 16-bit storage the pass reads its records instead,
+        // made straight from the input where the level is the input (level 0)
+        const bool direct = st != ST_F32 && down[L].tx.identity && down[L].ty.identity;
+        float* mov_l = c.mov_l.as<float>() + mov_off[L];
+        if (!direct) {
+            resample2(mov, et, 1, 0, mov_l, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
+            launch_pad_volume(mov_l, 4, g.nx, g.ny, nz, side);
+        }
+        if (direct) launch_moving_records_input(mov, et, g.nx, g.ny, nz, st, c.movq.as<char>() + q_off[L], side);
+        else if (st != ST_F32) launch_moving_records(mov_l, g.nx, g.ny, nz, st, c.movq.as<char>() + q_off[L], side);
         if (L == 0 && dtype == WR_F32 && st == ST_F32) {
             ref_lv[L] = ref;
         } else {
@@ -577,10 +589,12 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
                          static_cast<double>(nv), s);
 
         if (L - 2 >= 0) WR_TRY(build_level(L - 2));
+
         WR_HIP_CHECK(hipStreamWaitEvent(s, c.lvl_events[L], 0));
         LkBuffers b;
         b.store = st;
-        b.mov = c.mov_l.as<char>() + mov_off[L] * es;
+        b.mov = c.mov_l.as<float>() + mov_off[L];
+        b.movq = st != ST_F32 ? c.movq.as<char>() + q_off[L] : nullptr;
         b.ref = ref_lv[L];
         b.mref = mask_ref ? c.mref_l.as<uint8_t>() + ref_off[L] : nullptr;
         b.mmov = mask_mov ? c.mmov_l.as<float>() + mov_off[L] : nullptr;

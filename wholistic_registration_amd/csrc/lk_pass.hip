@@ -22,13 +22,24 @@
 // tiles) with z fastest, so z-neighbour tiles, which share moving-image
 // slices, run on the same XCD and hit its L2.
 //
-// The kernel is VALU-bound (a wave64 FP32 instruction per 4 clocks per SIMD;
-// packed FP32 is not used), so the per-voxel instruction count is what is
-// minimised: the moving image rows are padded by kPad replicated voxels at
-// both x ends so every clamped tap row is one plain 16- or 8-byte load; row
-// addresses are 32-bit byte offsets from a workgroup-uniform base; the CP
-// interpolation weights of the tile's rows are formed once in LDS; the six
-// gradient samples take a branch-free fast path away from the faces.
+// The moving image is read by per-lane gathers around the deformed
+// coordinate, and their cost is set by the vector-memory pipeline (address
+// and data path of the texture units, ~80 % busy), not by HBM bytes: removing
+// a quarter of the gather instructions saves more than halving their bytes.
+// So the moving image is stored the way each storage needs the fewest gather
+// words per voxel:
+//   * float32: the level image itself, padded by kPad replicated voxels at
+//     both x ends (and rows / planes in y and z), so each of the 12 tap rows
+//     of the six clipped samples is one plain 16- or 8-byte load (32 taps);
+//     where no used sample is clipped the gradients come from the central
+//     rows' x differences (eval_inner), else from the general clipped forms.
+//   * bfloat16 / float16: records (V, Dx, Dy, Dz) per voxel, D_a the central
+//     difference with replicated edges, so the warp and all three gradients
+//     are one trilinear interpolation of the 8 corner records: four 16-byte
+//     loads (x pairs of the four (y, z) corner rows) instead of 12 gathers.
+// Row addresses are 32-bit byte offsets (float32) or 64-bit (records); the CP
+// interpolation weights of the tile's rows are formed once in LDS; both
+// gradient scalings by 1/2 are applied to the per-patch sums (exact).
 
 #include <cfloat>
 #include <cmath>
@@ -75,12 +86,30 @@ struct PVol {
 };
 
 typedef float f4v __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f4a __attribute__((ext_vector_type(4), aligned(16)));
 typedef float f2v __attribute__((ext_vector_type(2), aligned(4)));
 typedef unsigned u2v __attribute__((ext_vector_type(2), aligned(4)));
+typedef unsigned u4v __attribute__((ext_vector_type(4), aligned(8)));
 
-// Bytes per stored voxel of a level image of storage S (Store).
+// The moving level image as the records the pass interpolates
+// (moving_records): record (x, y, z) = (V, Dx, Dy, Dz) with
+// D_a = V(a + 1) - V(a - 1), edges replicated, for x in [0, nx], y in
+// [0, ny], z in [0, nz] (the last of each axis repeats its neighbour, so the
+// upper corner of a coordinate clamped to the last voxel is a plain load).
+// Rows of nx + 1 records of kRec bytes; planes of ny + 1 rows.
+struct QVol {
+    const unsigned char* base;
+    unsigned rowB;           // bytes per stored row
+    unsigned plane_rows;     // stored rows per plane (ny + 1)
+    float hx, hy, hz;        // n - 1
+};
+
+// Bytes per stored voxel of a level image of storage S (Store), and per
+// record of the moving image.
 template <int S>
 constexpr unsigned kEs = S == ST_F32 ? 4u : 2u;
+template <int S>
+constexpr unsigned kRec = 4u * kEs<S>;
 
 // The two 16-bit values of a word as float32 (bfloat16: the high half of a
 // float32; float16: converted).
@@ -145,6 +174,28 @@ __device__ __forceinline__ unsigned voxel_off(const PVol& v, int fx, int fy, int
     return __umul24(row, v.pitchB) + static_cast<unsigned>(fx) * kEs<S>;
 }
 
+struct Grad {
+    float w, gx, gy, gz;
+};
+
+struct Coord {
+    float cx, cy, cz;
+    Samp sx, sy, sz;
+};
+
+__device__ __forceinline__ Coord make_coord(float hx, float hy, float hz, float cx, float cy, float cz) {
+    Coord c;
+    c.cx = cx;
+    c.cy = cy;
+    c.cz = cz;
+    c.sx = split(cx, hx);
+    c.sy = split(cy, hy);
+    c.sz = split(cz, hz);
+    return c;
+}
+
+// ---- float32 storage: taps of the padded level image
+
 // Sample of the 4-point line l[0..3] = f(clamp(c0 - 1 + q)) at the clipped
 // coordinate `c` (inside [0, n-1]), c0 = floor of the clamped centre.
 // Whenever the fraction is non-zero the two taps lie inside the line; with a
@@ -174,10 +225,6 @@ __device__ __forceinline__ float line_diff(float l0, float l1, float l2, float l
            line_sample(l0, l1, l2, l3, c0, __builtin_amdgcn_fmed3f(c - 1.0f, 0.0f, hi));
 }
 
-struct Grad {
-    float w, gx, gy, gz;
-};
-
 // The 32 voxels around one deformed coordinate, as 4 x 16-byte rows holding
 // X0..X3 at (Y1|Y2, Z1|Z2) and 8 x 8-byte rows holding X1, X2 for the y and
 // z lines (see eval).
@@ -186,28 +233,12 @@ struct Taps {
     f2v e01, e02, e31, e32, f10, f20, f13, f23;
 };
 
-struct Coord {
-    float cx, cy, cz;
-    Samp sx, sy, sz;
-};
-
-__device__ __forceinline__ Coord make_coord(const PVol& v, float cx, float cy, float cz) {
-    Coord c;
-    c.cx = cx;
-    c.cy = cy;
-    c.cz = cz;
-    c.sx = split(cx, v.hx);
-    c.sy = split(cy, v.hy);
-    c.sz = split(cz, v.hz);
-    return c;
-}
-
 // The 12 tap rows of the gather at c, as offsets from that of voxel
 // (X0, Y0, Z0) = (fx - 1, fy - 1, fz - 1), which the pads keep inside the
 // allocation.  Where the clamped reference would repeat an edge row or
 // plane, the pads hold it.
 template <int S>
-__device__ __forceinline__ Taps gather(const PVol& v, const Coord& c) {
+__device__ __forceinline__ Taps gather_taps(const PVol& v, const Coord& c) {
     const unsigned py = v.pitchB, pz = v.planeB, e = kEs<S>;
     const unsigned a = v.origin + voxel_off<S>(v, c.sx.f - 1, c.sy.f, c.sz.f) - pz - py;
     Taps t;
@@ -233,7 +264,7 @@ __device__ __forceinline__ Taps gather(const PVol& v, const Coord& c) {
 // sample's taps lie on the 4-point lines c0-1..c0+2 (clamped), so the 32
 // distinct voxels are the 2x2x2 cube around the centre plus, per axis, the
 // two planes one step further out.
-__device__ __forceinline__ Grad eval(const PVol& v, const Coord& c, const Taps& q) {
+__device__ __forceinline__ Grad eval_taps(const PVol& v, const Coord& c, const Taps& q) {
     const Samp& sx = c.sx;
     const Samp& sy = c.sy;
     const Samp& sz = c.sz;
@@ -261,9 +292,128 @@ __device__ __forceinline__ Grad eval(const PVol& v, const Coord& c, const Taps& 
 
     Grad o;
     o.w = lerpf(k1, k2, sz.t);
-    o.gx = line_diff(g0, g1, g2, g3, sx.f, sx.t, c.cx, v.hx) * 0.5f;
-    o.gy = line_diff(h0, h1, h2, h3, sy.f, sy.t, c.cy, v.hy) * 0.5f;
-    o.gz = line_diff(k0, k1, k2, k3, sz.f, sz.t, c.cz, v.hz) * 0.5f;
+    o.gx = line_diff(g0, g1, g2, g3, sx.f, sx.t, c.cx, v.hx);
+    o.gy = line_diff(h0, h1, h2, h3, sy.f, sy.t, c.cy, v.hy);
+    o.gz = line_diff(k0, k1, k2, k3, sz.f, sz.t, c.cz, v.hz);
+    return o;
+}
+
+// The same quantities where no perturbed coordinate is clipped (1 <= c <=
+// n - 2 on every axis): T(c + e_a) and T(c - e_a) then share c's fractions,
+// and T(c + e_a) - T(c - e_a) is the interpolant of the central differences
+// V(a + 1) - V(a - 1) at c.  Each central row (y, z) gives its x sample and
+// its x difference; the y and z differences reuse the central rows' x
+// samples.  The sample w is formed in the same order as in eval, so it is
+// equal bit for bit.
+__device__ __forceinline__ Grad eval_inner(const Coord& c, const Taps& q) {
+    const float tx = c.sx.t, ty = c.sy.t, tz = c.sz.t;
+    // x samples and x differences of the central rows (y1|y2, z1|z2)
+    const float a11 = lerpf(q.q11.y, q.q11.z, tx), a21 = lerpf(q.q21.y, q.q21.z, tx);
+    const float a12 = lerpf(q.q12.y, q.q12.z, tx), a22 = lerpf(q.q22.y, q.q22.z, tx);
+    const float d11 = lerpf(q.q11.z - q.q11.x, q.q11.w - q.q11.y, tx);
+    const float d21 = lerpf(q.q21.z - q.q21.x, q.q21.w - q.q21.y, tx);
+    const float d12 = lerpf(q.q12.z - q.q12.x, q.q12.w - q.q12.y, tx);
+    const float d22 = lerpf(q.q22.z - q.q22.x, q.q22.w - q.q22.y, tx);
+    // x samples of the outer rows of the y and z lines
+    const float b01 = lerpf(q.e01.x, q.e01.y, tx), b02 = lerpf(q.e02.x, q.e02.y, tx);
+    const float b31 = lerpf(q.e31.x, q.e31.y, tx), b32 = lerpf(q.e32.x, q.e32.y, tx);
+    const float e10 = lerpf(q.f10.x, q.f10.y, tx), e20 = lerpf(q.f20.x, q.f20.y, tx);
+    const float e13 = lerpf(q.f13.x, q.f13.y, tx), e23 = lerpf(q.f23.x, q.f23.y, tx);
+    Grad o;
+    o.w = lerpf(lerpf(a11, a21, ty), lerpf(a12, a22, ty), tz);
+    o.gx = lerpf(lerpf(d11, d21, ty), lerpf(d12, d22, ty), tz);
+    o.gy = lerpf(lerpf(a21 - b01, b31 - a11, ty), lerpf(a22 - b02, b32 - a12, ty), tz);
+    o.gz = lerpf(lerpf(a12 - e10, e13 - a11, tz), lerpf(a22 - e20, e23 - a21, tz), ty);
+    return o;
+}
+
+// 1 <= c <= hi - 1: neither perturbed sample of the axis is clipped.
+__device__ __forceinline__ bool inner1(float c, float hi) { return c >= 1.0f && c <= hi - 1.0f; }
+
+// ---- 16-bit storage: records of the level image
+
+// The eight records around a deformed coordinate, [z][y][x] for the
+// floors f and f + 1 of its clamped position, each as (V, Dx, Dy, Dz).
+struct Corners {
+    f4v r[2][2][2];
+};
+
+// Two records of 16-bit storage (one 16-byte word) as float32.
+template <int S>
+__device__ __forceinline__ void unpack2(const u4v w, f4v& a, f4v& b) {
+    a.x = lo16<S>(w.x);
+    a.y = hi16<S>(w.x);
+    a.z = lo16<S>(w.y);
+    a.w = hi16<S>(w.y);
+    b.x = lo16<S>(w.z);
+    b.y = hi16<S>(w.z);
+    b.z = lo16<S>(w.w);
+    b.w = hi16<S>(w.w);
+}
+
+// Four loads of 32 bytes (float32 records: two 16-byte loads each) or of
+// 16 bytes (16-bit records): the x pairs of the four (y, z) corner rows.
+template <int S>
+__device__ __forceinline__ Corners gather_rec(const QVol& q, const Coord& c) {
+    const unsigned row = __umul24(static_cast<unsigned>(c.sz.f), q.plane_rows) + static_cast<unsigned>(c.sy.f);
+    const unsigned char* p0 = q.base + static_cast<size_t>(row) * q.rowB + static_cast<unsigned>(c.sx.f) * kRec<S>;
+    const size_t pz = static_cast<size_t>(q.rowB) * q.plane_rows;
+    Corners o;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const unsigned char* p = p0 + k * pz + j * static_cast<size_t>(q.rowB);
+            if constexpr (S == ST_F32) {
+                o.r[k][j][0] = *reinterpret_cast<const f4a*>(p);
+                o.r[k][j][1] = *reinterpret_cast<const f4a*>(p + 16);
+            } else {
+                unpack2<S>(*reinterpret_cast<const u4v*>(p), o.r[k][j][0], o.r[k][j][1]);
+            }
+        }
+    }
+    return o;
+}
+
+__device__ __forceinline__ f4v lerp4(const f4v a, const f4v b, float t) {
+    f4v o;
+    o.x = lerpf(a.x, b.x, t);
+    o.y = lerpf(a.y, b.y, t);
+    o.z = lerpf(a.z, b.z, t);
+    o.w = lerpf(a.w, b.w, t);
+    return o;
+}
+
+// Warp value and central-difference gradients at the deformed coordinate c
+// (calFlow3d:37-66,134-206): W = T(c) and 2 I_a = T(c + e_a) - T(c - e_a),
+// where T is the clamped trilinear interpolant and only the perturbed
+// coordinate is clipped to [0, n_a - 1].
+//
+// With c_a inside [0, n_a - 1] both perturbed samples share c's fraction
+// (or sit on the clipped face, where the replicated edge of D reproduces the
+// clip), so 2 I_a is the trilinear interpolant of D_a = V(a + 1) - V(a - 1)
+// (edges replicated) at the clamped c.  Outside, at distance d from the
+// volume along a, the clipped difference is (1 - d) times that interpolant
+// (taken at the face), and 0 from d >= 1 on.  W is formed x, then y, then z,
+// as the reference's order-1 interpolation at the clamped coordinate.
+__device__ __forceinline__ Grad eval_rec(const Coord& c, const Corners& q) {
+    const f4v y0 = lerp4(lerp4(q.r[0][0][0], q.r[0][0][1], c.sx.t), lerp4(q.r[0][1][0], q.r[0][1][1], c.sx.t),
+                         c.sy.t);
+    const f4v y1 = lerp4(lerp4(q.r[1][0][0], q.r[1][0][1], c.sx.t), lerp4(q.r[1][1][0], q.r[1][1][1], c.sx.t),
+                         c.sy.t);
+    const f4v v = lerp4(y0, y1, c.sz.t);
+    Grad o;
+    o.w = v.x;
+    o.gx = v.y;
+    o.gy = v.z;
+    o.gz = v.w;
+    const float ux = static_cast<float>(c.sx.f) + c.sx.t, uy = static_cast<float>(c.sy.f) + c.sy.t;
+    const float uz = static_cast<float>(c.sz.f) + c.sz.t;
+    if (__any(c.cx != ux || c.cy != uy || c.cz != uz)) {   // uniform: a lane samples outside the volume
+        o.gx *= fmaxf(1.0f - fabsf(c.cx - ux), 0.0f);
+        o.gy *= fmaxf(1.0f - fabsf(c.cy - uy), 0.0f);
+        o.gz *= fmaxf(1.0f - fabsf(c.cz - uz), 0.0f);
+    }
     return o;
 }
 
@@ -280,7 +430,8 @@ __device__ __forceinline__ float tri(const PVol& v, float cx, float cy, float cz
 }
 
 struct PassArgs {
-    PVol mov;
+    PVol mov;              // float32 storage: the padded level image
+    QVol q;                // 16-bit storage: its records
     PVol mmov;
     const void* ref;
     const uint8_t* mref;
@@ -302,6 +453,7 @@ struct PassArgs {
     long long blocks;
     long long ncp;
     float z_ratio;
+    float hx, hy, hz;      // level extents - 1
 };
 
 // Decode the XCD-major, z-fastest tile order.
@@ -425,7 +577,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
             my = lerpf(ay, byv, t);
             mz = lerpf(az, bzv, t);
         }
-        return make_coord(a.mov, xf + mx, static_cast<float>(yc) + my, zf + mz);
+        return make_coord(a.hx, a.hy, a.hz, xf + mx, static_cast<float>(yc) + my, zf + mz);
     };
 
     float d0 = 0.0f, d1 = 0.0f;        // residual of the two previous rows
@@ -438,17 +590,30 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
     if (wave * kWaveCols < a.cols) {   // uniform: the wave has tile columns
         for (int s = 0; s < nsteps; ++s) {
             const Coord cc = coord_of(s);
-            const Taps tc = gather<S>(a.mov, cc);
+            Taps tt;
+            Corners tr;
+            if constexpr (S == ST_F32) tt = gather_taps<ST_F32>(a.mov, cc);
+            else tr = gather_rec<S>(a.q, cc);
             // template / mask row of this step: uniform base, per-lane column
             const int vrow = z * plane + clampi(y0 - 1 + s, 0, a.ny - 1) * a.nx;
             const float rc = ld_ref<S>(a.ref, vrow + xc);
             const int yy = y0 - 1 + s;
-            const Grad g = eval(a.mov, cc, tc);
-            const float gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
-            const float d = rc - g.w;
             bool mk = false;
             if (MREF) mk = (a.mref + vrow)[xc] != 0;
             if (MMOV) mk = mk || tri(a.mmov, cc.cx, cc.cy, cc.cz) > 0.0f;
+            Grad g;
+            if constexpr (S == ST_F32) {
+                // rows outside [y0, yend) and the uncovered tail carry no gradient
+                const bool grad_row = yy >= y0 && yy < yend && yy < cover_y;
+                // the clipped forms are needed only where a gradient is used
+                const bool inner = !(grad_row && wx > 0 && !mk) ||
+                                   (inner1(cc.cx, a.hx) && inner1(cc.cy, a.hy) && inner1(cc.cz, a.hz));
+                g = __all(inner) ? eval_inner(cc, tt) : eval_taps(a.mov, cc, tt);
+            } else {
+                g = eval_rec(cc, tr);
+            }
+            const float gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
+            const float d = rc - g.w;
             if (s >= 2) {
                 const float vsum = d0 + d1 + d;
                 const float left = __shfl_up(vsum, 1, 64);
@@ -458,7 +623,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
                 if (pmask) it = 0.0f;
                 const float ie = owned ? it : 0.0f;
                 err += static_cast<double>(ie) * static_cast<double>(ie);
-                const float wyf = yo < cover_y ? static_cast<float>(1 + (yo == a.ny - 1 ? a.extra_y : 0)) : 0.0f;
+                // row yo's weight: 0 outside [y0, yend) and the uncovered tail
+                const float wyf = (yo >= y0 && yo < yend && yo < cover_y)
+                                      ? static_cast<float>(1 + (yo == a.ny - 1 ? a.extra_y : 0))
+                                      : 0.0f;
                 const float m = pmask ? 0.0f : wxf * wyf;
                 const float mgx = m * pgx, mgy = m * pgy, mgz = m * pgz;
                 acc0 += mgx * pgx;
@@ -473,11 +641,9 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
             }
             d0 = d1;
             d1 = d;
-            // rows outside [y0, yend) and the uncovered tail carry no gradient
-            const bool grad_row = yy >= y0 && yy < yend && yy < cover_y;
-            pgx = grad_row ? g.gx : 0.0f;
-            pgy = grad_row ? g.gy : 0.0f;
-            pgz = grad_row ? gz : 0.0f;
+            pgx = g.gx;
+            pgy = g.gy;
+            pgz = gz;
             pmask = mk;
         }
     }
@@ -514,6 +680,9 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
             const int u0 = pi * a.p;
             const int u1 = min(u0 + a.p, lastc);
             for (int u = u0; u < u1; ++u) sacc += static_cast<double>(s_red[k][u]);
+            // the gradients were accumulated as T(c + e) - T(c - e), twice
+            // the reference's (exact power-of-two scalings)
+            sacc *= k < 6 ? 0.25 : 0.5;
             a.sums[static_cast<size_t>(k) * a.ncp + (static_cast<size_t>(z) * a.ncy + by) * a.ncx + gi] =
                 static_cast<float>(sacc);
         }
@@ -537,12 +706,186 @@ PVol make_pvol(const void* base, unsigned es, int nx, int ny, int nz) {
     return v;
 }
 
+// Records of the moving level image (QVol) from its padded float32 image
+// (padded_layout): four consecutive records per thread, rows (y, z) of
+// nx + 1 records.  The pads hold the replicated edges (V(-1) = V(0),
+// V(n) = V(n + 1) = V(n - 1)), so every record, the repeated last one
+// included, is a plain stencil of the padded image.
+template <int S>
+__global__ __launch_bounds__(256) void k_make_records(PVol v, unsigned char* __restrict__ out, unsigned rowB,
+                                                      int xblocks) {
+    const unsigned b = blockIdx.x;
+    const unsigned row = b / static_cast<unsigned>(xblocks);
+    const int x = (static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x)) * 4;
+    if (x > v.nx) return;
+    const int y = static_cast<int>(row % static_cast<unsigned>(v.ny + 1));
+    const int z = static_cast<int>(row / static_cast<unsigned>(v.ny + 1));
+    const unsigned char* c = v.base + v.origin + voxel_off<ST_F32>(v, x, y, z);
+    // x - 1 .. x + 4 of the row (the padded row ends at nx + 3; x + 4 past it
+    // reads the next stored row, only for records that are not stored)
+    const f2v l = *reinterpret_cast<const f2v*>(c - 4);
+    const f4v m = *reinterpret_cast<const f4v*>(c);
+    const float r4 = *reinterpret_cast<const float*>(c + 16);
+    const f4v ym = *reinterpret_cast<const f4v*>(c - v.pitchB), yp = *reinterpret_cast<const f4v*>(c + v.pitchB);
+    const f4v zm = *reinterpret_cast<const f4v*>(c - v.planeB), zp = *reinterpret_cast<const f4v*>(c + v.planeB);
+    const float xv[6] = {l.x, m.x, m.y, m.z, m.w, r4};
+    const float ymv[4] = {ym.x, ym.y, ym.z, ym.w}, ypv[4] = {yp.x, yp.y, yp.z, yp.w};
+    const float zmv[4] = {zm.x, zm.y, zm.z, zm.w}, zpv[4] = {zp.x, zp.y, zp.z, zp.w};
+    unsigned char* o = out + static_cast<size_t>(row) * rowB + static_cast<size_t>(x) * kRec<S>;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (x + i > v.nx) break;
+        const float q0 = xv[i + 1], q1 = xv[i + 2] - xv[i], q2 = ypv[i] - ymv[i], q3 = zpv[i] - zmv[i];
+        if constexpr (S == ST_F32) {
+            f4a r;
+            r.x = q0;
+            r.y = q1;
+            r.z = q2;
+            r.w = q3;
+            reinterpret_cast<f4a*>(o)[i] = r;
+        } else {
+            auto h = [](float f) -> unsigned { return S == ST_BF16 ? to_bf16(f) : to_f16(f); };
+            u2v r;
+            r.x = h(q0) | (h(q1) << 16);
+            r.y = h(q2) | (h(q3) << 16);
+            reinterpret_cast<u2v*>(o)[i] = r;
+        }
+    }
+}
+
+// Records of a level that equals the input (level 0) straight from the
+// dense uint16 / float32 input, indices clamped instead of padded: four
+// consecutive records per thread, rows (y, z) of nx + 1 records.
+template <int IN_T>
+__device__ __forceinline__ float in_at(const void* in, size_t i) {
+    if constexpr (IN_T == E_U16) return static_cast<float>(static_cast<const uint16_t*>(in)[i]);
+    else return static_cast<const float*>(in)[i];
+}
+template <int IN_T>
+__device__ __forceinline__ void in_row4(const void* in, size_t i, float o[4]) {
+    if constexpr (IN_T == E_U16) {
+        const u2v w = *reinterpret_cast<const u2v*>(static_cast<const uint16_t*>(in) + i);
+        o[0] = static_cast<float>(w.x & 0xffffu);
+        o[1] = static_cast<float>(w.x >> 16);
+        o[2] = static_cast<float>(w.y & 0xffffu);
+        o[3] = static_cast<float>(w.y >> 16);
+    } else {
+        const f4v w = *reinterpret_cast<const f4v*>(static_cast<const float*>(in) + i);
+        o[0] = w.x;
+        o[1] = w.y;
+        o[2] = w.z;
+        o[3] = w.w;
+    }
+}
+
+template <int IN_T, int S>
+__global__ __launch_bounds__(256) void k_make_records_in(const void* __restrict__ in, int nx, int ny, int nz,
+                                                         unsigned char* __restrict__ out, unsigned rowB, int xblocks) {
+    const unsigned b = blockIdx.x;
+    const unsigned row = b / static_cast<unsigned>(xblocks);
+    const int x = (static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x)) * 4;
+    if (x > nx) return;
+    const int y = static_cast<int>(row % static_cast<unsigned>(ny + 1));
+    const int z = static_cast<int>(row / static_cast<unsigned>(ny + 1));
+    const int yc = min(y, ny - 1), zc = min(z, nz - 1);
+    const size_t plane = static_cast<size_t>(nx) * ny;
+    const size_t r0 = (static_cast<size_t>(zc) * ny + yc) * nx;
+    const size_t rym = r0 - (yc > 0 ? nx : 0), ryp = r0 + (yc < ny - 1 ? nx : 0);
+    const size_t rzm = r0 - (zc > 0 ? plane : 0), rzp = r0 + (zc < nz - 1 ? plane : 0);
+    float xv[6], ym[4], yp[4], zm[4], zp[4];
+    if (x >= 4 && x + 4 <= nx - 1) {   // interior of the row: vector loads
+        xv[0] = in_at<IN_T>(in, r0 + x - 1);
+        in_row4<IN_T>(in, r0 + x, xv + 1);
+        xv[5] = in_at<IN_T>(in, r0 + x + 4);
+        in_row4<IN_T>(in, rym + x, ym);
+        in_row4<IN_T>(in, ryp + x, yp);
+        in_row4<IN_T>(in, rzm + x, zm);
+        in_row4<IN_T>(in, rzp + x, zp);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) xv[i] = in_at<IN_T>(in, r0 + clampi(x - 1 + i, 0, nx - 1));
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int xc = min(x + i, nx - 1);
+            ym[i] = in_at<IN_T>(in, rym + xc);
+            yp[i] = in_at<IN_T>(in, ryp + xc);
+            zm[i] = in_at<IN_T>(in, rzm + xc);
+            zp[i] = in_at<IN_T>(in, rzp + xc);
+        }
+    }
+    unsigned char* o = out + static_cast<size_t>(row) * rowB + static_cast<size_t>(x) * kRec<S>;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (x + i > nx) break;
+        // the record past the last voxel repeats the last one
+        const int j = x + i < nx ? i : i - 1;
+        const float q0 = xv[j + 1], q1 = xv[j + 2] - xv[j], q2 = yp[j] - ym[j], q3 = zp[j] - zm[j];
+        if constexpr (S == ST_F32) {
+            f4a r;
+            r.x = q0;
+            r.y = q1;
+            r.z = q2;
+            r.w = q3;
+            reinterpret_cast<f4a*>(o)[i] = r;
+        } else {
+            auto h = [](float f) -> unsigned { return S == ST_BF16 ? to_bf16(f) : to_f16(f); };
+            u2v r;
+            r.x = h(q0) | (h(q1) << 16);
+            r.y = h(q2) | (h(q3) << 16);
+            reinterpret_cast<u2v*>(o)[i] = r;
+        }
+    }
+}
+
+QVol make_qvol(const void* base, int store, int nx, int ny, int nz) {
+    QVol q;
+    q.base = static_cast<const unsigned char*>(base);
+    q.rowB = static_cast<unsigned>(nx + 1) * static_cast<unsigned>(4 * store_bytes(store));
+    q.plane_rows = static_cast<unsigned>(ny + 1);
+    q.hx = static_cast<float>(nx - 1);
+    q.hy = static_cast<float>(ny - 1);
+    q.hz = static_cast<float>(nz - 1);
+    return q;
+}
+
 }  // namespace
+
+size_t moving_records_bytes(int nx, int ny, int nz, int store) {
+    return static_cast<size_t>(nx + 1) * (ny + 1) * (nz + 1) * 4 * store_bytes(store);
+}
+
+void launch_moving_records(const float* mov_padded, int nx, int ny, int nz, int store, void* out, hipStream_t s) {
+    const PVol v = make_pvol(mov_padded, 4u, nx, ny, nz);
+    const QVol q = make_qvol(out, store, nx, ny, nz);
+    const int xblocks = (nx + 1 + 1023) / 1024;   // 4 records per thread
+    const dim3 grid(static_cast<unsigned>(static_cast<long long>(ny + 1) * (nz + 1) * xblocks));
+    unsigned char* o = static_cast<unsigned char*>(out);
+    if (store == ST_F32) hipLaunchKernelGGL(k_make_records<ST_F32>, grid, dim3(256), 0, s, v, o, q.rowB, xblocks);
+    else if (store == ST_BF16) hipLaunchKernelGGL(k_make_records<ST_BF16>, grid, dim3(256), 0, s, v, o, q.rowB, xblocks);
+    else hipLaunchKernelGGL(k_make_records<ST_F16>, grid, dim3(256), 0, s, v, o, q.rowB, xblocks);
+}
+
+void launch_moving_records_input(const void* in, Elem t, int nx, int ny, int nz, int store, void* out,
+                                 hipStream_t s) {
+    const QVol q = make_qvol(out, store, nx, ny, nz);
+    const int xblocks = (nx + 1 + 1023) / 1024;   // 4 records per thread
+    const dim3 grid(static_cast<unsigned>(static_cast<long long>(ny + 1) * (nz + 1) * xblocks));
+    unsigned char* o = static_cast<unsigned char*>(out);
+    using Fn = void (*)(const void*, int, int, int, unsigned char*, unsigned, int);
+    static const Fn table[2][3] = {
+        {k_make_records_in<E_F32, ST_F32>, k_make_records_in<E_F32, ST_BF16>, k_make_records_in<E_F32, ST_F16>},
+        {k_make_records_in<E_U16, ST_F32>, k_make_records_in<E_U16, ST_BF16>, k_make_records_in<E_U16, ST_F16>}};
+    hipLaunchKernelGGL(table[t == E_U16 ? 1 : 0][store], grid, dim3(256), 0, s, in, nx, ny, nz, o, q.rowB, xblocks);
+}
 
 long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool dense,
                          int check_parts, hipStream_t s) {
     PassArgs a;
-    a.mov = make_pvol(b.mov, static_cast<unsigned>(store_bytes(b.store)), g.nx, g.ny, g.nz);
+    a.mov = make_pvol(b.mov, 4u, g.nx, g.ny, g.nz);
+    a.q = make_qvol(b.movq, b.store, g.nx, g.ny, g.nz);
+    a.hx = static_cast<float>(g.nx - 1);
+    a.hy = static_cast<float>(g.ny - 1);
+    a.hz = static_cast<float>(g.nz - 1);
     a.mmov = make_pvol(b.mmov ? static_cast<const void*>(b.mmov) : b.mov, 4u, g.nx, g.ny, g.nz);
     a.ref = b.ref;
     a.mref = b.mref;

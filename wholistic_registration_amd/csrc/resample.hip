@@ -22,6 +22,7 @@
 #include <cmath>
 #include <type_traits>
 
+#include "lk_common.h"
 #include "wr_internal.h"
 
 namespace wr {
@@ -197,16 +198,6 @@ __device__ __forceinline__ double scaled(double v, double divisor) {
     if (divisor == 1.0) return v;
     if (__builtin_amdgcn_frexp_mant(divisor) == 0.5) return v * (1.0 / divisor);
     return v / divisor;
-}
-
-// 16-bit storage of a level image: the float32 value rounded to nearest even.
-__device__ __forceinline__ uint16_t to_bf16(float f) {
-    const unsigned u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);   // NaN stays NaN
-    return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ __forceinline__ uint16_t to_f16(float f) {
-    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(f));
 }
 
 __device__ __forceinline__ void store_out(void* out, size_t i, double v, int om, double divisor) {

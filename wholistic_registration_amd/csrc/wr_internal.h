@@ -191,9 +191,23 @@ struct LkTile {
 
 LkTile plan_tile(const LevelGeom& g, int np_x);
 
+// Records of the moving level image the LK pass interpolates: record
+// (x, y, z) = (V, V(x+1) - V(x-1), V(y+1) - V(y-1), V(z+1) - V(z-1)) of the
+// level image V (edges replicated), in the storage `store` (4 float32 or
+// 4 16-bit values), for x in [0, nx], y in [0, ny], z in [0, nz] (the last
+// of each axis repeats its neighbour): (nz + 1) planes of (ny + 1) rows of
+// nx + 1 records.  Built from the padded float32 level image.
+size_t moving_records_bytes(int nx, int ny, int nz, int store);
+void launch_moving_records(const float* mov_padded, int nx, int ny, int nz, int store, void* out, hipStream_t s);
+// The same records of a level equal to the input (level 0), straight from the
+// dense float32 / uint16 input (t = E_F32 or E_U16).
+void launch_moving_records_input(const void* in, Elem t, int nx, int ny, int nz, int store, void* out,
+                                 hipStream_t s);
+
 struct LkBuffers {
-    int store;               // element type of mov and ref (Store)
-    const void* mov;         // level moving image, padded volume (padded_layout), allocation start
+    int store;               // element type of ref and of movq's records (Store)
+    const void* mov;         // level moving image, float32 padded volume (padded_layout), allocation start
+    const void* movq;        // its records (launch_moving_records)
     const void* ref;         // level template
     const uint8_t* mref;     // level mask_ref (> 0 after resampling) or null
     const float* mmov;       // level mask_mov (continuous, padded like mov, allocation start) or null
@@ -381,7 +395,7 @@ void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int hs, int
 
 struct Context {
     int device = 0;
-    DevBuf mov_l, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;
+    DevBuf mov_l, movq, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;   // movq: records of the moving levels
     DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts, mdpart;
     DevBuf tab_w, tab_i, pen;
     DevBuf msel, mlab, mcnt, mtmp;       // mask statistics, component labels / areas, byte mask
