@@ -71,8 +71,12 @@ typedef struct wr_params {
                                their own single-workgroup kernels (testing)   */
     int precision;          /* storage of the pyramid level images the LK
                                iterations read: 0 = float32 (the reference's
-                               values), 1 = bfloat16, 2 = float16; every sum,
-                               interpolation and solve stays float32/float64 */
+                               values), 1 = bfloat16, 2 = float16 (moving
+                               image and its central differences); every
+                               interpolation and sum stays float32; the
+                               per-patch solve is float64 with precision 0,
+                               float32 with 1 and 2 except for systems whose
+                               condition number exceeds 1e6 (float64) */
     int reuse_template;     /* 1 = `ref` and mask_ref hold the same data as in
                                the previous wr_get_motion on this context, with
                                the same ref and mask_ref pointers, size, dtype,
@@ -313,6 +317,9 @@ typedef struct wr_stats {
     int lk_blocks[WR_MAX_LEVELS]; /* workgroups per LK pass                  */
     int lk_exec[WR_MAX_LEVELS];   /* LK passes that did work (timing = 1)   */
     double lk_ms[WR_MAX_LEVELS];  /* summed duration of those passes, ms    */
+    int solve_fp64[WR_MAX_LEVELS];/* with 16-bit storage (float32 solves):
+                                     systems solved in float64 because their
+                                     condition number exceeded 1e6            */
 } wr_stats;
 int wr_get_stats(wr_ctx* ctx, wr_stats* out);
 

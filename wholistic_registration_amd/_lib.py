@@ -64,6 +64,7 @@ class WrStats(ctypes.Structure):
         ("lk_blocks", ctypes.c_int * WR_MAX_LEVELS),
         ("lk_exec", ctypes.c_int * WR_MAX_LEVELS),
         ("lk_ms", ctypes.c_double * WR_MAX_LEVELS),
+        ("solve_fp64", ctypes.c_int * WR_MAX_LEVELS),
     ]
 
 

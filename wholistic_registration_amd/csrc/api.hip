@@ -492,9 +492,9 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     WR_TRY(grow(c.mdpart, mdpart_slots(static_cast<long long>(ncp_max)) * sizeof(float)));
     WR_TRY(grow(c.ctl, sizeof(Ctl)));
     WR_TRY(grow(c.log, static_cast<size_t>(layer + 1) * iter * 4 * sizeof(double)));
-    WR_TRY(grow(c.counts, static_cast<size_t>(layer + 1) * 3 * sizeof(int)));
+    WR_TRY(grow(c.counts, static_cast<size_t>(layer + 1) * 4 * sizeof(int)));
     fill(c.log.as<float>(), static_cast<size_t>(layer + 1) * iter * 8, 0.0f, s);
-    fill(c.counts.as<float>(), static_cast<size_t>(layer + 1) * 3, 0.0f, s);
+    fill(c.counts.as<float>(), static_cast<size_t>(layer + 1) * 4, 0.0f, s);
 
     const bool timing = prm->timing != 0;
     if (timing) {
@@ -606,7 +606,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         b.penpart = c.penpart.as<double>();
         b.ctl = c.ctl.as<Ctl>();
         b.log = c.log.as<double>() + static_cast<size_t>(L) * iter * 4;
-        b.counts = c.counts.as<int>() + L * 3;
+        b.counts = c.counts.as<int>() + L * 4;
         b.mdpart = c.mdpart.as<float>();
         b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
         b.cp_start = cp_start ? 1 : 0;
@@ -626,13 +626,13 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
 
     // ---- logs back to the host
     std::vector<double> hlog(static_cast<size_t>(layer + 1) * iter * 4);
-    std::vector<int> hcnt(static_cast<size_t>(layer + 1) * 3);
+    std::vector<int> hcnt(static_cast<size_t>(layer + 1) * 4);
     WR_HIP_CHECK(hipMemcpyAsync(hlog.data(), c.log.p, hlog.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     WR_HIP_CHECK(hipMemcpyAsync(hcnt.data(), c.counts.p, hcnt.size() * sizeof(int), hipMemcpyDeviceToHost, s));
     WR_HIP_CHECK(hipStreamSynchronize(s));
     if (timing) {
         for (int L = 0; L <= layer; ++L) {
-            const int n = std::min(hcnt[L * 3 + 0], iter);
+            const int n = std::min(hcnt[L * 4 + 0], iter);
             double tot = 0.0;
             for (int k = 0; k < n; ++k) {
                 float ms = 0.0f;
@@ -644,11 +644,12 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             c.stats.lk_ms[L] = tot;
         }
     }
+    for (int L = 0; L <= layer; ++L) c.stats.solve_fp64[L] = hcnt[L * 4 + 3];
     if (log) {
         for (int L = 0; L <= layer; ++L) {
-            log->records[L] = hcnt[L * 3 + 0];
-            log->updates[L] = hcnt[L * 3 + 1];
-            log->nans[L] = hcnt[L * 3 + 2];
+            log->records[L] = hcnt[L * 4 + 0];
+            log->updates[L] = hcnt[L * 4 + 1];
+            log->nans[L] = hcnt[L * 4 + 2];
             for (int k = 0; k < iter; ++k) {
                 const double* row = &hlog[(static_cast<size_t>(L) * iter + k) * 4];
                 const size_t o = static_cast<size_t>(L) * iter + k;

@@ -47,6 +47,7 @@ __global__ __launch_bounds__(256) void k_ctl_reset(Ctl* c, int iter, int frec, i
     c->rec = 0;
     c->upd = 0;
     c->nan = 0;
+    c->solve64 = 0;
     c->iter = iter;
     c->frec = frec;
     c->fupd = fupd;
@@ -80,13 +81,20 @@ __device__ __forceinline__ void block_max_to(float md, float* __restrict__ dst) 
     if (threadIdx.x == 0) *dst = fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]));
 }
 
-// Regularised 3x3 solve per CP and the CP update (float64 solve).
+// Regularised 3x3 solve per CP and the CP update.  The solve is float64
+// (the reference's arithmetic); with solve32 (the 16-bit storage variants) it
+// is float32, and float64 only where the system's condition number exceeds
+// 1e6 (estimated in the Frobenius norm, kappa_F = |A|_F |adj A|_F / |det A|,
+// an upper bound of the 2-norm condition number; a zero or non-finite
+// determinant also takes the float64 path).
 // With fuse_err, every workgroup first evaluates iteration k's error terms and
 // error-based stop rule from the LK launch's partials (the same result in
 // all of them; workgroup 0 records it) instead of a separate k_err_finish.
+constexpr float kSolve32MaxCond = 1e6f;
+
 __global__ __launch_bounds__(256) void k_solve(Ctl* __restrict__ c, const float* __restrict__ sums,
                                                 const float* __restrict__ dn, float* __restrict__ cp,
-                                                long long ncp, double mov_range, double z_ratio,
+                                                long long ncp, double mov_range, double z_ratio, int solve32,
                                                 float* __restrict__ mdpart, int k, int fuse_err,
                                                 const double* __restrict__ errpart, long long nerr,
                                                 const double* __restrict__ penpart, long long npen,
@@ -109,6 +117,31 @@ __global__ __launch_bounds__(256) void k_solve(Ctl* __restrict__ c, const float*
     float md = 0.0f;
     if (i < ncp) {
         const double s_pen = c->s_pen;
+        double v[3];
+        bool solved = false;
+        if (solve32) {
+            const float sp = static_cast<float>(s_pen);
+            const float xx = sums[i] + sp, xy = sums[i + ncp], xz = sums[i + 2 * ncp];
+            const float yy = sums[i + 3 * ncp] + sp, yz = sums[i + 4 * ncp], zz = sums[i + 5 * ncp] + sp;
+            const float xt = sums[i + 6 * ncp] + sp * dn[i];
+            const float yt = sums[i + 7 * ncp] + sp * dn[i + ncp];
+            const float zt = sums[i + 8 * ncp] + sp * dn[i + 2 * ncp];
+            const float m11 = yy * zz - yz * yz, m12 = -(xy * zz - yz * xz), m13 = xy * yz - yy * xz;
+            const float m22 = xx * zz - xz * xz, m23 = -(xx * yz - xy * xz), m33 = xx * yy - xy * xy;
+            const float det = xx * m11 + xy * m12 + xz * m13;
+            const float na = sqrtf(xx * xx + yy * yy + zz * zz + 2.0f * (xy * xy + xz * xz + yz * yz));
+            const float nadj = sqrtf(m11 * m11 + m22 * m22 + m33 * m33 + 2.0f * (m12 * m12 + m13 * m13 + m23 * m23));
+            const float cond = na * nadj / fabsf(det);
+            if (cond <= kSolve32MaxCond) {   // false for NaN, infinite or zero-determinant systems
+                v[0] = (m11 * xt + m12 * yt + m13 * zt) / det;
+                v[1] = (m12 * xt + m22 * yt + m23 * zt) / det;
+                v[2] = (m13 * xt + m23 * yt + m33 * zt) / det;
+                solved = true;
+            } else {
+                atomicAdd(&c->solve64, 1);
+            }
+        }
+        if (!solved) {
         const double xx = static_cast<double>(sums[i]) + s_pen;
         const double xy = sums[i + ncp];
         const double xz = sums[i + 2 * ncp];
@@ -125,10 +158,10 @@ __global__ __launch_bounds__(256) void k_solve(Ctl* __restrict__ c, const float*
         const double m22 = xx * zz - xz * xz;
         const double m23 = -(xx * yz - xy * xz);
         const double m33 = xx * yy - xy * xy;
-        double v[3];
         v[0] = (m11 * xt + m12 * yt + m13 * zt) / det;
         v[1] = (m12 * xt + m22 * yt + m23 * zt) / det;
         v[2] = (m13 * xt + m23 * yt + m33 * zt) / det;
+        }
         int bad = 0;
         for (int q = 0; q < 3; ++q)
             if (isnan(v[q])) {
@@ -283,7 +316,10 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
     const unsigned b = blockIdx.x;
     const unsigned row = b / static_cast<unsigned>(xblocks);
     const int x = static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x);
-    if (b == 0 && threadIdx.x == 0) counts[2] = c->nan;
+    if (b == 0 && threadIdx.x == 0) {
+        counts[2] = c->nan;
+        counts[3] = c->solve64;
+    }
     if (x >= nx) return;
     const int y = static_cast<int>(row % static_cast<unsigned>(ny));
     const int z = static_cast<int>(row / static_cast<unsigned>(ny));
@@ -386,7 +422,8 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     const long long np = launch_lk_pass(g, t, b, k, dense, check_now, s);
     if (b.events) (void)hipEventRecord(b.events[2 * k + 1], s);
     if (!fuse_err) k_err_finish<<<1, 1024, 0, s>>>(b.ctl, k, b.errpart, nerr, b.penpart, np, b.log, b.counts);
-    k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d, b.mdpart, k,
+    k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d,
+                                b.store != ST_F32 ? 1 : 0, b.mdpart, k,
                                 fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts);
     if (dense) {
         const size_t lds = dense_lds_bytes(g);

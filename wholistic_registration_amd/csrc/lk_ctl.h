@@ -155,6 +155,7 @@ __device__ __forceinline__ void check_record(Ctl* __restrict__ c, int k, float m
     c->upd = k + 1;
     counts[1] = k + 1;
     counts[2] = c->nan;
+    counts[3] = c->solve64;
     if (stop) c->done = 1;
 }
 

@@ -158,6 +158,7 @@ struct Ctl {
     int rec;             // error records so far (len(currentError))
     int upd;             // updates so far (len(max_diff_motion))
     int nan;             // NaN components zeroed by the solve
+    int solve64;         // float32-solve systems that took the float64 path (condition > 1e6)
     int iter;            // option["iter"]
     int frec, fupd;      // forced schedule of this level, -1 = natural
     unsigned maxdiff;    // float bits of max |delta motion| of the current update
@@ -219,7 +220,7 @@ struct LkBuffers {
     double* penpart;         // per-workgroup sum of |D|^2 (one per 64 CPs at most)
     Ctl* ctl;
     double* log;             // [level][iter][4]: diff, pen, cur, maxdiff
-    int* counts;             // [level][3]: rec, upd, nan
+    int* counts;             // [level][4]: rec, upd, nan, float64 fallbacks of the float32 solve
     hipEvent_t* events;      // null, or 2*iter events of this level bracketing the LK passes
     float* mdpart;           // per-workgroup max |delta motion| partials
     int cp_start = 0;        // 1: the level starts from the CP field itself (no dense field m0):
