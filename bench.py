@@ -113,11 +113,94 @@ def algorithmic_bytes(w, records, s_in, s_p=4, s_m=1):
     return tot
 
 
+def _cpu_iteration_seconds(args):
+    """Seconds of one level-0 iteration (error record, gradients, full-grid
+    window sums, solve, update) of the reference's CPU path on a crop: the
+    oracle in the reference's own form (scipy.ndimage.correlate with a
+    ones((2r+1, 2r+1, 1)) kernel and map_coordinates, its dtype flow) run with
+    the schedule of one record per level, then with one more level-0
+    iteration; the difference isolates that iteration.  Runs in a worker
+    process of the all-cores leg as well."""
+    mov, ref, opt, layer = args
+    from oracle import lk_oracle as O
+    one = {L: (1, 0) for L in range(layer + 1)}
+    two = dict(one)
+    two[0] = (2, 1)
+    t = []
+    for sched in (one, two):
+        t0 = time.perf_counter()
+        O.get_motion(mov, ref, dict(opt), sums="correlate", schedule=sched, keep_images=False)
+        t.append(time.perf_counter() - t0)
+    return t[1] - t[0], t[0] + t[1]
+
+
+def cpu_baseline_measure(tmpl, mov, mask, w, records, updates, crop=(2, 320, 640), workers=0):
+    """The reference's CPU path on the host cores, extrapolated.  A full
+    registration of the workload takes hours on a CPU (at r = 16 every
+    iteration correlates 9 full-resolution products with a 33 x 33 kernel), so
+    one level-0 iteration is timed on a z-slab crop of the same pair
+    (_cpu_iteration_seconds) and the registration time is extrapolated as
+    seconds per voxel-iteration x sum over levels of (updates executed x level
+    voxels), with the schedule the GPU run executed.  Error-only records, the
+    pyramid and the final warp are left out, so the estimate favours the CPU.
+    One thread (scipy.ndimage and numpy elementwise work are single-threaded),
+    then `workers` processes at once (the all-cores leg: independent
+    registrations, aggregate throughput)."""
+    import multiprocessing as mp
+    import platform
+
+    from oracle import lk_oracle as O
+    nz, ny, nx = w["zyx"]
+    cz, cy, cx = crop
+    z0, y0, x0 = (nz - cz) // 2, (ny - cy) // 2, (nx - cx) // 2
+    sl = (slice(z0, z0 + cz), slice(y0, y0 + cy), slice(x0, x0 + cx))
+    ref = np.ascontiguousarray(tmpl[sl].float().cpu().numpy().transpose(2, 1, 0))
+    mv = np.ascontiguousarray(mov[sl].float().cpu().numpy().transpose(2, 1, 0))
+    mk = (mask[sl].cpu().numpy() > 0).transpose(2, 1, 0) if mask is not None else np.zeros(ref.shape, bool)
+    sp = O.smooth_penalty_factor(ref, mk) * RAW_PENALTY
+    opt = dict(layer=w["layer"], iter=w["iter"], r=w["r"], zRatio=1.0, movRange=5.0, tol=1e-3,
+               save_ite=10 ** 9, mask_ref=mk, mask_mov=np.zeros(ref.shape, bool), smoothPenalty=sp)
+    n0_crop = cx * cy * cz
+    vox_iters = sum(updates[L] * int(nx / 2 ** L) * int(ny / 2 ** L) * nz for L in range(w["layer"] + 1))
+    job = (mv, ref, opt, w["layer"])
+    t_iter, t_all = _cpu_iteration_seconds(job)
+    per_vox_iter = t_iter / n0_crop
+    t_reg = per_vox_iter * vox_iters
+    n0 = nx * ny * nz
+    try:
+        model = next((ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")),
+                     platform.processor())
+    except OSError:
+        model = platform.processor()
+    nproc = os.cpu_count() or 1
+    out = {"value": n0 / t_reg, "unit": "registered voxels/s", "cores": 1, "kind": "port",
+           "extrapolated": True, "seconds_per_registration": t_reg, "seconds_per_voxel_iteration": per_vox_iter,
+           "cpu_model": model, "nproc": nproc,
+           "sample": f"reference CPU path (the oracle in the reference's form: full-grid scipy correlate window "
+                     f"sums, map_coordinates warps, its dtype flow), one level-0 iteration isolated on the centre "
+                     f"crop {cz}x{cy}x{cx} (z,y,x) of the same pair ({t_all:.1f} s of CPU work), extrapolated "
+                     f"to {vox_iters / 1e6:.0f}M voxel-iterations (the GPU run's updates per level "
+                     f"{dict((f'L{L}', updates[L]) for L in range(w['layer'], -1, -1))}); one thread"}
+    workers = workers or min(nproc, 16)
+    if workers > 1:
+        ctx = mp.get_context("spawn")   # fresh interpreters: the parent holds a GPU context
+        t0 = time.perf_counter()
+        with ctx.Pool(workers) as pool:
+            res = pool.map(_cpu_iteration_seconds, [job] * workers)
+        wall = time.perf_counter() - t0
+        agg = sum(n0 / (ti / n0_crop * vox_iters) for ti, _ in res)
+        out["all_cores"] = {"value": agg, "unit": "registered voxels/s", "cores": workers, "processes": workers,
+                            "extrapolated": True, "wall_s": wall,
+                            "sample": f"{workers} independent processes running the same timed sample at once "
+                                      f"(aggregate of independent registrations)"}
+    return out
+
+
 def crop_checks(tmpl, mov, mask, w, crop, want_cpu, want_epe, precisions=()):
     """On the centre crop of the pair: time the oracle in the reference's
-    dtype flow (the CPU baseline, one thread: scipy.ndimage is
-    single-threaded), and compare the GPU field with the all-float64 oracle
-    on the same crop (the EPE <= 0.1 voxel gate)."""
+    dtype flow with direct per-patch window sums (one thread), and compare
+    the GPU field with the all-float64 oracle on the same crop (the EPE <= 0.1
+    voxel gate)."""
     import torch
 
     from oracle import lk_oracle as O
@@ -141,11 +224,11 @@ def crop_checks(tmpl, mov, mask, w, crop, want_cpu, want_epe, precisions=()):
         m, _, _, _ = O.get_motion(mv, ref, opt, keep_images=False)
         O.correct_motion(mv, m)
         dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {
+        out["cpu_oracle_window_sums"] = {
             "value": nvox / dt, "unit": "registered voxels/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (numpy/scipy restatement of the reference, its dtype flow, direct "
-                      f"per-patch window sums) on the centre crop {cz}x{cy}x{cx} (z,y,x) of the "
-                      f"same pair, same parameters, one thread, {dt:.1f} s"}
+            "sample": f"oracle with direct per-patch window sums instead of the reference's full-grid "
+                      f"correlates (faster than the reference's form) on the centre crop {cz}x{cy}x{cx} "
+                      f"(z,y,x) of the same pair, same parameters, one thread, {dt:.1f} s"}
     if want_epe:
         m64, _, _, _ = O.get_motion(mv, ref, opt, mode="fp64", keep_images=False)
         dev = tmpl.device
@@ -331,18 +414,27 @@ def mask_measure(vol, crop, reps=3):
     }
 
 
-def learned_measure(n_frames=6, reps=5):
-    """Plain FP32 pyramid LK against the learned start (+ FP32 and + BF16
-    level images) on held-out frame pairs of the 64 x 512 x 1024 uint16
-    frame geometry (template seed 90000, frames 90001..; the network never
-    saw them).  Per frame: median wall time of getMotion + correctMotion over
-    ``reps`` runs (the learned path also computes the moving frame's level-3
-    image and runs the network; the template's level-3 image is computed once
-    per template), endpoint error of every field against the plain FP32 field
-    (layer 4, r 8, 15 iterations: the gate's yardstick, equal to the float64
-    oracle within 1e-3 voxel at oracle sizes) and against the generating
-    field, and whether the fallback rule (keep the plain field when its final
-    residual is lower) would fire."""
+# Held-out pairs of the learned start, declared in advance: templates 90000 +
+# 1000 k (k < 16) of the 64 x 512 x 1024 uint16 frame geometry, frames
+# template + 1 .. template + 16.  Training used templates 10000 + 1000 i
+# (i < 64) and validation 20000, 21000: none of these.
+HELD_OUT_TEMPLATES = tuple(90000 + 1000 * k for k in range(16))
+HELD_OUT_FRAMES = 16
+
+
+def learned_measure(n_templates=16, per_template=HELD_OUT_FRAMES, reps=3):
+    """Plain FP32 pyramid LK against the learned start on the declared
+    held-out pairs (up to 16 x 16 = 256).  Per pair: median wall time of
+    getMotion + correctMotion over ``reps`` runs for plain FP32 and the
+    learned start with FP32 and BF16 level images (the learned path also
+    computes the moving frame's level-3 image and runs the network; the
+    template's level-3 image once per template); one timed run of the learned
+    BF16 start with the fallback as specified (the plain pyramid method also
+    runs and its field is kept when its final residual is lower), which is
+    what the learned start costs when every frame is checked; endpoint error
+    of every field against the plain FP32 field (layer 4, r 8, 15 iterations:
+    the gate's yardstick) and against the generating field; the same
+    schedule from zero (the method without the network)."""
     import torch
 
     from wholistic_registration_amd import device as D
@@ -350,68 +442,90 @@ def learned_measure(n_frames=6, reps=5):
     from wholistic_registration_amd import train_init as T
     dev = torch.device("cuda", 0)
     net = Lr.InitNet.load()
-    clean, tmpl = T.make_template(90000, dev)
-    sp = T.penalty(tmpl)
-    base = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=sp, tol=1e-3)
-    ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
-    nz, ny, nx = tmpl.shape
+    nz, ny, nx = T.FRAME
     n0 = nz * ny * nx
     warped = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
     motion = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device=dev)
 
-    def plain(mov, layer, it, precision="fp32"):
-        r = D.get_motion(mov, tmpl, layer=layer, iter=it, precision=precision, motion_out=motion, **base)
-        D.correct_motion(mov, motion, out=warped)
-        return r
+    def timed(fn, n):
+        fn()
+        ts = []
+        for _ in range(n):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return r, 1e3 * float(np.median(ts))
 
-    def learned(mov, precision):
-        r, _ = Lr.register_learned(mov, tmpl, net, ref_level=ref_l, precision=precision, motion_out=motion,
-                                   **Lr.SCHEDULE, **base)
-        D.correct_motion(mov, motion, out=warped)
-        return r
+    names = ("plain_fp32", "learned_fp32", "learned_bf16", "learned_bf16_fallback", "plain_fp32_same_schedule")
+    rows = {k: {"ms": [], "epe": [], "epe_truth": []} for k in names}
+    pairs, used_plain, frames_first = [], 0, []
+    for tseed in HELD_OUT_TEMPLATES[:n_templates]:
+        clean, tmpl = T.make_template(tseed, dev)
+        base = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=T.penalty(tmpl), tol=1e-3)
+        ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
 
-    variants = {
-        "plain_fp32": lambda mov: plain(mov, 4, 15),
-        "learned_fp32": lambda mov: learned(mov, "fp32"),
-        "learned_bf16": lambda mov: learned(mov, "bf16"),
-        "plain_fp32_same_schedule": lambda mov: plain(mov, 0, Lr.SCHEDULE["iter"]),
-    }
-    rows = {k: {"ms": [], "epe": [], "epe_truth": [], "final_error": []} for k in variants}
-    fallback = 0
-    frames = []
-    for i in range(n_frames):
-        mov, d = T.make_moving(clean, 90001 + i, dev)
-        frames.append(mov)
-        truth = T.truth_field(d)
-        plain_field = None
-        for name, fn in variants.items():
-            fn(mov)                                           # warm (workspace sizes)
-            ts = []
-            for _ in range(reps):
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                res = fn(mov)
-                torch.cuda.synchronize()
-                ts.append(time.perf_counter() - t0)
-            if plain_field is None:
-                plain_field = motion.clone()
-            row = rows[name]
-            row["ms"].append(1e3 * float(np.median(ts)))
-            row["epe"].append(float(torch.sqrt(((motion - plain_field) ** 2).sum(-1)).mean()))
-            row["epe_truth"].append(float(torch.sqrt(((motion - truth) ** 2).sum(-1)).mean()))
-            row["final_error"].append(res.final_error)
-        if rows["learned_fp32"]["final_error"][-1] > rows["plain_fp32"]["final_error"][-1]:
-            fallback += 1
-        del d, truth, plain_field
-    # the same frames (each twice) on 4 concurrent contexts
+        def plain(mov, layer, it):
+            r = D.get_motion(mov, tmpl, layer=layer, iter=it, motion_out=motion, **base)
+            D.correct_motion(mov, motion, out=warped)
+            return r
+
+        def learned(mov, precision, fallback=None):
+            r, info = Lr.register_learned(mov, tmpl, net, ref_level=ref_l, precision=precision, motion_out=motion,
+                                          fallback=fallback, **Lr.SCHEDULE, **base)
+            D.correct_motion(mov, motion, out=warped)
+            return r, info
+
+        for i in range(1, per_template + 1):
+            mov, d = T.make_moving(clean, tseed + i, dev)
+            if tseed == HELD_OUT_TEMPLATES[0] and len(frames_first) < 8:
+                frames_first.append((mov, tmpl, base))
+            truth = T.truth_field(d)
+            _, ms = timed(lambda: plain(mov, 4, 15), reps)
+            pf = motion.clone()
+            res_t = {"plain_fp32": ms}
+            _, res_t["learned_fp32"] = timed(lambda: learned(mov, "fp32"), reps)
+            fields = {"learned_fp32": motion.clone()}
+            _, res_t["learned_bf16"] = timed(lambda: learned(mov, "bf16"), reps)
+            fields["learned_bf16"] = motion.clone()
+            (_, info), res_t["learned_bf16_fallback"] = timed(lambda: learned(mov, "bf16", "plain"), 1)
+            used_plain += info["used"] == "plain"
+            fields["learned_bf16_fallback"] = motion.clone()
+            _, res_t["plain_fp32_same_schedule"] = timed(lambda: plain(mov, 0, Lr.SCHEDULE["iter"]), 1)
+            fields["plain_fp32_same_schedule"] = motion.clone()
+            fields["plain_fp32"] = pf
+            for k in names:
+                rows[k]["ms"].append(res_t[k])
+                rows[k]["epe"].append(float(torch.sqrt(((fields[k] - pf) ** 2).sum(-1)).mean()))
+                rows[k]["epe_truth"].append(float(torch.sqrt(((fields[k] - truth) ** 2).sum(-1)).mean()))
+            pairs.append(tseed + i)
+            del d, truth, fields, pf
+    out = {"workload": f"{len(pairs)} declared held-out pairs of {nz}x{ny}x{nx} (z,y,x) uint16 frames (templates "
+                       f"{HELD_OUT_TEMPLATES[0]}..{HELD_OUT_TEMPLATES[n_templates - 1]} step 1000, frames +1..+"
+                       f"{per_template}; 6-mode fields, max |d| 6 voxels), getMotion + correctMotion per frame",
+           "epe_against": "plain FP32 field, layer 4, r 8, 15 iterations", "gate": "mean EPE per frame <= 0.1 voxel",
+           "learned_schedule": dict(Lr.SCHEDULE), "fallback_kept_plain": used_plain}
+    for k, row in rows.items():
+        ms = float(np.mean(row["ms"]))
+        e = np.asarray(row["epe"])
+        miss = [(pairs[j], round(float(e[j]), 4)) for j in np.flatnonzero(e > 0.1)]
+        out[k] = {"value": n0 / (ms * 1e-3), "unit": "registered voxels/s", "ms_per_frame": ms,
+                  "epe_vs_plain_mean": float(e.mean()), "epe_vs_plain_max": float(e.max()),
+                  "frames_within_gate": int((e <= 0.1).sum()), "frames": len(e), "misses": miss[:32],
+                  "epe_vs_truth_mean": float(np.mean(row["epe_truth"]))}
+    for k in ("learned_fp32", "learned_bf16", "learned_bf16_fallback"):
+        out[f"speedup_{k}"] = out["plain_fp32"]["ms_per_frame"] / out[k]["ms_per_frame"]
+    # the first template's frames on 4 concurrent contexts
     from wholistic_registration_amd.batch import FrameBatch
-    batch = frames + frames
+    batch = [f[0] for f in frames_first] * 2
+    tmpl, base = frames_first[0][1], frames_first[0][2]
     conc = {}
     with FrameBatch(workers=4) as fb:
         wo = torch.empty((len(batch), nz, ny, nx), dtype=torch.float32, device=dev)
         mo = torch.empty((len(batch), nz, ny, nx, 3), dtype=torch.float32, device=dev)
         for name, kw in (("plain_fp32_4ctx", dict(layer=4, iter=15)),
-                         ("learned_fp32_4ctx", dict(net=net, **Lr.SCHEDULE))):
+                         ("learned_bf16_4ctx", dict(net=net, precision="bf16", **Lr.SCHEDULE))):
             fb.register(batch[:4], tmpl, motion_out=mo[:4], warped_out=wo[:4], **kw, **base)
             ts = []
             for _ in range(3):
@@ -423,11 +537,12 @@ def learned_measure(n_frames=6, reps=5):
             ms = 1e3 * float(np.median(ts)) / len(batch)
             conc[name] = {"value": n0 / (ms * 1e-3), "unit": "registered voxels/s", "ms_per_frame": ms,
                           "frames": len(batch)}
-    del frames, batch, wo, mo
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    m_l = D.pyramid_level(T.make_moving(clean, 90001, dev)[0], Lr.LEVEL)
-    torch.cuda.synchronize()
+    out.update(conc)
+    out["speedup_learned_bf16_4ctx"] = conc["plain_fp32_4ctx"]["ms_per_frame"] / conc["learned_bf16_4ctx"]["ms_per_frame"]
+    del frames_first, batch, wo, mo
+    m_l = D.pyramid_level(T.make_moving(T.make_template(HELD_OUT_TEMPLATES[0], dev)[0], HELD_OUT_TEMPLATES[0] + 1,
+                                        dev)[0], Lr.LEVEL)
+    ref_l = D.pyramid_level(T.make_template(HELD_OUT_TEMPLATES[0], dev)[1], Lr.LEVEL)
     for _ in range(3):
         net.predict(m_l, ref_l)
     torch.cuda.synchronize()
@@ -437,21 +552,6 @@ def learned_measure(n_frames=6, reps=5):
     torch.cuda.synchronize()
     net_ms = 1e3 * (time.perf_counter() - t0) / 10
     flops = Lr.forward_flops(*m_l.shape)
-    out = {"workload": f"{n_frames} held-out pairs of {nz}x{ny}x{nx} (z,y,x) uint16 frames (6-mode fields, "
-                       f"max |d| 6 voxels), getMotion + correctMotion per frame",
-           "epe_against": "plain FP32 field, layer 4, r 8, 15 iterations",
-           "learned_schedule": dict(Lr.SCHEDULE), "fallback_would_fire": fallback}
-    for name, row in rows.items():
-        ms = float(np.mean(row["ms"]))
-        out[name] = {"value": n0 / (ms * 1e-3), "unit": "registered voxels/s", "ms_per_frame": ms,
-                     "epe_vs_plain_mean": float(np.mean(row["epe"])), "epe_vs_plain_max": float(np.max(row["epe"])),
-                     "frames_within_gate": int(sum(e <= 0.1 for e in row["epe"])),
-                     "epe_vs_truth_mean": float(np.mean(row["epe_truth"])),
-                     "per_frame": {"ms": [round(v, 3) for v in row["ms"]], "epe": [round(v, 4) for v in row["epe"]]}}
-    out.update(conc)
-    out["speedup_learned_fp32"] = out["plain_fp32"]["ms_per_frame"] / out["learned_fp32"]["ms_per_frame"]
-    out["speedup_learned_fp32_4ctx"] = conc["plain_fp32_4ctx"]["ms_per_frame"] / conc["learned_fp32_4ctx"]["ms_per_frame"]
-    out["speedup_learned_bf16"] = out["plain_fp32"]["ms_per_frame"] / out["learned_bf16"]["ms_per_frame"]
     out["network"] = {"forward_ms": net_ms, "flops": flops, "achieved_tflops": flops / (net_ms * 1e-3) / 1e12,
                       "peak_tflops": 2500.0, "frac": flops / (net_ms * 1e-3) / 2.5e15,
                       "input": list(m_l.shape) + [8], "note": "bf16 MFMA, dense peak; wall time of predict()"}
@@ -574,7 +674,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-epe-check", action="store_true",
                     help="skip the float64-oracle endpoint-error check on the crop")
-    ap.add_argument("--cpu-crop", default="8,512,1024", help="z,y,x of the CPU-baseline crop")
+    ap.add_argument("--cpu-crop", default="8,512,1024", help="z,y,x of the EPE-check (and window-sum oracle) crop")
+    ap.add_argument("--cpu-workers", type=int, default=0,
+                    help="processes of the all-cores CPU leg (0: the host's cores, at most 16)")
     ap.add_argument("--tile-patches-x", type=int, default=0)
     ap.add_argument("--precisions", default="bf16,fp16",
                     help="reduced-precision level-image variants to time beside the float32 path "
@@ -583,8 +685,8 @@ def main():
                     help="frames of the time-series measurement (64x512x1024 frames; 0 to skip)")
     ap.add_argument("--no-mask-measure", action="store_true",
                     help="skip timing the harness's automatic mask kernels")
-    ap.add_argument("--learned-frames", type=int, default=6,
-                    help="held-out frame pairs of the plain vs learned-start comparison (0 to skip)")
+    ap.add_argument("--learned-templates", type=int, default=16,
+                    help="held-out templates (16 frames each) of the plain vs learned-start comparison (0 to skip)")
     ap.add_argument("--crops", type=int, default=16,
                     help="crops of uneven sizes for the tile-autotuner measurement (0 to skip)")
     ap.add_argument("--crops-only", action="store_true",
@@ -615,7 +717,7 @@ def main():
     from wholistic_registration_amd import device as D
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.learned_only:
-        print(json.dumps(learned_measure(max(args.learned_frames, 1))), flush=True)
+        print(json.dumps(learned_measure(max(args.learned_templates, 1))), flush=True)
         return
     if args.crops_only:
         print(json.dumps(crops_measure(max(args.crops, 1))), flush=True)
@@ -658,6 +760,7 @@ def main():
         lk_ms += st.lk_ms[0]
         lk_n += st.lk_exec[0]
         recs = res.records
+        upds = res.updates
         log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms, records {res.records[::-1]}")
     total = time.perf_counter() - t_all
     ms_per_step = 1e3 * total / args.steps
@@ -708,14 +811,18 @@ def main():
     if args.crops > 0:
         log(f"[bench] {args.crops} crops of uneven sizes")
         out["crops"] = crops_measure(args.crops)
-    if args.learned_frames > 0:
-        log(f"[bench] plain vs learned start on {args.learned_frames} held-out frames")
-        out["learned_init"] = learned_measure(args.learned_frames)
+    if args.learned_templates > 0:
+        log(f"[bench] plain vs learned start on {16 * args.learned_templates} held-out frames")
+        out["learned_init"] = learned_measure(args.learned_templates)
     if not args.no_mask_measure:
         log("[bench] fixed-plane projections")
         out["plane_projections"] = splat_measure()
         log("[bench] harness mask of the moving volume")
         out["harness_masks"] = mask_measure(mov, tuple(int(v) for v in args.cpu_crop.split(",")))
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (the reference's CPU path on a crop, extrapolated)")
+        out["cpu_baseline"] = cpu_baseline_measure(tmpl, mov, mask, w, recs, upds,
+                                                   workers=args.cpu_workers)
     if not (args.no_cpu_baseline and args.no_epe_check):
         crop = tuple(int(v) for v in args.cpu_crop.split(","))
         log(f"[bench] oracle runs on crop {crop}")

@@ -192,3 +192,46 @@ def test_learned_frames_with_fallback_on_concurrent_contexts(setup):
                                         **Lr.SCHEDULE, **s["params"])
         assert torch.equal(single.motion, m)
         assert single.records == res.records
+
+
+# The held-out set of the gate, declared before measuring: templates 90000,
+# 91000, 92000, 93000 and their frames +1 .. +10 (the bench's pairs 90001 ..
+# 90006 included).  None was seen in training or validation.
+GATE_TEMPLATES = (90000, 91000, 92000, 93000)
+GATE_FRAMES = 10
+GATE_MIN_FRACTION = 0.9
+
+
+def test_learned_gate_on_declared_held_out_set(setup, cuda):
+    """The learned start (network field, layer 0, 5 iterations) on every
+    pair of the declared held-out set: at least 90 % of the frames end within
+    the 0.1-voxel gate of the plain FP32 field, the mean over frames is
+    within it, and the same schedule from zero is worse on average.  Misses
+    are printed, not hidden."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    net = setup["net"]
+    epe, epe0, names = [], [], []
+    for tseed in GATE_TEMPLATES:
+        s = setup if tseed == 90000 else dict(setup, **_other_template(tseed, cuda))
+        p = s["params"]
+        for i in range(1, GATE_FRAMES + 1):
+            mov, _ = T.make_moving(s["clean"], tseed + i, cuda)
+            plain = D.get_motion(mov, s["tmpl"], layer=4, iter=15, **p).motion
+            res, _ = Lr.register_learned(mov, s["tmpl"], net, ref_level=s["ref_l"], **Lr.SCHEDULE, **p)
+            zero = D.get_motion(mov, s["tmpl"], **Lr.SCHEDULE, **p).motion
+            epe.append(_epe(res.motion, plain))
+            epe0.append(_epe(zero, plain))
+            names.append(tseed + i)
+            del mov, plain, zero, res
+        torch.cuda.empty_cache()
+    epe, epe0 = np.asarray(epe), np.asarray(epe0)
+    misses = [(n, round(float(e), 4)) for n, e in zip(names, epe) if e > 0.1]
+    print(f"learned start: {int((epe <= 0.1).sum())}/{len(epe)} frames within 0.1 voxel, mean {epe.mean():.4f}, "
+          f"max {epe.max():.4f}, misses {misses}; same schedule from zero: mean {epe0.mean():.4f}")
+    assert (epe <= 0.1).mean() >= GATE_MIN_FRACTION, misses
+    assert epe.mean() <= 0.1
+    assert epe.mean() < epe0.mean()
