@@ -646,6 +646,59 @@ def crops_measure(n_crops=16, seed=3, reps=3):
     return out
 
 
+def net_layer_table(d, h, w, reps=20, warmup=3, device=0):
+    """Per-layer timing of the learned start's convolutions: each 3x3x3
+    convolution of the shipped network (forward, bf16 channels-last, fused
+    bias and LeakyReLU) at the grid it has inside the network for a (D, H, W)
+    input, on random data; the mean time of ``reps`` launches timed with HIP
+    events on the launch stream.  Returns [(name, cin, cout, (d, h, w), us,
+    flops, algorithmic bytes)] (activations in and out once, weights once)."""
+    import ctypes
+
+    from wholistic_registration_amd import _lib
+    from wholistic_registration_amd.learned import SLOPE, InitNet, unet_layers
+    import torch
+    lib = _lib.load()
+    ctx = _lib.context(device)
+    dev = torch.device("cuda", device)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream(dev)
+    s = ctypes.c_void_p(stream.cuda_stream)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    layers, depth = unet_layers(InitNet.load().channels)
+    rows = []
+    for name, cin, cout, act in layers:
+        k = 2 ** depth[name]
+        dd, hh, ww = d // k, h // k, w // k
+        x = torch.randn((1, dd, hh, ww, cin), device=dev, generator=gen).to(torch.bfloat16)
+        wt = (0.1 * torch.randn((cout, 27, cin), device=dev, generator=gen)).to(torch.bfloat16)
+        b = torch.zeros(cout, device=dev)
+        y = torch.empty((1, dd, hh, ww, cout), dtype=torch.bfloat16, device=dev)
+        args = [ctypes.c_void_p(t.data_ptr()) for t in (x, wt, b, y)]
+
+        def launch():
+            _lib.check(lib.wr_net_conv3d(ctx, *args, 1, dd, hh, ww, cin, cout, int(act), SLOPE, s),
+                       "wr_net_conv3d")
+
+        for _ in range(warmup):
+            launch()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        torch.cuda.synchronize(dev)
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        vox = dd * hh * ww
+        flops = 2 * 27 * cin * cout * vox
+        nbytes = 2 * (cin + cout) * vox + 2 * 27 * cin * cout + 4 * cout
+        rows.append((name, cin, cout, (dd, hh, ww), us, flops, nbytes))
+    return rows
+
+
+
 def truth_epe(motion, d, mask):
     """Endpoint error of the GPU field against the synthetic registration
     field m = d(p + m) (fixed point, evaluated with the library's warp)."""
@@ -693,6 +746,9 @@ def main():
                     help="only run the crops measurement (prints its JSON)")
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain vs learned-start comparison (prints its JSON)")
+    ap.add_argument("--net-layers", default=None, metavar="D,H,W",
+                    help="only time each convolution of the learned start's network for a (D, H, W) input "
+                         "(64,64,128: the level-3 image of a 64x512x1024 frame) and print one line per layer")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -705,6 +761,16 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}))
+        return
+    if args.net_layers:
+        d, h, w = (int(v) for v in args.net_layers.split(","))
+        tot_us = tot_f = 0.0
+        for name, cin, cout, (dd, hh, ww), us, fl, nb in net_layer_table(d, h, w):
+            tot_us += us
+            tot_f += fl
+            print(f"{name:6s} {cin:3d}->{cout:3d} @ {dd}x{hh}x{ww}: {us:7.1f} us {fl / us / 1e6:7.1f} TFLOP/s "
+                  f"{nb / 1e6:6.1f} MB {nb / us / 1e3:6.0f} GB/s", flush=True)
+        print(f"total {tot_us:.1f} us {tot_f / tot_us / 1e6:.1f} TFLOP/s", flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames or z-slabs "

@@ -30,15 +30,28 @@ import numpy as np
 SLOPE = 0.1
 PSCALE = 4.0
 
-# (name, cin, cout, leaky) of the convolutions, in forward order
-LAYERS = (
-    ("enc1a", 8, 16, True), ("enc1b", 16, 16, True),
-    ("enc2a", 16, 32, True), ("enc2b", 32, 32, True),
-    ("bota", 32, 64, True), ("botb", 64, 64, True),
-    ("dec2a", 96, 32, True), ("dec2b", 32, 32, True),
-    ("dec1a", 48, 16, True), ("dec1b", 16, 16, True),
-    ("out", 16, 16, False),
-)
+CHANNELS = (16, 32, 64)
+
+
+def unet_layers(channels=CHANNELS, cin=8):
+    """(name, cin, cout, leaky) of the convolutions of the U-Net with the
+    given channels per level (the bottleneck last), in forward order."""
+    n = len(channels)
+    layers, c = [], cin
+    for i in range(n - 1):
+        layers += [(f"enc{i + 1}a", c, channels[i], True), (f"enc{i + 1}b", channels[i], channels[i], True)]
+        c = channels[i]
+    layers += [("bota", c, channels[-1], True), ("botb", channels[-1], channels[-1], True)]
+    c = channels[-1]
+    for i in range(n - 2, -1, -1):
+        layers += [(f"dec{i + 1}a", c + channels[i], channels[i], True), (f"dec{i + 1}b", channels[i], channels[i], True)]
+        c = channels[i]
+    layers.append(("out", channels[0], 16, False))
+    return tuple(layers)
+
+
+# (name, cin, cout, leaky) of the default network's convolutions, in forward order
+LAYERS = unet_layers()
 
 
 def bf16(a):
@@ -146,11 +159,11 @@ def adam(w, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     return w - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
-def init_params(seed=0):
+def init_params(seed=0, channels=CHANNELS):
     """He-normal weights (LeakyReLU gain) and zero biases, float32."""
     rng = np.random.default_rng(seed)
     params = {}
-    for name, cin, cout, leaky in LAYERS:
+    for name, cin, cout, leaky in unet_layers(channels):
         fan_in = 27 * (2 if name == "enc1a" else cin)      # enc1a: 2 live input channels of 8
         std = np.sqrt(2.0 / ((1 + SLOPE ** 2) * fan_in)) if leaky else np.sqrt(1.0 / fan_in)
         w = (rng.standard_normal((cout, 27, cin)) * std).astype(np.float32)
@@ -169,6 +182,13 @@ class UNet:
         self.p = params
         self.r = round_bf16
         self.slope = slope
+        ch, i = [], 1
+        while f"enc{i}a" in params:
+            ch.append(np.asarray(params[f"enc{i}a"][0]).shape[0])
+            i += 1
+        ch.append(np.asarray(params["bota"][0]).shape[0])
+        self.channels = tuple(ch)
+        self.leaky = {n: l for n, _, _, l in unet_layers(self.channels)}
 
     def _q(self, a):
         return bf16(a).astype(np.float64) if self.r else np.asarray(a, dtype=np.float64)
@@ -179,8 +199,7 @@ class UNet:
 
     def _conv(self, name, x):
         w, b = self._w(name)
-        leaky = dict((n, l) for n, _, _, l in LAYERS)[name]
-        return self._q(conv3d(x, w, b, leaky, self.slope))
+        return self._q(conv3d(x, w, b, self.leaky[name], self.slope))
 
     def _pool(self, x):
         return self._q(down2(x, 0.125)) if self.r else down2(x, 0.125, np.float64)
@@ -189,28 +208,27 @@ class UNet:
         return up2(x, 1.0).astype(np.float64) if self.r else up2(x, 1.0, np.float64)
 
     def forward(self, x):
+        n = len(self.channels)
         a = {"x": self._q(x)}
-        a["enc1a"] = self._conv("enc1a", a["x"])
-        a["enc1b"] = self._conv("enc1b", a["enc1a"])
-        a["p1"] = self._pool(a["enc1b"])
-        a["enc2a"] = self._conv("enc2a", a["p1"])
-        a["enc2b"] = self._conv("enc2b", a["enc2a"])
-        a["p2"] = self._pool(a["enc2b"])
-        a["bota"] = self._conv("bota", a["p2"])
-        a["botb"] = self._conv("botb", a["bota"])
-        a["cat2"] = np.concatenate([self._up(a["botb"]), a["enc2b"]], axis=-1)
-        a["dec2a"] = self._conv("dec2a", a["cat2"])
-        a["dec2b"] = self._conv("dec2b", a["dec2a"])
-        a["cat1"] = np.concatenate([self._up(a["dec2b"]), a["enc1b"]], axis=-1)
-        a["dec1a"] = self._conv("dec1a", a["cat1"])
-        a["dec1b"] = self._conv("dec1b", a["dec1a"])
-        a["out"] = self._conv("out", a["dec1b"])
+        h = a["x"]
+        for i in range(n - 1):
+            a[f"enc{i + 1}a"] = self._conv(f"enc{i + 1}a", h)
+            a[f"enc{i + 1}b"] = self._conv(f"enc{i + 1}b", a[f"enc{i + 1}a"])
+            h = a[f"p{i + 1}"] = self._pool(a[f"enc{i + 1}b"])
+        a["bota"] = self._conv("bota", h)
+        h = a["botb"] = self._conv("botb", a["bota"])
+        for i in range(n - 2, -1, -1):
+            a[f"cat{i + 1}"] = np.concatenate([self._up(h), a[f"enc{i + 1}b"]], axis=-1)
+            a[f"dec{i + 1}a"] = self._conv(f"dec{i + 1}a", a[f"cat{i + 1}"])
+            h = a[f"dec{i + 1}b"] = self._conv(f"dec{i + 1}b", a[f"dec{i + 1}a"])
+        a["out"] = self._conv("out", h)
         self.a = a
         return a["out"]
 
     def backward(self, dout):
         """Gradients {name: (dw, db)} for d loss / d out = dout."""
         a, q, s = self.a, self._q, self.slope
+        n = len(self.channels)
         grads = {}
 
         def conv_back(name, d, x_in, data=True):
@@ -227,29 +245,25 @@ class UNet:
         d = q(dout)
         d = conv_back("out", d, a["dec1b"])
         d = q(leaky_bwd(d, a["dec1b"], s))
-        d = conv_back("dec1b", d, a["dec1a"])
-        d = q(leaky_bwd(d, a["dec1a"], s))
-        dcat1 = conv_back("dec1a", d, a["cat1"])
-        du1, ds1 = dcat1[..., :32], dcat1[..., 32:]
-        d = down1(du1)
-        d = q(leaky_bwd(d, a["dec2b"], s))
-        d = conv_back("dec2b", d, a["dec2a"])
-        d = q(leaky_bwd(d, a["dec2a"], s))
-        dcat2 = conv_back("dec2a", d, a["cat2"])
-        du2, ds2 = dcat2[..., :64], dcat2[..., 64:]
-        d = down1(du2)
-        d = q(leaky_bwd(d, a["botb"], s))
+        skips = {}
+        for i in range(n - 1):
+            cu = self.channels[i + 1]
+            d = conv_back(f"dec{i + 1}b", d, a[f"dec{i + 1}a"])
+            d = q(leaky_bwd(d, a[f"dec{i + 1}a"], s))
+            dcat = conv_back(f"dec{i + 1}a", d, a[f"cat{i + 1}"])
+            du, skips[i] = dcat[..., :cu], dcat[..., cu:]
+            d = down1(du)
+            d = q(leaky_bwd(d, a["botb" if i == n - 2 else f"dec{i + 2}b"], s))
         d = conv_back("botb", d, a["bota"])
         d = q(leaky_bwd(d, a["bota"], s))
-        dp2 = conv_back("bota", d, a["p2"])
-        ds2 = up_acc(dp2, ds2)
-        d = q(leaky_bwd(ds2, a["enc2b"], s))
-        d = conv_back("enc2b", d, a["enc2a"])
-        d = q(leaky_bwd(d, a["enc2a"], s))
-        dp1 = conv_back("enc2a", d, a["p1"])
-        ds1 = up_acc(dp1, ds1)
-        d = q(leaky_bwd(ds1, a["enc1b"], s))
-        d = conv_back("enc1b", d, a["enc1a"])
-        d = q(leaky_bwd(d, a["enc1a"], s))
-        conv_back("enc1a", d, a["x"], data=False)
+        dp = conv_back("bota", d, a[f"p{n - 1}"])
+        for i in range(n - 2, -1, -1):
+            ds = up_acc(dp, skips[i])
+            d = q(leaky_bwd(ds, a[f"enc{i + 1}b"], s))
+            d = conv_back(f"enc{i + 1}b", d, a[f"enc{i + 1}a"])
+            d = q(leaky_bwd(d, a[f"enc{i + 1}a"], s))
+            if i > 0:
+                dp = conv_back(f"enc{i + 1}a", d, a[f"p{i}"])
+            else:
+                conv_back("enc1a", d, a["x"], data=False)
         return grads

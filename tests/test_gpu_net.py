@@ -182,16 +182,17 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_network_gradient_matches_oracle(cuda):
+@pytest.mark.parametrize("channels,dhw", [((16, 32, 64), (4, 8, 32)), ((16, 32, 64, 128), (16, 16, 64))])
+def test_network_gradient_matches_oracle(cuda, channels, dhw):
     import torch
 
     from wholistic_registration_amd.learned import InitNet
-    params = N.init_params(seed=3)
+    params = N.init_params(seed=3, channels=channels)
     rng = np.random.default_rng(4)
     for name in params:                       # nonzero biases exercise every term
         w, b = params[name]
         params[name] = (w, (0.05 * rng.standard_normal(b.shape)).astype(np.float32))
-    B, D, H, W = 2, 4, 8, 32
+    B, (D, H, W) = 2, dhw
     x = np.zeros((B, D, H, W, 8), np.float32)
     x[..., :2] = rng.standard_normal((B, D, H, W, 2))
     x = N.bf16(x)
@@ -207,13 +208,20 @@ def test_network_gradient_matches_oracle(cuda):
     assert loss.item() == pytest.approx(want_loss, rel=1e-2)
     net.backward(dout)
     grads = ref.backward(want_dout)
-    for name, _, _, _ in N.LAYERS:
+    errs = {}
+    for name, _, _, _ in N.unet_layers(channels):
         gw, gb = net.dw[name].cpu().numpy(), net.db[name].cpu().numpy()
         ww, wb = grads[name]
         if name == "enc1a":
             gw, ww = gw[..., :2], ww[..., :2]
-        assert _rel(gw, ww) <= 2e-2, name
-        assert _rel(gb, wb) <= 2e-2, name
+        errs[name] = (round(_rel(gw, ww), 4), round(_rel(gb, wb), 4))
+    print(f"channels {channels}: relative Frobenius errors (weight, bias) {errs}")
+    # bfloat16 storage points are the same in both; the float32 accumulation
+    # order of the matrix cores differs from the oracle's float64 sums, and
+    # the difference grows toward the bottleneck: measured 0.2 % (3 levels)
+    # and up to 3 % (4 levels, 2 x 2 x 8 bottleneck grid)
+    tol = 2e-2 if len(channels) == 3 else 4e-2
+    assert all(max(e) <= tol for e in errs.values()), errs
 
 
 def test_training_reduces_loss(cuda):

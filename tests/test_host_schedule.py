@@ -62,16 +62,16 @@ def test_pair_batches_share_one_shape():
         def __init__(self, shape):
             self.shape = shape
     ps = train_init.PairSet.__new__(train_init.PairSet)
-    ps.x = [X((1, 64, 64, 128, 8))] * 6 + [X((1, 32, 64, 128, 8))] * 3
+    ps.mov = [X((64, 64, 128))] * 6 + [X((32, 64, 128))] * 3
     ps.groups = {}
-    for i, x in enumerate(ps.x):
-        ps.groups.setdefault(tuple(x.shape), []).append(i)
+    for i, m in enumerate(ps.mov):
+        ps.groups.setdefault(tuple(m.shape), []).append(i)
     rng = np.random.default_rng(0)
     seen = set()
     for _ in range(200):
         idx = ps.sample(rng, 4)
-        shapes = {ps.x[i].shape for i in idx}
+        shapes = {ps.mov[i].shape for i in idx}
         assert len(shapes) == 1 and len(set(idx)) == len(idx)
         seen |= shapes
-        assert len(idx) == (4 if shapes == {(1, 64, 64, 128, 8)} else 3)
+        assert len(idx) == (4 if shapes == {(64, 64, 128)} else 3)
     assert len(seen) == 2

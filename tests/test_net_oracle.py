@@ -105,19 +105,21 @@ def test_adam_first_step_closed_form():
     assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
 
 
-def test_unet_gradient_finite_difference():
+@pytest.mark.parametrize("channels,shape", [((16, 32, 64), (1, 4, 4, 8, 8)), ((16, 32, 64, 128), (1, 8, 8, 16, 8))])
+def test_unet_gradient_finite_difference(channels, shape):
     """Whole-network backward (float64, no rounding) against central
-    differences of the loss, on a few weights of every layer."""
-    params = N.init_params(seed=7)
+    differences of the loss, on a few weights of every layer, for the
+    3-level and the 4-level network."""
+    params = N.init_params(seed=7, channels=channels)
     net = N.UNet(params, round_bf16=False)
     rng = np.random.default_rng(8)
-    x = rng.standard_normal((1, 4, 4, 8, 8))
+    x = rng.standard_normal(shape)
     x[..., 2:] = 0.0
-    t = rng.standard_normal((1, 4, 4, 8, 3))
+    t = rng.standard_normal(shape[:4] + (3,))
     loss, dout = N.mse(net.forward(x), t)
     grads = net.backward(dout)
     e = 1e-6          # small against the distance of pre-activations to LeakyReLU's kink
-    for name, _, _, _ in N.LAYERS:
+    for name, _, _, _ in N.unet_layers(channels):
         w, b = params[name]
         dw, db = grads[name]
         for j in range(2):

@@ -20,11 +20,7 @@ import threading
 from . import _lib
 from . import autotune as _tune
 from . import device as _dev
-
-
-def _torch():
-    import torch
-    return torch
+from ._util import torch_module as _torch
 
 
 class FrameBatch:

@@ -35,22 +35,10 @@ from __future__ import annotations
 import numpy as np
 
 from . import device as _dev
+from ._util import torch_module as _torch, is_tensor as _is_tensor, hip_device as _device
 
 REQUIRED_KEYS = ("layer", "iter", "r", "zRatio", "movRange", "smoothPenalty", "tol",
                  "save_ite", "mask_ref", "mask_mov")
-
-
-def _torch():
-    import torch
-    return torch
-
-
-def _is_tensor(a):
-    try:
-        import torch
-    except ImportError:        # pragma: no cover
-        return False
-    return isinstance(a, torch.Tensor)
 
 
 def _volume_to_device(a, dev, name):
@@ -104,13 +92,6 @@ def _field_to_device(f, shape_xyz, dev):
     if not np.any(arr):
         return None                 # resampling zeros gives zeros
     return torch.from_numpy(np.ascontiguousarray(arr.transpose(2, 1, 0, 3))).to(dev)
-
-
-def _device():
-    torch = _torch()
-    if not torch.cuda.is_available():
-        raise RuntimeError("wholistic_registration_amd needs a HIP device (none is visible)")
-    return torch.device("cuda", torch.cuda.current_device())
 
 
 def getMotion(dat_mov, dat_ref, option, verbose=False):

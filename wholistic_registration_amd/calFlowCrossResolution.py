@@ -24,26 +24,7 @@ import numpy as np
 
 from . import _lib
 from . import device as _dev
-
-
-def _torch():
-    import torch
-    return torch
-
-
-def _is_tensor(a):
-    try:
-        import torch
-    except ImportError:        # pragma: no cover
-        return False
-    return isinstance(a, torch.Tensor)
-
-
-def _device():
-    torch = _torch()
-    if not torch.cuda.is_available():
-        raise RuntimeError("wholistic_registration_amd needs a HIP device (none is visible)")
-    return torch.device("cuda", torch.cuda.current_device())
+from ._util import torch_module as _torch, is_tensor as _is_tensor, hip_device as _device
 
 
 def _f32_device(a, dev):

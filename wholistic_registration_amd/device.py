@@ -16,11 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-
-
-def _torch():
-    import torch
-    return torch
+from ._util import torch_module as _torch
 
 
 def _stream():

@@ -16,11 +16,13 @@ weight gradients), LeakyReLU, 2x2x2 average pooling, nearest upsampling,
 concatenation, the squared-error loss and Adam.  Torch tensors only hold
 device memory.
 
-Network (channels-last (B, D, H, W, C), D = Z, H = Y / 8, W = X / 8):
-  enc1 8->16->16 | pool | enc2 16->32->32 | pool | bottleneck 32->64->64 |
-  up, concat enc2 | dec2 96->32->32 | up, concat enc1 | dec1 48->16->16 |
-  out 16->16 (channels 0..2: the field / 4)
-D, H must be multiples of 4 and W of 32.
+Network (channels-last (B, D, H, W, C), D = Z, H = Y / 8, W = X / 8): a 3-D
+U-Net of n levels with channels c_0 .. c_{n-1} (CHANNELS; the weights file
+fixes them):
+  enc_i c_{i-1}->c_i->c_i | pool | ... | bottleneck c_{n-2}->c_{n-1}->c_{n-1} |
+  up, concat enc_i | dec_i (c_{i+1}+c_i)->c_i->c_i | ... | out c_0->16
+(channels 0..2 of the output: the field / 4).  D, H must be multiples of
+2^(n-1) and W of 8 * 2^(n-1).
 """
 
 from __future__ import annotations
@@ -32,21 +34,37 @@ import numpy as np
 
 from . import _lib
 from . import device as _dev
+from ._util import torch_module as _torch
 
 SLOPE = 0.1
 PSCALE = 4.0          # field = PSCALE * network output
 LEVEL = 3             # pyramid level the network reads
-LAYERS = (
-    ("enc1a", 8, 16, True), ("enc1b", 16, 16, True),
-    ("enc2a", 16, 32, True), ("enc2b", 32, 32, True),
-    ("bota", 32, 64, True), ("botb", 64, 64, True),
-    ("dec2a", 96, 32, True), ("dec2b", 32, 32, True),
-    ("dec1a", 48, 16, True), ("dec1b", 16, 16, True),
-    ("out", 16, 16, False),
-)
-# halvings of the input grid each layer runs at (0 = the input's own size)
-LAYER_DEPTH = {"enc1a": 0, "enc1b": 0, "enc2a": 1, "enc2b": 1, "bota": 2, "botb": 2, "dec2a": 1, "dec2b": 1,
-               "dec1a": 0, "dec1b": 0, "out": 0}
+CHANNELS = (16, 32, 64)          # levels of the default network (the bottleneck last)
+
+
+def unet_layers(channels=CHANNELS, cin=8):
+    """(name, cin, cout, leaky) of every 3x3x3 convolution of the U-Net with
+    the given channels per level, in forward order, and the level (number of
+    halvings of the input grid) each runs at."""
+    n = len(channels)
+    layers, depth, c = [], {}, cin
+    for i in range(n - 1):
+        layers += [(f"enc{i + 1}a", c, channels[i], True), (f"enc{i + 1}b", channels[i], channels[i], True)]
+        depth[f"enc{i + 1}a"] = depth[f"enc{i + 1}b"] = i
+        c = channels[i]
+    layers += [("bota", c, channels[-1], True), ("botb", channels[-1], channels[-1], True)]
+    depth["bota"] = depth["botb"] = n - 1
+    c = channels[-1]
+    for i in range(n - 2, -1, -1):
+        layers += [(f"dec{i + 1}a", c + channels[i], channels[i], True), (f"dec{i + 1}b", channels[i], channels[i], True)]
+        depth[f"dec{i + 1}a"] = depth[f"dec{i + 1}b"] = i
+        c = channels[i]
+    layers.append(("out", channels[0], 16, False))
+    depth["out"] = 0
+    return tuple(layers), depth
+
+
+LAYERS, LAYER_DEPTH = unet_layers(CHANNELS)
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "init_net.safetensors")
 # getMotion schedule after the network's start: the full-resolution level
 # only, 5 iterations (a start at the coarsest level is pulled back to that
@@ -54,21 +72,16 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "i
 SCHEDULE = dict(layer=0, iter=5)
 
 
-def _torch():
-    import torch
-    return torch
-
-
 def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-def init_params(seed=0):
+def init_params(seed=0, channels=CHANNELS):
     """He-normal weights (LeakyReLU gain), zero biases; the output layer
     scaled by 0.1 (float32 numpy)."""
     rng = np.random.default_rng(seed)
     params = {}
-    for name, cin, cout, leaky in LAYERS:
+    for name, cin, cout, leaky in unet_layers(channels)[0]:
         fan_in = 27 * (2 if name == "enc1a" else cin)
         std = np.sqrt(2.0 / ((1 + SLOPE ** 2) * fan_in)) if leaky else np.sqrt(1.0 / fan_in)
         w = (rng.standard_normal((cout, 27, cin)) * std).astype(np.float32)
@@ -78,10 +91,20 @@ def init_params(seed=0):
     return params
 
 
+def channels_of(params):
+    """The per-level channels of a parameter set (enc1a, enc2a, ..., bota)."""
+    ch, i = [], 1
+    while f"enc{i}a" in params:
+        ch.append(int(np.asarray(params[f"enc{i}a"][0]).shape[0]))
+        i += 1
+    ch.append(int(np.asarray(params["bota"][0]).shape[0]))
+    return tuple(ch)
+
+
 class InitNet:
     """Parameters, optimizer state and activations of the network on one device."""
 
-    def __init__(self, params=None, device=0, seed=0, lr=1e-3):
+    def __init__(self, params=None, device=0, seed=0, lr=1e-3, channels=CHANNELS):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError("wholistic_registration_amd needs a HIP device (none is visible)")
@@ -89,10 +112,13 @@ class InitNet:
         self.lib = _lib.load()
         self.ctx = _lib.context(device)
         if params is None:
-            params = init_params(seed)
+            params = init_params(seed, channels)
+        self.channels = channels_of(params)
+        self.layers, self.depth = unet_layers(self.channels)
+        self.spec = {name: (cin, cout, leaky) for name, cin, cout, leaky in self.layers}
         self.w32, self.b32, self.w16, self.dw, self.db = {}, {}, {}, {}, {}
         self.adam_m, self.adam_v = {}, {}
-        for name, cin, cout, _ in LAYERS:
+        for name, cin, cout, _ in self.layers:
             w, b = params[name]
             w = np.asarray(w, dtype=np.float32)
             if w.shape != (cout, 27, cin):
@@ -130,66 +156,72 @@ class InitNet:
         return t
 
     def _conv(self, name, x, y, dims, batch):
-        _, cin, cout, leaky = next(layer for layer in LAYERS if layer[0] == name)
+        cin, cout, leaky = self.spec[name]
         d, h, w = dims
         self._check(self.lib.wr_net_conv3d(self.ctx, _p(x), _p(self.w16[name]), _p(self.b32[name]), _p(y), batch,
                                            d, h, w, cin, cout, 1 if leaky else 0, SLOPE, self._s()),
                     "wr_net_conv3d")
         return y
 
+    def grid(self):
+        """(dz, dy, dx): the input's D, H and W must be multiples of these."""
+        k = 2 ** (len(self.channels) - 1)
+        return k, k, 8 * k
+
     # ---------------------------------------------------------------- forward
 
     def forward(self, x):
         """x: (B, D, H, W, 8) bfloat16 device tensor -> (B, D, H, W, 16)."""
         B, D, H, W, C = x.shape
-        if C != 8 or D % 4 or H % 4 or W % 32:
-            raise ValueError(f"network input must be (B, D, H, W, 8) with D, H % 4 == 0 and W % 32 == 0, "
+        gd, gh, gw = self.grid()
+        if C != 8 or D % gd or H % gh or W % gw:
+            raise ValueError(f"network input must be (B, D, H, W, 8) with D % {gd}, H % {gh}, W % {gw} == 0, "
                              f"got {tuple(x.shape)}")
         self.shape = (B, D, H, W)
-        d0, d1, d2 = (D, H, W), (D // 2, H // 2, W // 2), (D // 4, H // 4, W // 4)
+        n = len(self.channels)
+        dims = [(D >> i, H >> i, W >> i) for i in range(n)]
         a, L = self.act, self.lib
         a["x"] = x
-        self._conv("enc1a", x, self._buf("enc1a", (B,) + d0 + (16,)), d0, B)
-        self._conv("enc1b", a["enc1a"], self._buf("enc1b", (B,) + d0 + (16,)), d0, B)
-        self._check(L.wr_net_down2(self.ctx, _p(a["enc1b"]), _p(self._buf("p1", (B,) + d1 + (16,))), B, *d0, 16,
-                                   0.125, self._s()), "wr_net_down2")
-        self._conv("enc2a", a["p1"], self._buf("enc2a", (B,) + d1 + (32,)), d1, B)
-        self._conv("enc2b", a["enc2a"], self._buf("enc2b", (B,) + d1 + (32,)), d1, B)
-        self._check(L.wr_net_down2(self.ctx, _p(a["enc2b"]), _p(self._buf("p2", (B,) + d2 + (32,))), B, *d1, 32,
-                                   0.125, self._s()), "wr_net_down2")
-        self._conv("bota", a["p2"], self._buf("bota", (B,) + d2 + (64,)), d2, B)
-        self._conv("botb", a["bota"], self._buf("botb", (B,) + d2 + (64,)), d2, B)
-        u2 = self._buf("u2", (B,) + d1 + (64,))
-        self._check(L.wr_net_up2(self.ctx, _p(a["botb"]), _p(u2), B, *d1, 64, 1.0, 0, self._s()), "wr_net_up2")
-        n1 = B * d1[0] * d1[1] * d1[2]
-        self._check(L.wr_net_concat(self.ctx, _p(u2), 64, _p(a["enc2b"]), 32, n1,
-                                    _p(self._buf("cat2", (B,) + d1 + (96,))), self._s()), "wr_net_concat")
-        self._conv("dec2a", a["cat2"], self._buf("dec2a", (B,) + d1 + (32,)), d1, B)
-        self._conv("dec2b", a["dec2a"], self._buf("dec2b", (B,) + d1 + (32,)), d1, B)
-        u1 = self._buf("u1", (B,) + d0 + (32,))
-        self._check(L.wr_net_up2(self.ctx, _p(a["dec2b"]), _p(u1), B, *d0, 32, 1.0, 0, self._s()), "wr_net_up2")
-        n0 = B * D * H * W
-        self._check(L.wr_net_concat(self.ctx, _p(u1), 32, _p(a["enc1b"]), 16, n0,
-                                    _p(self._buf("cat1", (B,) + d0 + (48,))), self._s()), "wr_net_concat")
-        self._conv("dec1a", a["cat1"], self._buf("dec1a", (B,) + d0 + (16,)), d0, B)
-        self._conv("dec1b", a["dec1a"], self._buf("dec1b", (B,) + d0 + (16,)), d0, B)
-        return self._conv("out", a["dec1b"], self._buf("out", (B,) + d0 + (16,)), d0, B)
+        h = x
+        for i in range(n - 1):               # encoder level i, then 2x2x2 average pooling
+            c = self.channels[i]
+            h = self._conv(f"enc{i + 1}a", h, self._buf(f"enc{i + 1}a", (B,) + dims[i] + (c,)), dims[i], B)
+            h = self._conv(f"enc{i + 1}b", h, self._buf(f"enc{i + 1}b", (B,) + dims[i] + (c,)), dims[i], B)
+            p = self._buf(f"p{i + 1}", (B,) + dims[i + 1] + (c,))
+            self._check(L.wr_net_down2(self.ctx, _p(h), _p(p), B, *dims[i], c, 0.125, self._s()), "wr_net_down2")
+            h = p
+        cb = self.channels[-1]
+        h = self._conv("bota", h, self._buf("bota", (B,) + dims[-1] + (cb,)), dims[-1], B)
+        h = self._conv("botb", h, self._buf("botb", (B,) + dims[-1] + (cb,)), dims[-1], B)
+        cu = cb
+        for i in range(n - 2, -1, -1):       # decoder level i: upsample, concatenate the skip, two convs
+            c = self.channels[i]
+            u = self._buf(f"u{i + 1}", (B,) + dims[i] + (cu,))
+            self._check(L.wr_net_up2(self.ctx, _p(h), _p(u), B, *dims[i], cu, 1.0, 0, self._s()), "wr_net_up2")
+            nv = B * dims[i][0] * dims[i][1] * dims[i][2]
+            cat = self._buf(f"cat{i + 1}", (B,) + dims[i] + (cu + c,))
+            self._check(L.wr_net_concat(self.ctx, _p(u), cu, _p(a[f"enc{i + 1}b"]), c, nv, _p(cat), self._s()),
+                        "wr_net_concat")
+            h = self._conv(f"dec{i + 1}a", cat, self._buf(f"dec{i + 1}a", (B,) + dims[i] + (c,)), dims[i], B)
+            h = self._conv(f"dec{i + 1}b", h, self._buf(f"dec{i + 1}b", (B,) + dims[i] + (c,)), dims[i], B)
+            cu = c
+        return self._conv("out", h, self._buf("out", (B,) + dims[0] + (16,)), dims[0], B)
 
     # ---------------------------------------------------------------- backward
 
     def zero_grad(self):
-        for name, _, _, _ in LAYERS:
+        for name, _, _, _ in self.layers:
             self.dw[name].zero_()
             self.db[name].zero_()
 
     def _wgrad(self, name, d, x_in, dims, batch):
-        _, cin, cout, _ = next(layer for layer in LAYERS if layer[0] == name)
+        cin, cout, _ = self.spec[name]
         self._check(self.lib.wr_net_conv3d_bwd_weight(self.ctx, _p(d), _p(x_in), _p(self.dw[name]),
                                                       _p(self.db[name]), batch, *dims, cin, cout, self._s()),
                     "wr_net_conv3d_bwd_weight")
 
     def _dgrad(self, name, d, out_key, dims, batch):
-        _, cin, cout, _ = next(layer for layer in LAYERS if layer[0] == name)
+        cin, cout, _ = self.spec[name]
         dx = self._buf(out_key, (batch,) + tuple(dims) + (cin,))
         self._check(self.lib.wr_net_conv3d_bwd_data(self.ctx, _p(d), _p(self.w16[name]), _p(dx), batch, *dims, cin,
                                                     cout, self._s()), "wr_net_conv3d_bwd_data")
@@ -204,40 +236,42 @@ class InitNet:
         """Accumulates the weight and bias gradients for d loss / d out = dout
         (the activations of the last forward)."""
         B, D, H, W = self.shape
-        d0, d1, d2 = (D, H, W), (D // 2, H // 2, W // 2), (D // 4, H // 4, W // 4)
+        n = len(self.channels)
+        dims = [(D >> i, H >> i, W >> i) for i in range(n)]
         a, L, s = self.act, self.lib, self._s
-        n0, n1 = B * D * H * W, B * d1[0] * d1[1] * d1[2]
 
-        def layer(name, d, x_in, dims, out_key, y_in):
-            self._wgrad(name, d, x_in, dims, B)
-            g = self._dgrad(name, d, out_key, dims, B)
+        def layer(name, d, x_in, dm, out_key, y_in):
+            self._wgrad(name, d, x_in, dm, B)
+            g = self._dgrad(name, d, out_key, dm, B)
             return self._leaky(g, y_in) if y_in is not None else g
 
-        d = layer("out", dout, a["dec1b"], d0, "g_dec1b", a["dec1b"])
-        d = layer("dec1b", d, a["dec1a"], d0, "g_dec1a", a["dec1a"])
-        dcat1 = layer("dec1a", d, a["cat1"], d0, "g_cat1", None)
-        du1, ds1 = self._buf("g_u1", (B,) + d0 + (32,)), self._buf("g_s1", (B,) + d0 + (16,))
-        self._check(L.wr_net_split(self.ctx, _p(dcat1), 32, 16, n0, _p(du1), _p(ds1), s()), "wr_net_split")
-        d = self._buf("g_dec2b", (B,) + d1 + (32,))
-        self._check(L.wr_net_down2(self.ctx, _p(du1), _p(d), B, *d0, 32, 1.0, s()), "wr_net_down2")
-        self._leaky(d, a["dec2b"])
-        d = layer("dec2b", d, a["dec2a"], d1, "g_dec2a", a["dec2a"])
-        dcat2 = layer("dec2a", d, a["cat2"], d1, "g_cat2", None)
-        du2, ds2 = self._buf("g_u2", (B,) + d1 + (64,)), self._buf("g_s2", (B,) + d1 + (32,))
-        self._check(L.wr_net_split(self.ctx, _p(dcat2), 64, 32, n1, _p(du2), _p(ds2), s()), "wr_net_split")
-        d = self._buf("g_botb", (B,) + d2 + (64,))
-        self._check(L.wr_net_down2(self.ctx, _p(du2), _p(d), B, *d1, 64, 1.0, s()), "wr_net_down2")
-        self._leaky(d, a["botb"])
-        d = layer("botb", d, a["bota"], d2, "g_bota", a["bota"])
-        dp2 = layer("bota", d, a["p2"], d2, "g_p2", None)
-        self._check(L.wr_net_up2(self.ctx, _p(dp2), _p(ds2), B, *d1, 32, 0.125, 1, s()), "wr_net_up2")
-        self._leaky(ds2, a["enc2b"])
-        d = layer("enc2b", ds2, a["enc2a"], d1, "g_enc2a", a["enc2a"])
-        dp1 = layer("enc2a", d, a["p1"], d1, "g_p1", None)
-        self._check(L.wr_net_up2(self.ctx, _p(dp1), _p(ds1), B, *d0, 16, 0.125, 1, s()), "wr_net_up2")
-        self._leaky(ds1, a["enc1b"])
-        d = layer("enc1b", ds1, a["enc1a"], d0, "g_enc1a", a["enc1a"])
-        self._wgrad("enc1a", d, a["x"], d0, B)
+        d = layer("out", dout, a["dec1b"], dims[0], "g_dec1b", a["dec1b"])
+        skips = {}
+        for i in range(n - 1):               # decoders, finest first
+            c = self.channels[i]
+            cu = self.channels[i + 1]
+            d = layer(f"dec{i + 1}b", d, a[f"dec{i + 1}a"], dims[i], f"g_dec{i + 1}a", a[f"dec{i + 1}a"])
+            dcat = layer(f"dec{i + 1}a", d, a[f"cat{i + 1}"], dims[i], f"g_cat{i + 1}", None)
+            nv = B * dims[i][0] * dims[i][1] * dims[i][2]
+            du, ds = self._buf(f"g_u{i + 1}", (B,) + dims[i] + (cu,)), self._buf(f"g_s{i + 1}", (B,) + dims[i] + (c,))
+            self._check(L.wr_net_split(self.ctx, _p(dcat), cu, c, nv, _p(du), _p(ds), s()), "wr_net_split")
+            skips[i] = ds
+            up_in = "botb" if i == n - 2 else f"dec{i + 2}b"
+            d = self._buf(f"g_{up_in}", (B,) + dims[i + 1] + (cu,))
+            self._check(L.wr_net_down2(self.ctx, _p(du), _p(d), B, *dims[i], cu, 1.0, s()), "wr_net_down2")
+            self._leaky(d, a[up_in])
+        d = layer("botb", d, a["bota"], dims[-1], "g_bota", a["bota"])
+        dp = layer("bota", d, a[f"p{n - 1}"], dims[-1], f"g_p{n - 1}", None)
+        for i in range(n - 2, -1, -1):       # encoders, coarsest first
+            c = self.channels[i]
+            ds = skips[i]
+            self._check(L.wr_net_up2(self.ctx, _p(dp), _p(ds), B, *dims[i], c, 0.125, 1, s()), "wr_net_up2")
+            self._leaky(ds, a[f"enc{i + 1}b"])
+            d = layer(f"enc{i + 1}b", ds, a[f"enc{i + 1}a"], dims[i], f"g_enc{i + 1}a", a[f"enc{i + 1}a"])
+            if i > 0:
+                dp = layer(f"enc{i + 1}a", d, a[f"p{i}"], dims[i], f"g_p{i}", None)
+            else:
+                self._wgrad("enc1a", d, a["x"], dims[0], B)
 
     def loss(self, out, target):
         """Mean squared error of PSCALE * out[..., :3] against target (B, D, H,
@@ -254,7 +288,7 @@ class InitNet:
         """One Adam step on every parameter; refreshes the bfloat16 weights."""
         self.step_count += 1
         lr = self.lr if lr is None else lr
-        for name, _, _, _ in LAYERS:
+        for name, _, _, _ in self.layers:
             m, v = self.adam_m[name], self.adam_v[name]
             self._check(self.lib.wr_net_adam(self.ctx, _p(self.w32[name]), _p(self.dw[name]), _p(m[0]), _p(v[0]),
                                              _p(self.w16[name]), self.w32[name].numel(), lr, beta1, beta2, eps,
@@ -299,8 +333,9 @@ class InitNet:
         if mov_l.shape != ref_l.shape or mov_l.dtype != torch.float32 or ref_l.dtype != torch.float32:
             raise ValueError("level images must be float32 volumes of one shape")
         ncx, ncy = _dev.cp_grid(full_shape, r, level)
-        # the network's grid: edge-padded up to multiples of (4, 4, 32)
-        dp, hp, wp = -(-d // 4) * 4, -(-h // 4) * 4, -(-w // 32) * 32
+        # the network's grid: edge-padded up to multiples of grid()
+        gd, gh, gw = self.grid()
+        dp, hp, wp = -(-d // gd) * gd, -(-h // gh) * gh, -(-w // gw) * gw
         x = torch.empty((1, dp, hp, wp, 8), dtype=torch.bfloat16, device=self.dev)
         mov_c, ref_c = mov_l.contiguous(), ref_l.contiguous()
         self._check(self.lib.wr_net_input_padded(self.ctx, _p(mov_c), _p(ref_c), d, h, w, dp, hp, wp, _p(x),
@@ -315,7 +350,7 @@ class InitNet:
     # ---------------------------------------------------------------- storage
 
     def params(self):
-        return {name: (self.w32[name].cpu().numpy(), self.b32[name].cpu().numpy()) for name, _, _, _ in LAYERS}
+        return {name: (self.w32[name].cpu().numpy(), self.b32[name].cpu().numpy()) for name, _, _, _ in self.layers}
 
     def save(self, path):
         from safetensors.numpy import save_file
@@ -329,16 +364,18 @@ class InitNet:
     def load(cls, path=WEIGHTS, device=0):
         from safetensors.numpy import load_file
         t = load_file(path)
-        params = {name: (t[f"{name}.weight"], t[f"{name}.bias"]) for name, _, _, _ in LAYERS}
+        names = sorted({k.rsplit(".", 1)[0] for k in t})
+        params = {name: (t[f"{name}.weight"], t[f"{name}.bias"]) for name in names}
         return cls(params, device=device)
 
 
-def forward_flops(d, h, w, batch=1):
+def forward_flops(d, h, w, batch=1, channels=CHANNELS):
     """Multiply-add work of one forward pass (2 flops per MAC, the 8 padded
     input channels included): what the matrix cores execute."""
+    layers, depth = unet_layers(channels)
     total = 0
-    for name, cin, cout, _ in LAYERS:
-        k = 2 ** LAYER_DEPTH[name]
+    for name, cin, cout, _ in layers:
+        k = 2 ** depth[name]
         total += 2 * 27 * cin * cout * (d // k) * (h // k) * (w // k)
     return batch * total
 

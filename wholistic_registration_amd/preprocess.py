@@ -13,11 +13,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import device as _dev
-
-
-def _torch():
-    import torch
-    return torch
+from ._util import torch_module as _torch
 
 
 def _to_zyx(a, dev, name, dtype=None):

@@ -32,23 +32,11 @@ import numpy as np
 from . import calFlow3d_Wei_v1, mask, option, reference
 from . import device as _dev
 from . import preprocess as prep
+from ._util import torch_module as _torch, is_tensor as _is_tensor, hip_device as _device
 
 # The module default of zRatio, snapshot at import (registration.py:19-23):
 # the fallback for configs without MetaData.zRatio.
 _DEFAULT_ZRATIO = option["zRatio"]
-
-
-def _torch():
-    import torch
-    return torch
-
-
-def _is_tensor(a):
-    try:
-        import torch
-    except ImportError:        # pragma: no cover
-        return False
-    return isinstance(a, torch.Tensor)
 
 
 def load_config(config_file):
@@ -61,13 +49,6 @@ def load_config(config_file):
         return tomli.loads(data.decode() if isinstance(data, bytes) else data)
     with open(os.fspath(config_file), "rb") as f:
         return tomli.load(f)
-
-
-def _device():
-    torch = _torch()
-    if not torch.cuda.is_available():
-        raise RuntimeError("wholistic_registration_amd needs a HIP device (none is visible)")
-    return torch.device("cuda", torch.cuda.current_device())
 
 
 def _upload_zyx(a, dev):

@@ -30,6 +30,7 @@ import numpy as np
 from . import device as D
 from . import learned as Lr
 from . import synth
+from ._util import torch_module as _torch
 
 FRAME = (64, 512, 1024)
 LK_PARAMS = dict(layer=4, iter=15, r=8, zRatio=1.0, movRange=5.0, tol=1e-3)
@@ -45,11 +46,6 @@ RECIPES = {
     "volume_96x1024x2048": dict(zyx=(96, 1024, 2048), blobs=400_000, modes=8, dmax=8.0, mask_fraction=0.10,
                                 lk=dict(layer=4, iter=20, r=16, zRatio=1.0, movRange=5.0, tol=1e-3)),
 }
-
-
-def _torch():
-    import torch
-    return torch
 
 
 def make_template(seed, dev, zyx=FRAME, n_blobs=100_000):
@@ -105,18 +101,28 @@ def penalty(tmpl):
 
 
 class PairSet:
-    """Network inputs and targets of generated pairs, resident on the device."""
+    """Generated pairs resident on the device in compact form: per pair the
+    moving frame's normalised level-3 channel (bfloat16) and the target field
+    at level 3 (float16); per template its normalised level-3 channel.  A
+    batch is assembled into the network's 8-channel input on the fly, so 20k
+    pairs of the frame geometry take about 85 GB."""
 
     def __init__(self, dev, templates, fields_per_template, seed0, target="lk", log=None, recipe="frame_64x512x1024",
-                 depths=None):
+                 depths=None, budget_s=None):
         torch = _torch()
         rc0 = RECIPES[recipe]
-        self.x, self.t, self.meta = [], [], []
+        self.dev = dev
+        self.mov, self.t, self.tid, self.meta, self.ref = [], [], [], [], []
         # frames of several depths (z extents): each template is made at every depth, blobs
         # in proportion; a batch is drawn from pairs of one shape
         depths = [rc0["zyx"][0]] if not depths else list(depths)
         jobs = [(ti, dz) for ti in range(templates) for dz in depths]
+        t_start = time.time()
         for ti, dz in jobs:
+            if budget_s is not None and time.time() - t_start > budget_s:
+                if log:
+                    log(f"[pairs] time budget of {budget_s:.0f} s used: {len(self.mov)} pairs")
+                break
             rc = dict(rc0, zyx=(dz,) + tuple(rc0["zyx"][1:]), blobs=int(rc0["blobs"] * dz / rc0["zyx"][0]))
             clean, tmpl = make_template(seed0 + 1000 * ti + 100 * depths.index(dz), dev, zyx=rc["zyx"],
                                         n_blobs=rc["blobs"])
@@ -124,6 +130,7 @@ class PairSet:
             mask = make_mask(seed0 + 1000 * ti, rc["zyx"], rc["mask_fraction"], dev) if rc["mask_fraction"] else None
             sp = penalty(tmpl) if mask is None else float(D.smooth_penalty_factor(
                 tmpl.float().contiguous(), mask)) * RAW_PENALTY
+            k = len(self.ref)
             for fi in range(fields_per_template):
                 seed = seed0 + 1000 * ti + 100 * depths.index(dz) + 1 + fi
                 mov, d = make_moving(clean, seed, dev, rc["modes"], rc["dmax"])
@@ -134,23 +141,35 @@ class PairSet:
                     full = truth_field(d)
                 else:
                     raise ValueError("target must be 'lk' or 'truth'")
-                mov_l = D.pyramid_level(mov, Lr.LEVEL)
-                x = Lr.net_input(mov_l, ref_l)
-                self.x.append(x)
-                self.t.append(to_level(full).unsqueeze(0))
+                x = Lr.net_input(D.pyramid_level(mov, Lr.LEVEL), ref_l)
+                if fi == 0:
+                    self.ref.append(x[0, ..., 1].clone())
+                self.mov.append(x[0, ..., 0].clone())
+                self.t.append(to_level(full).to(torch.float16))
+                self.tid.append(k)
                 self.meta.append(seed)
-                del mov, d, full
+                del mov, d, full, x
             del clean, tmpl
-            torch.cuda.empty_cache()
-            if log:
-                log(f"[pairs] template {ti + 1}/{templates} depth {dz}: {len(self.x)} pairs")
+            if log and (ti + 1) % max(1, templates // 20) == 0:
+                log(f"[pairs] template {ti + 1}/{templates} depth {dz}: {len(self.mov)} pairs, "
+                    f"{time.time() - t_start:.0f} s")
+        torch.cuda.empty_cache()
         self.groups = {}
-        for i, x in enumerate(self.x):
-            self.groups.setdefault(tuple(x.shape), []).append(i)
+        for i, m in enumerate(self.mov):
+            self.groups.setdefault(tuple(m.shape), []).append(i)
+
+    def __len__(self):
+        return len(self.mov)
 
     def batch(self, idx):
+        """(x (B, D, H, W, 8) bfloat16 network input, t (B, D, H, W, 3) float32)."""
         torch = _torch()
-        return torch.cat([self.x[i] for i in idx]), torch.cat([self.t[i] for i in idx])
+        d, h, w = self.mov[idx[0]].shape
+        x = torch.zeros((len(idx), d, h, w, 8), dtype=torch.bfloat16, device=self.dev)
+        for j, i in enumerate(idx):
+            x[j, ..., 0] = self.mov[i]
+            x[j, ..., 1] = self.ref[self.tid[i]]
+        return x, torch.stack([self.t[i] for i in idx]).float()
 
     def sample(self, rng, batch):
         """Indices of ``batch`` pairs of one shape (shapes drawn in proportion
@@ -189,13 +208,14 @@ def evaluate(net, pairs):
     """Mean squared error of the network on ``pairs`` (and of the zero field)."""
     torch = _torch()
     tot, zero = 0.0, 0.0
-    for i in range(len(pairs.x)):
-        out = net.forward(pairs.x[i])
-        loss, _ = net.loss(out, pairs.t[i])
+    for i in range(len(pairs)):
+        x, t = pairs.batch([i])
+        out = net.forward(x)
+        loss, _ = net.loss(out, t)
         tot += float(loss.item())
-        zero += float((pairs.t[i] ** 2).mean().item())
+        zero += float((t ** 2).mean().item())
     torch.cuda.synchronize()
-    return tot / len(pairs.x), zero / len(pairs.x)
+    return tot / len(pairs), zero / len(pairs)
 
 
 def main(argv=None):
@@ -215,6 +235,9 @@ def main(argv=None):
     ap.add_argument("--init", default=None, help="safetensors weights to start from (fine-tuning)")
     ap.add_argument("--depths", default=None, help="comma list of frame depths (z extents) to mix")
     ap.add_argument("--report", default=None, help="JSON file for the loss curve and validation")
+    ap.add_argument("--channels", default="16,32,64,128", help="channels per U-Net level, the bottleneck last")
+    ap.add_argument("--data-budget-s", type=float, default=None,
+                    help="stop generating training templates after this many seconds")
     args = ap.parse_args(argv)
     torch = _torch()
 
@@ -224,10 +247,12 @@ def main(argv=None):
     dev = torch.device("cuda", 0)
     t0 = time.time()
     depths = [int(v) for v in args.depths.split(",")] if args.depths else None
-    pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log, args.recipe, depths)
+    pairs = PairSet(dev, args.templates, args.fields, 10000, args.target, log, args.recipe, depths,
+                    budget_s=args.data_budget_s)
     val = PairSet(dev, args.val_templates, 4, 20000, args.target, log, args.recipe, depths)
     t1 = time.time()
-    net = Lr.InitNet.load(args.init) if args.init else Lr.InitNet(seed=args.seed, lr=args.lr)
+    channels = tuple(int(v) for v in args.channels.split(","))
+    net = Lr.InitNet.load(args.init) if args.init else Lr.InitNet(seed=args.seed, lr=args.lr, channels=channels)
     torch.cuda.synchronize()
     t2 = time.time()
     curve = train(net, pairs, args.steps, args.batch, args.lr, log, warmup=args.warmup)
@@ -235,7 +260,8 @@ def main(argv=None):
     t3 = time.time()
     v_loss, v_zero = evaluate(net, val)
     net.save(args.out)
-    rep = {"recipe": args.recipe, "init": args.init, "depths": depths, "pairs": len(pairs.x), "val_pairs": len(val.x), "target": args.target,
+    rep = {"recipe": args.recipe, "init": args.init, "depths": depths, "pairs": len(pairs), "val_pairs": len(val),
+           "target": args.target, "channels": list(net.channels),
            "steps": args.steps,
            "batch": args.batch, "lr": args.lr, "data_s": t1 - t0, "train_s": t3 - t2,
            "ms_per_step": 1e3 * (t3 - t2) / args.steps, "curve": curve,
