@@ -422,7 +422,7 @@ HELD_OUT_TEMPLATES = tuple(90000 + 1000 * k for k in range(16))
 HELD_OUT_FRAMES = 16
 
 
-def learned_measure(n_templates=16, per_template=HELD_OUT_FRAMES, reps=3):
+def learned_measure(n_templates=16, per_template=HELD_OUT_FRAMES, reps=3, weights=None):
     """Plain FP32 pyramid LK against the learned start on the declared
     held-out pairs (up to 16 x 16 = 256).  Per pair: median wall time of
     getMotion + correctMotion over ``reps`` runs for plain FP32 and the
@@ -441,7 +441,7 @@ def learned_measure(n_templates=16, per_template=HELD_OUT_FRAMES, reps=3):
     from wholistic_registration_amd import learned as Lr
     from wholistic_registration_amd import train_init as T
     dev = torch.device("cuda", 0)
-    net = Lr.InitNet.load()
+    net = Lr.InitNet.load(weights) if weights else Lr.InitNet.load()
     nz, ny, nx = T.FRAME
     n0 = nz * ny * nx
     warped = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
@@ -551,8 +551,10 @@ def learned_measure(n_templates=16, per_template=HELD_OUT_FRAMES, reps=3):
         net.predict(m_l, ref_l)
     torch.cuda.synchronize()
     net_ms = 1e3 * (time.perf_counter() - t0) / 10
-    flops = Lr.forward_flops(*m_l.shape)
-    out["network"] = {"forward_ms": net_ms, "flops": flops, "achieved_tflops": flops / (net_ms * 1e-3) / 1e12,
+    flops = Lr.forward_flops(*m_l.shape, channels=net.channels)
+    n_par = sum(w.size + b.size for w, b in net.params().values())
+    out["network"] = {"channels": list(net.channels), "parameters": int(n_par),
+                      "forward_ms": net_ms, "flops": flops, "achieved_tflops": flops / (net_ms * 1e-3) / 1e12,
                       "peak_tflops": 2500.0, "frac": flops / (net_ms * 1e-3) / 2.5e15,
                       "input": list(m_l.shape) + [8], "note": "bf16 MFMA, dense peak; wall time of predict()"}
     return out
@@ -744,6 +746,8 @@ def main():
                     help="crops of uneven sizes for the tile-autotuner measurement (0 to skip)")
     ap.add_argument("--crops-only", action="store_true",
                     help="only run the crops measurement (prints its JSON)")
+    ap.add_argument("--learned-weights", default=None, metavar="PATH",
+                    help="network weights of the learned-start comparison (default: the shipped file)")
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain vs learned-start comparison (prints its JSON)")
     ap.add_argument("--net-layers", default=None, metavar="D,H,W",
@@ -783,7 +787,8 @@ def main():
     from wholistic_registration_amd import device as D
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.learned_only:
-        print(json.dumps(learned_measure(max(args.learned_templates, 1))), flush=True)
+        print(json.dumps(learned_measure(max(args.learned_templates, 1), weights=args.learned_weights)),
+              flush=True)
         return
     if args.crops_only:
         print(json.dumps(crops_measure(max(args.crops, 1))), flush=True)
@@ -879,7 +884,7 @@ def main():
         out["crops"] = crops_measure(args.crops)
     if args.learned_templates > 0:
         log(f"[bench] plain vs learned start on {16 * args.learned_templates} held-out frames")
-        out["learned_init"] = learned_measure(args.learned_templates)
+        out["learned_init"] = learned_measure(args.learned_templates, weights=args.learned_weights)
     if not args.no_mask_measure:
         log("[bench] fixed-plane projections")
         out["plane_projections"] = splat_measure()

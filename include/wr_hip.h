@@ -106,6 +106,14 @@ typedef struct wr_params {
                                reference: the start of the learned
                                initialisation, which skips resampling a dense
                                field that the first update replaces anyway */
+    int resample_start;     /* 0 = automatic; 1 = every level below the
+                               coarsest starts from the resampler's upsample
+                               of the previous level's dense end field,
+                               written out first (testing: the automatic path
+                               forms the same values from the control points) */
+    int maxdiff_gather;     /* 0 = automatic; 1 = the |delta motion| maximum
+                               of a level's first update always runs in the
+                               gather kernel, not the LDS-staged one (testing) */
 } wr_params;
 
 /* Per-level iteration log, filled on the host when wr_get_motion returns
@@ -320,6 +328,9 @@ typedef struct wr_stats {
     int solve_fp64[WR_MAX_LEVELS];/* with 16-bit storage (float32 solves):
                                      systems solved in float64 because their
                                      condition number exceeded 1e6            */
+    int start_from_cp[WR_MAX_LEVELS];/* 1 = the level's start field was formed
+                                     from the previous level's control points
+                                     (no dense end field written)             */
 } wr_stats;
 int wr_get_stats(wr_ctx* ctx, wr_stats* out);
 

@@ -236,7 +236,7 @@ def test_training_reduces_loss(cuda):
     # target: a smooth function of the inputs
     t = np.stack([x[..., 0] - x[..., 1], 0.5 * x[..., 0], np.full(x.shape[:4], 0.25)], -1).astype(np.float32)
     tt = torch.from_numpy(t).to(cuda)
-    net = InitNet(seed=6, lr=2e-3)
+    net = InitNet(seed=6, lr=2e-3, channels=(16, 32, 64))
     losses = []
     for _ in range(100):
         net.zero_grad()

@@ -47,11 +47,13 @@ def test_tile_cache_keys_separate_geometries():
 
 def test_network_flops_and_recipes():
     # 2 * 27 * cin * cout per voxel and layer, at the three resolutions
-    assert learned.forward_flops(64, 64, 128) == 69_759_664_128
+    assert learned.forward_flops(64, 64, 128, channels=(16, 32, 64)) == 69_759_664_128
+    assert learned.forward_flops(64, 64, 128, channels=(16, 32, 64, 128)) == 78_366_375_936
+    k = 2 ** (len(learned.CHANNELS) - 1)
     for name, rc in train_init.RECIPES.items():
         z, y, x = rc["zyx"]
         d, h, w = z, y // 2 ** learned.LEVEL, x // 2 ** learned.LEVEL
-        assert d % 4 == 0 and h % 4 == 0 and w % 32 == 0, name      # the network's input rule
+        assert d % k == 0 and h % k == 0 and w % (8 * k) == 0, name      # the network's input rule
         assert max_layer(rc["zyx"], rc["lk"]["r"]) >= rc["lk"]["layer"], name
 
 

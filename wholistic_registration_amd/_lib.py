@@ -40,6 +40,8 @@ class WrParams(ctypes.Structure):
         ("init_ny", ctypes.c_int),
         ("tile_patches_level", ctypes.POINTER(ctypes.c_int)),
         ("init_cp", ctypes.c_int),
+        ("resample_start", ctypes.c_int),
+        ("maxdiff_gather", ctypes.c_int),
     ]
 
 
@@ -65,6 +67,7 @@ class WrStats(ctypes.Structure):
         ("lk_exec", ctypes.c_int * WR_MAX_LEVELS),
         ("lk_ms", ctypes.c_double * WR_MAX_LEVELS),
         ("solve_fp64", ctypes.c_int * WR_MAX_LEVELS),
+        ("start_from_cp", ctypes.c_int * WR_MAX_LEVELS),
     ]
 
 

@@ -556,13 +556,22 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     };
     for (int L = layer; L >= std::max(0, layer - 1); --L) WR_TRY(build_level(L));
 
+    // level L (below the coarsest) forms its start field from the previous
+    // level's CP field (launch_level_start) instead of upsampling a dense
+    // end field written by launch_level_end
+    auto start_from_cp = [&](int L) {
+        return L < layer && !prm->resample_start &&
+               level_start_fits(up[L].tx, up[L].ty, up[L].x_first, up[L].fu);
+    };
+    auto starts_at_cp = [&](int L) { return L == layer && motion_init && prm->init_cp; };
+
     for (int L = layer; L >= 0; --L) {
         const LevelGeom& g = geo[L];
         const LkTile& t = tiles[L];
         const size_t nv = static_cast<size_t>(g.nx) * g.ny * nz;
 
         // initial dense field of the level, 3 planes (or, with init_cp, the CP field itself)
-        const bool cp_start = L == layer && motion_init && prm->init_cp;
+        const bool cp_start = starts_at_cp(L);
         if (cp_start) {
             WR_HIP_CHECK(hipMemcpyAsync(c.cp.p, motion_init, 3 * static_cast<size_t>(g.ncx) * g.ncy * nz * sizeof(float),
                                         hipMemcpyDeviceToDevice, s));
@@ -576,6 +585,10 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             } else {
                 fill(m0, 3 * nv, 0.0f, s);
             }
+        } else if (start_from_cp(L)) {
+            launch_level_start(g, geo[L + 1], up[L].tx, up[L].ty, up[L].fu, tw, ti, c.ctl.as<Ctl>(),
+                               c.cp.as<float>(), starts_at_cp(L + 1) ? 1 : 0, mprev, m0, s);
+            c.stats.start_from_cp[L] = 1;
         } else {
             const size_t np = static_cast<size_t>(geo[L + 1].nx) * geo[L + 1].ny * nz;
             for (int q = 0; q < 3; ++q)
@@ -610,6 +623,7 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         b.mdpart = c.mdpart.as<float>();
         b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
         b.cp_start = cp_start ? 1 : 0;
+        b.maxdiff_gather = prm->maxdiff_gather ? 1 : 0;
         for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, k, prm->stop_kernels != 0, s));
         c.stats.lk_launches[L] = iter;
         c.stats.lk_blocks[L] = static_cast<int>(t.blocks);
@@ -617,7 +631,8 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             c.stats.tile_cols = t.cols;
             c.stats.tile_rows = t.rows;
         }
-        if (L > 0) launch_level_end(g, b, L, iter, mprev, nullptr, nullptr, s);
+        if (L > 0 && start_from_cp(L - 1)) launch_level_keep(g, b, mprev, s);
+        else if (L > 0) launch_level_end(g, b, L, iter, mprev, nullptr, nullptr, s);
         else launch_level_end(g, b, L, iter, nullptr, motion_out, coords_out, s);
         WR_HIP_CHECK(hipGetLastError());
     }

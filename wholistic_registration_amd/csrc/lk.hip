@@ -352,6 +352,133 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
     }
 }
 
+// End of a level above 0 whose successor starts with k_level_start: the
+// NaN and float64-fallback counts, and, only where the level made no update
+// and did not start from its CP field (its end field is then its start
+// field), m0 copied to `prev`, since the next level's start overwrites m0.
+__global__ __launch_bounds__(256) void k_level_keep(const Ctl* __restrict__ c, const float* __restrict__ m0,
+                                                     float* __restrict__ prev, long long n, int* __restrict__ counts,
+                                                     int cp_start) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counts[2] = c->nan;
+        counts[3] = c->solve64;
+    }
+    if (cp_start || c->upd > 0) return;
+    const long long step = static_cast<long long>(gridDim.x) * 256;
+    for (long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x; i < n; i += step) prev[i] = m0[i];
+}
+
+// Float64 taps in tap order, each product and sum rounded on its own (no
+// fused multiply-add), as the resampler forms them.
+template <int K>
+__device__ __forceinline__ double tap_sum(const double (&w)[K], const float (&v)[K]) {
+#pragma clang fp contract(off)
+    double acc = w[0] * static_cast<double>(v[0]);
+#pragma unroll
+    for (int t = 1; t < K; ++t) acc = acc + w[t] * static_cast<double>(v[t]);
+    return acc;
+}
+
+__device__ __forceinline__ double tap_sum_y(const double* __restrict__ wy, const int* __restrict__ iy, int k,
+                                            int ylo, const double (*xr)[256]) {
+#pragma clang fp contract(off)
+    double acc = wy[0] * xr[iy[0] - ylo][threadIdx.x];
+    for (int t = 1; t < k; ++t) acc = acc + wy[t] * xr[iy[t] - ylo][threadIdx.x];
+    return acc;
+}
+
+// Start field of level L from the end field of level L + 1
+// (calFlow3d_Wei_v1.py:381-410): imresize(..., 'bilinear') of each
+// component to level L's size, the x and y components times 2.  The end
+// field of level L + 1 is its CP interpolant when that level made an update
+// or started from its CP field; the interpolant is formed here at the source
+// voxels the taps read, with the float32 arithmetic of k_level_end, so the
+// dense level-(L+1) field is never written.  Otherwise the end field is that
+// level's start field, which k_level_keep has copied to `prev`.  The taps
+// are those of the resampler's upsample (k_resample_up): a thread owns one
+// output column of a block of output rows of one z slice, puts the x taps
+// of each source row into its own LDS column, then applies the y taps.
+struct StartPass {
+    const double* wx;
+    const int* ix;
+    const double* wy;
+    const int* iy;
+    const int* span;            // per block of output rows: first and last source row
+    int nx, ny, ky, rows, nblk, xblocks;   // level L and its y table
+    int nxp, nyp, rp, pp, ncxp, ncyp;      // level L + 1
+};
+
+template <int KX>
+__global__ __launch_bounds__(256) void k_level_start(const Ctl* __restrict__ c, const float* __restrict__ cp,
+                                                      int cp_start, const float* __restrict__ prev,
+                                                      float* __restrict__ m0, StartPass u, int nz) {
+    __shared__ double xr[kUpRows][256];
+    const unsigned b = blockIdx.x;
+    const int xb = static_cast<int>(b % static_cast<unsigned>(u.xblocks));
+    const unsigned rest = b / static_cast<unsigned>(u.xblocks);
+    const int yb = static_cast<int>(rest % static_cast<unsigned>(u.nblk));
+    const int z = static_cast<int>(rest / static_cast<unsigned>(u.nblk));
+    const int x = xb * 256 + static_cast<int>(threadIdx.x);
+    if (x >= u.nx) return;
+    const bool from_cp = cp_start || c->upd > 0;
+    const int ylo = u.span[2 * yb];
+    const int nr = u.span[2 * yb + 1] - ylo + 1;
+    double w[KX];
+    int ix[KX];
+    Axis1 ax[KX];
+#pragma unroll
+    for (int t = 0; t < KX; ++t) {
+        w[t] = u.wx[static_cast<size_t>(x) * KX + t];
+        ix[t] = u.ix[static_cast<size_t>(x) * KX + t];
+        ax[t] = cp_axis(ix[t], u.rp, u.pp, u.ncxp);
+    }
+    const size_t ncp = static_cast<size_t>(u.ncxp) * u.ncyp * nz;
+    const size_t np = static_cast<size_t>(u.nxp) * u.nyp * nz;
+    const size_t nv = static_cast<size_t>(u.nx) * u.ny * nz;
+    const int y0 = yb * u.rows;
+    const int y1 = min(y0 + u.rows, u.ny);
+    for (int q = 0; q < 3; ++q) {
+        const float* cq = cp + q * ncp + static_cast<size_t>(z) * u.ncyp * u.ncxp;
+        const float* pq = prev + q * np + static_cast<size_t>(z) * u.nyp * u.nxp;
+        // the x interpolation of CP rows c0 and c1 at the thread's taps (the
+        // inner lerps of cp_bilinear); source rows visit CP rows in
+        // ascending order, so each is formed once
+        int c0 = -1, c1 = -1;
+        float h0[KX], h1[KX];
+        for (int j = 0; j < nr; ++j) {
+            const int ys = ylo + j;
+            float v[KX];
+            if (from_cp) {
+                const Axis1 ay = cp_axis(ys, u.rp, u.pp, u.ncyp);
+                if (ay.i0 != c0 || ay.i1 != c1) {
+#pragma unroll
+                    for (int t = 0; t < KX; ++t) {
+                        h0[t] = ay.i0 == c1 ? h1[t]
+                                            : lerpf(cq[ay.i0 * u.ncxp + ax[t].i0], cq[ay.i0 * u.ncxp + ax[t].i1],
+                                                    ax[t].t);
+                        h1[t] = lerpf(cq[ay.i1 * u.ncxp + ax[t].i0], cq[ay.i1 * u.ncxp + ax[t].i1], ax[t].t);
+                    }
+                    c0 = ay.i0;
+                    c1 = ay.i1;
+                }
+#pragma unroll
+                for (int t = 0; t < KX; ++t) v[t] = lerpf(h0[t], h1[t], ay.t);
+            } else {
+#pragma unroll
+                for (int t = 0; t < KX; ++t) v[t] = pq[static_cast<size_t>(ys) * u.nxp + ix[t]];
+            }
+            xr[j][threadIdx.x] = tap_sum<KX>(w, v);
+        }
+        float* out = m0 + q * nv + static_cast<size_t>(z) * u.ny * u.nx + x;
+        for (int yo = y0; yo < y1; ++yo) {
+            const double acc = tap_sum_y(u.wy + static_cast<size_t>(yo) * u.ky, u.iy + static_cast<size_t>(yo) * u.ky,
+                                         u.ky, ylo, xr);
+            // times 2 (exact) for the x and y components
+            out[static_cast<size_t>(yo) * u.nx] = static_cast<float>(q < 2 ? acc * 2.0 : acc);
+        }
+    }
+}
+
 inline unsigned blocks_for(long long n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 }  // namespace
@@ -386,6 +513,7 @@ LkTile plan_tile(const LevelGeom& g, int np_x) {
     t.nbx = (g.nx + t.cols - 1) / t.cols;
     t.nby = (g.ny + t.rows - 1) / t.rows;
     t.blocks = static_cast<long long>(t.nbx) * t.nby * g.nz;
+    t.nwx = (t.cols + kWaveCols - 1) / kWaveCols;
     return t;
 }
 
@@ -427,7 +555,7 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
                                 fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts);
     if (dense) {
         const size_t lds = dense_lds_bytes(g);
-        if (lds <= kDenseLdsMax)
+        if (lds <= kDenseLdsMax && !b.maxdiff_gather)
             k_dense_maxdiff_lds<<<nd, 256, lds, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
                                                      b.mdpart + cpb);
         else
@@ -437,6 +565,50 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     if (!fuse_check(k))
         k_check<<<1, 1024, 0, s>>>(b.ctl, k, b.mdpart, static_cast<int>(parts_of(k)), b.log, b.counts);
     return WR_OK;
+}
+
+bool level_start_fits(const Table& tx, const Table& ty, bool x_first, const FusedY& fu) {
+    return !tx.identity && !ty.identity && x_first && fu.rows > 0 && tx.k >= 2 && tx.k <= 4;
+}
+
+void launch_level_start(const LevelGeom& g, const LevelGeom& gp, const Table& tx, const Table& ty,
+                        const FusedY& fu, const double* w, const int* idx, const Ctl* ctl, const float* cp,
+                        int cp_start, const float* prev, float* m0, hipStream_t s) {
+    StartPass u;
+    u.wx = w + tx.w_off;
+    u.ix = idx + tx.i_off;
+    u.wy = w + ty.w_off;
+    u.iy = idx + ty.i_off;
+    u.span = idx + fu.s_off;
+    u.nx = g.nx;
+    u.ny = g.ny;
+    u.ky = ty.k;
+    u.rows = fu.rows;
+    u.nblk = fu.nblk;
+    u.xblocks = (g.nx + 255) / 256;
+    u.nxp = gp.nx;
+    u.nyp = gp.ny;
+    u.rp = gp.r;
+    u.pp = gp.p;
+    u.ncxp = gp.ncx;
+    u.ncyp = gp.ncy;
+    const unsigned blocks = static_cast<unsigned>(static_cast<long long>(u.xblocks) * fu.nblk * g.nz);
+    switch (tx.k) {
+        case 2:
+            k_level_start<2><<<blocks, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz);
+            break;
+        case 3:
+            k_level_start<3><<<blocks, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz);
+            break;
+        default:
+            k_level_start<4><<<blocks, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz);
+    }
+}
+
+void launch_level_keep(const LevelGeom& g, const LkBuffers& b, float* prev, hipStream_t s) {
+    const long long n = 3LL * g.nx * g.ny * g.nz;
+    const unsigned blocks = static_cast<unsigned>(std::min<long long>(blocks_for(n, 256), 4096));
+    k_level_keep<<<blocks, 256, 0, s>>>(b.ctl, b.m0, prev, n, b.counts, b.cp_start);
 }
 
 void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter, float* dense_soa,

@@ -53,7 +53,6 @@ namespace wr {
 namespace {
 
 constexpr int kMaxThreads = 256;
-constexpr int kWaveCols = 62;   // tile columns owned by one wave (64 lanes - 2 halo)
 
 // Coordinate clamped to [0, hi] (map_coordinates mode='nearest') split into
 // floor and fraction.
@@ -918,7 +917,7 @@ long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b
     a.blocks = t.blocks;
     a.ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
     a.z_ratio = g.z_ratio;
-    const int threads = 64 * ((t.cols + kWaveCols - 1) / kWaveCols);
+    const int threads = 64 * t.nwx;
     a.npen = (a.ncp + threads - 1) / threads;
     const dim3 grid(static_cast<unsigned>(t.blocks + a.npen));
     const bool mr = b.mref != nullptr, mm = b.mmov != nullptr;

@@ -452,9 +452,6 @@ __global__ __launch_bounds__(256) void k_resample_xy(const void* __restrict__ in
 // column, then applies the y taps of every output row from there.  Same
 // float64 arithmetic and order as the two separate passes; no intermediate in
 // HBM and no barrier (a thread reads back only what it wrote).
-constexpr int kUpRows = 16;
-constexpr int kUpTaps = 6;   // x taps handled (kernel instantiated for 2..kUpTaps)
-
 struct UpPass {
     const double* wx;
     const int* ix;

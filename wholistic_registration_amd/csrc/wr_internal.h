@@ -120,6 +120,10 @@ void launch_resample_x(const void* in, Elem in_t, int in_stride, int in_comp, vo
 // possible), block b reading source rows [span[2b], span[2b+1]] (ints at
 // s_off in the arena), at most 64 of them.
 constexpr int kFusedMinRows = 8;
+// Upsampling x-then-y blocks read at most kUpRows source rows; x tables of
+// 2..kUpTaps taps.
+constexpr int kUpRows = 16;
+constexpr int kUpTaps = 6;
 struct FusedY {
     int rows = 0, nblk = 0;
     size_t s_off = 0;
@@ -180,7 +184,8 @@ struct LevelGeom {
 };
 
 // Widest LK tile: 4 waves, each owning 62 columns plus a halo lane per side.
-constexpr int kMaxTileCols = 248;
+constexpr int kWaveCols = 62;
+constexpr int kMaxTileCols = 4 * kWaveCols;
 
 struct LkTile {
     int np_x;            // patches per workgroup along x
@@ -188,6 +193,7 @@ struct LkTile {
     int rows;            // p owned rows (one patch row)
     int nbx, nby;        // workgroups along x, y
     long long blocks;    // nbx * nby * nz
+    int nwx;             // waves of the workgroup, one per 62 tile columns
 };
 
 LkTile plan_tile(const LevelGeom& g, int np_x);
@@ -225,6 +231,7 @@ struct LkBuffers {
     float* mdpart;           // per-workgroup max |delta motion| partials
     int cp_start = 0;        // 1: the level starts from the CP field itself (no dense field m0):
                              // iteration 0 is a CP pass and its |delta motion| test uses CP maxima
+    int maxdiff_gather = 0;  // 1: the first update's dense |delta motion| maximum uses the gather kernel
 };
 
 // Partial-maximum slots needed by one iteration with `ncp` control points.
@@ -245,6 +252,19 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
                      hipStream_t s);
 void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter,
                       float* dense_soa, float* motion_aos, float* coords_aos, hipStream_t s);
+// Start field of level L (geometry g) from the end of level L + 1 (gp): the
+// bilinear upsample (tables tx, ty, blocking fu of plan_up_y) of that level's
+// CP interpolant, formed from `cp` on the fly, or of `prev` where that level
+// made no update (see launch_level_keep).  level_start_fits: the tables this
+// kernel handles (x first, 2..4 x taps).
+bool level_start_fits(const Table& tx, const Table& ty, bool x_first, const FusedY& fu);
+void launch_level_start(const LevelGeom& g, const LevelGeom& gp, const Table& tx, const Table& ty,
+                        const FusedY& fu, const double* w, const int* idx, const Ctl* ctl, const float* cp,
+                        int cp_start, const float* prev, float* m0, hipStream_t s);
+// End of a level above 0 followed by launch_level_start: its counts, and its
+// start field copied to `prev` where it made no update and did not start
+// from its CP field.
+void launch_level_keep(const LevelGeom& g, const LkBuffers& b, float* prev, hipStream_t s);
 
 // ---------------------------------------------------------------- warp
 

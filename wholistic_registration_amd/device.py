@@ -91,7 +91,7 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
                schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
                stop_kernels=False, precision="fp32", reuse_template=False, context=None, stream=None,
-               motion_init_cp=None):
+               motion_init_cp=None, resample_start=False, maxdiff_gather=False):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
@@ -114,6 +114,10 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     ``motion_init_cp`` (instead of ``motion_init``): the coarsest level's
     control-point field itself, (3, Z, ncy, ncx) float32 in that level's
     voxel units (``cp_grid``); the level starts from its interpolant.
+    ``resample_start`` and ``maxdiff_gather`` select the reference-shaped
+    kernels that the automatic path replaces (a materialised dense field per
+    level end, upsampled by the resampler; the gather form of the first
+    update's |delta motion| maximum), for tests that compare the two.
     """
     torch = _torch()
     _check_vol(mov, "mov")
@@ -154,6 +158,8 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     else:
         prm.tile_patches_x = int(tile_patches_x)
     prm.stop_kernels = 1 if stop_kernels else 0
+    prm.resample_start = 1 if resample_start else 0
+    prm.maxdiff_gather = 1 if maxdiff_gather else 0
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
     prm.precision = PRECISIONS[precision]

@@ -39,7 +39,7 @@ from ._util import torch_module as _torch
 SLOPE = 0.1
 PSCALE = 4.0          # field = PSCALE * network output
 LEVEL = 3             # pyramid level the network reads
-CHANNELS = (16, 32, 64)          # levels of the default network (the bottleneck last)
+CHANNELS = (16, 32, 64, 128)     # levels of the shipped network (the bottleneck last)
 
 
 def unet_layers(channels=CHANNELS, cin=8):
