@@ -283,13 +283,15 @@ def precision_measure(w, n0, step, motion_fp32, field, mask, steps, prec):
     }
 
 
-def batch_measure(n_frames, worker_counts, seed=2):
+def batch_measure(n_frames, worker_counts, seed=2, batched=((8, 1), (16, 1), (32, 1), (16, 2))):
     """Frames of a time series against one template (the time-series frame
     geometry): a 64 x 512 x 1024 uint16 template with 100k blobs, frames
     pull-warped by their own 6-mode fields (max |d| = 6) with fresh Poisson
     noise; layer 4, r = 8, 15 iterations, no mask.  Registered voxels/s with
-    one context (frames in sequence) and with several concurrent contexts
-    (FrameBatch), on the same frames."""
+    one context (frames in sequence), with several concurrent contexts
+    (FrameBatch.register), and with the batched launch over chunks of frames
+    (FrameBatch.register_batched, `batched` = (chunk, contexts) pairs), on the
+    same frames."""
     import torch
 
     from wholistic_registration_amd import device as D
@@ -325,6 +327,18 @@ def batch_measure(n_frames, worker_counts, seed=2):
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
         out[f"workers_{wk}"] = {"value": n_vox / dt, "ms_per_frame": 1e3 * dt / n_frames}
+    # the batched launch: every kernel of a level-iteration one launch over a chunk of frames
+    stack = torch.stack(frames).contiguous()
+    del frames
+    for ch, wk in batched:
+        with FrameBatch(workers=wk) as fb:
+            fb.register_batched(stack[: min(n_frames, ch * wk)], tmpl, chunk=ch, **params)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fb.register_batched(stack, tmpl, chunk=ch, warped_out=warped_out, **params)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        out[f"batched_chunk{ch}_workers_{wk}"] = {"value": n_vox / dt, "ms_per_frame": 1e3 * dt / n_frames}
     out["unit"] = "registered voxels/s"
     return out
 

@@ -154,6 +154,29 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype,
                   float* motion_out, float* coords_out, wr_log* log,
                   void* stream);
 
+/* getMotion of `frames` moving volumes against one template, the frames of a
+ * time series (the serial loop of registration.py:299-346, which calls
+ * getMotion once per frame).  The frames go through one launch sequence:
+ * every kernel of a level-iteration is one launch over all frames, each frame
+ * with its own control-point field, stop rules and logs, so every frame's
+ * result equals a wr_get_motion call on it alone, bit for bit.
+ *   mov           frame f at mov + f * mov_stride elements, (Z,Y,X) each
+ *   mask_mov      NULL, or frame f at mask_mov + f * mask_mov_stride floats
+ *   motion_init   NULL, or frame f at motion_init + f * init_stride floats
+ *                 (the layout params->init_nx / init_ny / init_cp select)
+ *   motion_out    (frames, Z, Y, X, 3) float32
+ *   logs          NULL or `frames` host structs
+ * ref, mask_ref, params and stream as for wr_get_motion (params->reuse_template
+ * applies to the template).  Device memory: one workspace per frame, about
+ * 21 bytes per voxel (11 more with 16-bit level storage). */
+int wr_get_motion_batch(wr_ctx* ctx, const void* mov, long long mov_stride,
+                        const void* ref, int dtype, int nx, int ny, int nz,
+                        int frames, const float* mask_ref,
+                        const float* mask_mov, long long mask_mov_stride,
+                        const float* motion_init, long long init_stride,
+                        const wr_params* params, float* motion_out,
+                        wr_log* logs, void* stream);
+
 /* Replaces correctMotion(data_raw, motion_field)
  * (calFlow3d_Wei_v1.py:676-706): out(p) = trilinear(in, p + motion(p)),
  * clamp to edge, float32 out. */

@@ -105,3 +105,90 @@ def test_second_register_on_a_new_template_at_the_same_address(cuda):
         one = D.get_motion(f, buf, **PARAMS)
         torch.cuda.synchronize()
         assert torch.equal(motion[i], one.motion), i
+
+
+def _frames_case(cuda, seed, frames, shape=(6, 48, 56), uint16=True):
+    import torch
+    tmpl, movs = harness_stack(seed=seed, shape_zyx=shape, frames=frames, uint16=uint16)
+    ref = torch.from_numpy(tmpl).to(cuda)
+    stack = torch.from_numpy(np.ascontiguousarray(movs)).to(cuda)
+    return tmpl, ref, stack, movs
+
+
+def _same_result(a, b):
+    import torch
+    assert torch.equal(a.motion, b.motion)
+    assert a.records == b.records and a.updates == b.updates and a.nans == b.nans
+    for k in ("diff_error", "penalty_error", "current_error", "max_diff_motion"):
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k), err_msg=k)
+    assert a.final_error == b.final_error or (np.isnan(a.final_error) and np.isnan(b.final_error))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("masks", [False, True])
+def test_batched_launch_equals_single_calls(cuda, precision, masks):
+    """wr_get_motion_batch: one launch per kernel and level-iteration over all
+    frames, each frame's field, schedule and logs equal to wr_get_motion on
+    that frame alone, bit for bit (frames stop at different iterations)."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    tmpl, ref, stack, movs = _frames_case(cuda, 910 + masks, frames=5)
+    mref = torch.from_numpy((tmpl > 2000).astype(np.float32)).to(cuda) if masks else None
+    mmov = torch.from_numpy((movs > 2500).astype(np.float32)).to(cuda) if masks else None
+    prm = dict(layer=2, iter=8, r=3, zRatio=1.0, movRange=5.0, smoothPenalty=50.0, tol=1e-3, precision=precision)
+    res = D.get_motion_batch(stack, ref, mask_ref=mref, mask_mov=mmov, **prm)
+    scheds = set()
+    for f in range(stack.shape[0]):
+        one = D.get_motion(stack[f], ref, mask_ref=mref, mask_mov=None if mmov is None else mmov[f], **prm)
+        _same_result(res[f], one)
+        scheds.add(tuple(sorted(one.schedule().items())))
+    print(f"{len(scheds)} distinct schedules over {stack.shape[0]} frames")
+
+
+def test_batched_launch_with_warm_start_and_forced_schedule(cuda):
+    import torch
+
+    from wholistic_registration_amd import device as D
+    _, ref, stack, _ = _frames_case(cuda, 912, frames=3, shape=(4, 64, 80), uint16=False)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    init = (0.3 * torch.randn((3,) + tuple(stack.shape[1:]) + (3,), generator=g)).to(cuda).contiguous()
+    prm = dict(layer=1, iter=6, r=4, zRatio=2.0, movRange=5.0, smoothPenalty=30.0, tol=1e-3)
+    sched = {1: (3, 2), 0: (6, 5)}
+    for kw in (dict(), dict(schedule=sched)):
+        res = D.get_motion_batch(stack, ref, motion_init=init, **prm, **kw)
+        for f in range(stack.shape[0]):
+            one = D.get_motion(stack[f], ref, motion_init=init[f], **prm, **kw)
+            _same_result(res[f], one)
+
+
+def test_batched_launch_single_frame_and_argument_errors(cuda):
+    import torch
+
+    from wholistic_registration_amd import device as D
+    _, ref, stack, _ = _frames_case(cuda, 913, frames=2)
+    prm = dict(layer=1, iter=5, r=3, zRatio=1.0, movRange=5.0, smoothPenalty=50.0, tol=1e-3)
+    res = D.get_motion_batch(stack[:1].contiguous(), ref, **prm)
+    _same_result(res[0], D.get_motion(stack[0], ref, **prm))
+    with pytest.raises(ValueError):
+        D.get_motion_batch(stack, ref[:, :, :-1].contiguous(), **prm)
+    with pytest.raises(ValueError):
+        D.get_motion_batch(stack, ref, mask_mov=torch.zeros_like(stack[0], dtype=torch.float32), **prm)
+
+
+def test_register_batched_equals_register(cuda):
+    """FrameBatch.register_batched (chunks of frames per batched launch,
+    chunks spread over two contexts, warps included) equals register."""
+    import torch
+
+    from wholistic_registration_amd.batch import FrameBatch
+    _, ref, stack, _ = _frames_case(cuda, 914, frames=5)
+    prm = dict(layer=1, iter=6, r=3, zRatio=1.0, movRange=5.0, smoothPenalty=50.0, tol=1e-3)
+    w1 = torch.empty(stack.shape, dtype=torch.float32, device=cuda)
+    w2 = torch.empty_like(w1)
+    with FrameBatch(workers=2) as fb:
+        m1, r1 = fb.register(list(stack), ref, warped_out=w1, **prm)
+        m2, r2 = fb.register_batched(stack, ref, chunk=2, warped_out=w2, **prm)
+    torch.cuda.synchronize()
+    assert torch.equal(m1, m2) and torch.equal(w1, w2)
+    assert [r.records for r in r1] == [r.records for r in r2]

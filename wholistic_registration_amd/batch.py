@@ -149,6 +149,63 @@ class FrameBatch:
             raise errors[0]
         return motion_out, results
 
+    def register_batched(self, frames, ref, chunk=16, motion_out=None, warped_out=None, masks_mov=None,
+                         **params):
+        """Register every frame of ``frames`` (a contiguous ``(T, Z, Y, X)``
+        device tensor) with the batched launch (``device.get_motion_batch``):
+        chunks of ``chunk`` frames, each chunk one launch sequence in which
+        every kernel of a level-iteration is one launch over the chunk's
+        frames, the chunks spread over the workers' contexts.  ``masks_mov``:
+        ``(T, Z, Y, X)`` float32 or None.  Returns ``(motion, results)`` as
+        ``register`` does, with the same values bit for bit."""
+        torch = _torch()
+        if frames.dim() != 4 or not frames.is_contiguous():
+            raise ValueError("frames must be a contiguous (T, Z, Y, X) tensor")
+        if chunk < 1:
+            raise ValueError("chunk must be at least 1")
+        n = int(frames.shape[0])
+        shape = tuple(frames.shape[1:])
+        if tuple(ref.shape) != shape:
+            raise ValueError("ref must have the frames' shape")
+        if motion_out is None:
+            motion_out = torch.empty((n,) + shape + (3,), dtype=torch.float32, device=ref.device)
+        torch.cuda.current_stream(ref.device).synchronize()
+        self.invalidate()
+        chunks = [(i, min(n, i + chunk)) for i in range(0, n, chunk)]
+        results = [None] * n
+        errors = []
+        w = len(self.contexts)
+
+        def work(k):
+            try:
+                ctx, strm = self.contexts[k], self.streams[k]
+                for c0, c1 in chunks[k::w]:
+                    key = (ref.data_ptr(), shape, params.get("layer"), params.get("precision", "fp32"),
+                           None if params.get("mask_ref") is None else params["mask_ref"].data_ptr())
+                    res = _dev.get_motion_batch(
+                        frames[c0:c1], ref, motion_out=motion_out[c0:c1], context=ctx, stream=strm,
+                        reuse_template=self._ref_seen[k] == key,
+                        mask_mov=None if masks_mov is None else masks_mov[c0:c1], **params)
+                    self._ref_seen[k] = key
+                    results[c0:c1] = res
+                    if warped_out is not None:
+                        for i in range(c0, c1):
+                            _dev.correct_motion(frames[i], motion_out[i], out=warped_out[i], context=ctx,
+                                                stream=strm)
+                strm.synchronize()
+            except BaseException as e:          # re-raised on the calling thread
+                self._ref_seen[k] = None
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(k,)) for k in range(min(w, len(chunks)))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        return motion_out, results
+
     def register_crops(self, pairs, layer=None, warp=False, net=None, learned_min_depth=48, plain_iter=15, **params):
         """Register crops of uneven sizes: ``pairs`` is a sequence of ``(mov,
         ref)`` or ``(mov, ref, mask_ref)`` device volumes, each pair of one

@@ -36,6 +36,18 @@ void fill(float* p, size_t n, float v, hipStream_t s) {
     k_fill<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(p, n, v);
 }
 
+// n floats of every frame's copy of a per-frame buffer (Frames).
+__global__ void k_fill_frames(float* p, size_t n, float v, long long fs) {
+    p = at_frame(p, fs);
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256)
+        p[i] = v;
+}
+
+void fill_frames(float* p, size_t n, float v, const Frames& fr, hipStream_t s) {
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 8192));
+    k_fill_frames<<<dim3(blocks, fr.n), 256, 0, s>>>(p, n, v, fr.fs);
+}
+
 // Number of control points along an axis: len(range(r, n - 1, 2r + 1))
 // (calFlow3d:434-435).
 int n_cp(int n, int r) {
@@ -223,8 +235,7 @@ void wr_destroy(wr_ctx* ctx) {
     if (!ctx) return;
     Context& c = ctx->c;
     (void)hipSetDevice(c.device);
-    for (DevBuf* b : {&c.mov_l, &c.movq, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.mmov_l, &c.m0, &c.mprev, &c.cp,
-                      &c.dnei, &c.sums, &c.errpart, &c.penpart, &c.ctl, &c.log, &c.counts, &c.mdpart, &c.tab_w, &c.tab_i, &c.pen,
+    for (DevBuf* b : {&c.fws, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.tab_w, &c.tab_i, &c.pen,
                       &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig, &c.splat, &c.netw})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
@@ -341,10 +352,21 @@ int wr_correct_motion(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, in
     return WR_OK;
 }
 
-int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
-                  const float* mask_ref, const float* mask_mov, const float* motion_init, const wr_params* prm,
-                  float* motion_out, float* coords_out, wr_log* log, void* stream) {
+namespace {
+
+// Moving frames of one registration call: frame f of mov, mask_mov and
+// motion_init starts f * stride elements after frame 0; frame f's field
+// f * out_stride floats after frame 0's.
+struct FrameArgs {
+    int n = 1;
+    long long mov_stride = 0, mmov_stride = 0, init_stride = 0, out_stride = 0;
+};
+
+static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
+                      const float* mask_ref, const float* mask_mov, const float* motion_init, const wr_params* prm,
+                      float* motion_out, float* coords_out, wr_log* logs, const FrameArgs& fa, void* stream) {
     WR_ARG(ctx && mov && ref && prm && motion_out, "wr_get_motion: null argument");
+    WR_ARG(fa.n >= 1 && fa.n <= 65535, "wr_get_motion_batch: frames must be 1 .. 65535");
     WR_ARG(dtype == WR_F32 || dtype == WR_U16, "wr_get_motion: dtype must be WR_F32 or WR_U16");
     WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_get_motion: empty volume");
     // the LK kernels index the three field planes with 32-bit offsets
@@ -471,30 +493,45 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         ref_off[L] = ref_tot;
         ref_tot += (static_cast<size_t>(geo[L].nx) * geo[L].ny * nz + 255) & ~static_cast<size_t>(255);
     }
-    WR_TRY(grow(c.mov_l, mov_tot * sizeof(float)));
-    if (q_tot) WR_TRY(grow(c.movq, q_tot));
     WR_TRY(grow(c.ref_l, ref_tot * es));
     if (mask_ref) WR_TRY(grow(c.mref_l, ref_tot));
-    if (mask_mov) WR_TRY(grow(c.mmov_l, mov_tot * sizeof(float)));
     if (!c.side) WR_HIP_CHECK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
     while (c.lvl_events.size() < static_cast<size_t>(layer + 2)) {
         hipEvent_t e;
         WR_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         c.lvl_events.push_back(e);
     }
-    WR_TRY(grow(c.m0, 3 * n0 * sizeof(float)));
-    if (layer > 0) WR_TRY(grow(c.mprev, 3 * nvox_prev * sizeof(float)));
-    WR_TRY(grow(c.cp, 3 * ncp_max * sizeof(float)));
-    WR_TRY(grow(c.dnei, 3 * ncp_max * sizeof(float)));
-    WR_TRY(grow(c.sums, 9 * ncp_max * sizeof(float)));
-    WR_TRY(grow(c.errpart, nblk_max * sizeof(double)));
-    WR_TRY(grow(c.penpart, ncpb_max * sizeof(double)));
-    WR_TRY(grow(c.mdpart, mdpart_slots(static_cast<long long>(ncp_max)) * sizeof(float)));
-    WR_TRY(grow(c.ctl, sizeof(Ctl)));
-    WR_TRY(grow(c.log, static_cast<size_t>(layer + 1) * iter * 4 * sizeof(double)));
-    WR_TRY(grow(c.counts, static_cast<size_t>(layer + 1) * 4 * sizeof(int)));
-    fill(c.log.as<float>(), static_cast<size_t>(layer + 1) * iter * 8, 0.0f, s);
-    fill(c.counts.as<float>(), static_cast<size_t>(layer + 1) * 4, 0.0f, s);
+    // every per-frame buffer of a frame in one workspace; frame f's sits f * W
+    // bytes after frame 0's (Frames)
+    size_t W = 0;
+    auto carve = [&W](size_t bytes) {
+        const size_t o = W;
+        W += (bytes + 255) & ~static_cast<size_t>(255);
+        return o;
+    };
+    const size_t o_mov = carve(mov_tot * sizeof(float));
+    const size_t o_movq = carve(q_tot);
+    const size_t o_mmov = mask_mov ? carve(mov_tot * sizeof(float)) : 0;
+    const size_t o_m0 = carve(3 * n0 * sizeof(float));
+    const size_t o_prev = carve(3 * nvox_prev * sizeof(float));
+    const size_t o_cp = carve(3 * ncp_max * sizeof(float));
+    const size_t o_dnei = carve(3 * ncp_max * sizeof(float));
+    const size_t o_sums = carve(9 * ncp_max * sizeof(float));
+    const size_t o_err = carve(nblk_max * sizeof(double));
+    const size_t o_pen = carve(ncpb_max * sizeof(double));
+    const size_t o_md = carve(mdpart_slots(static_cast<long long>(ncp_max)) * sizeof(float));
+    const size_t o_ctl = carve(sizeof(Ctl));
+    const size_t o_log = carve(static_cast<size_t>(layer + 1) * iter * 4 * sizeof(double));
+    const size_t o_cnt = carve(static_cast<size_t>(layer + 1) * 4 * sizeof(int));
+    WR_TRY(grow(c.fws, W * static_cast<size_t>(fa.n)));
+    char* const ws0 = c.fws.as<char>();
+    auto at = [&](size_t off, int f) { return ws0 + static_cast<size_t>(f) * W + off; };
+    Frames fr;
+    fr.n = fa.n;
+    fr.fs = static_cast<long long>(W);
+    fr.ofs = fa.out_stride * static_cast<long long>(sizeof(float));
+    fill_frames(reinterpret_cast<float*>(at(o_log, 0)), static_cast<size_t>(layer + 1) * iter * 8, 0.0f, fr, s);
+    fill_frames(reinterpret_cast<float*>(at(o_cnt, 0)), static_cast<size_t>(layer + 1) * 4, 0.0f, fr, s);
 
     const bool timing = prm->timing != 0;
     if (timing) {
@@ -509,8 +546,13 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     const int* ti = c.tab_i.as<int>();
     double* tmp = c.tmp.as<double>();
     double* tmp2 = c.tmp2.as<double>();
-    float* m0 = c.m0.as<float>();
-    float* mprev = c.mprev.as<float>();
+    float* m0 = reinterpret_cast<float*>(at(o_m0, 0));
+    float* mprev = reinterpret_cast<float*>(at(o_prev, 0));
+    Ctl* ctl = reinterpret_cast<Ctl*>(at(o_ctl, 0));
+    const size_t in_es = dtype == WR_U16 ? 2 : 4;
+    auto mov_of = [&](int f) {
+        return static_cast<const void*>(static_cast<const char*>(mov) + static_cast<size_t>(f) * fa.mov_stride * in_es);
+    };
     std::memset(&c.stats, 0, sizeof(c.stats));
 
     // ---- level images and masks of every level, on the side stream (coarsest
@@ -529,13 +571,16 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         // gathers from; with 16-bit storage the pass reads its records instead,
         // made straight from the input where the level is the input (level 0)
         const bool direct = st != ST_F32 && down[L].tx.identity && down[L].ty.identity;
-        float* mov_l = c.mov_l.as<float>() + mov_off[L];
-        if (!direct) {
-            resample2(mov, et, 1, 0, mov_l, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
-            launch_pad_volume(mov_l, 4, g.nx, g.ny, nz, side);
+        for (int f = 0; f < fa.n; ++f) {
+            float* mov_l = reinterpret_cast<float*>(at(o_mov, f)) + mov_off[L];
+            char* movq = at(o_movq, f) + q_off[L];
+            if (!direct) {
+                resample2(mov_of(f), et, 1, 0, mov_l, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
+                launch_pad_volume(mov_l, 4, g.nx, g.ny, nz, side);
+            }
+            if (direct) launch_moving_records_input(mov_of(f), et, g.nx, g.ny, nz, st, movq, side);
+            else if (st != ST_F32) launch_moving_records(mov_l, g.nx, g.ny, nz, st, movq, side);
         }
-        if (direct) launch_moving_records_input(mov, et, g.nx, g.ny, nz, st, c.movq.as<char>() + q_off[L], side);
-        else if (st != ST_F32) launch_moving_records(mov_l, g.nx, g.ny, nz, st, c.movq.as<char>() + q_off[L], side);
         if (L == 0 && dtype == WR_F32 && st == ST_F32) {
             ref_lv[L] = ref;
         } else {
@@ -546,11 +591,13 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
         if (mask_ref && !reuse)
             resample2(mask_ref, E_F32, 1, 0, c.mref_l.as<uint8_t>() + ref_off[L], OUT_MASK, 1.0, down[L], tw, ti, nz,
                       tmp, dense(g.nx), side);
-        if (mask_mov) {
-            float* mm = c.mmov_l.as<float>() + mov_off[L];
-            resample2(mask_mov, E_F32, 1, 0, mm, OUT_F32, 1.0, down[L], tw, ti, nz, tmp, padded, side);
-            launch_pad_volume(mm, 4, g.nx, g.ny, nz, side);
-        }
+        if (mask_mov)
+            for (int f = 0; f < fa.n; ++f) {
+                float* mm = reinterpret_cast<float*>(at(o_mmov, f)) + mov_off[L];
+                resample2(mask_mov + static_cast<size_t>(f) * fa.mmov_stride, E_F32, 1, 0, mm, OUT_F32, 1.0, down[L],
+                          tw, ti, nz, tmp, padded, side);
+                launch_pad_volume(mm, 4, g.nx, g.ny, nz, side);
+            }
         WR_HIP_CHECK(hipEventRecord(c.lvl_events[L], side));
         return WR_OK;
     };
@@ -572,55 +619,66 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
 
         // initial dense field of the level, 3 planes (or, with init_cp, the CP field itself)
         const bool cp_start = starts_at_cp(L);
+        float* const cp0 = reinterpret_cast<float*>(at(o_cp, 0));
         if (cp_start) {
-            WR_HIP_CHECK(hipMemcpyAsync(c.cp.p, motion_init, 3 * static_cast<size_t>(g.ncx) * g.ncy * nz * sizeof(float),
-                                        hipMemcpyDeviceToDevice, s));
+            for (int f = 0; f < fa.n; ++f)
+                WR_HIP_CHECK(hipMemcpyAsync(at(o_cp, f), motion_init + static_cast<size_t>(f) * fa.init_stride,
+                                            3 * static_cast<size_t>(g.ncx) * g.ncy * nz * sizeof(float),
+                                            hipMemcpyDeviceToDevice, s));
         } else if (L == layer) {
             if (motion_init) {
                 const double div[3] = {1.0 * nx / g.nx, 1.0 * ny / g.ny, 1.0 * nz / g.nz};
-                for (int q = 0; q < 3; ++q)
-                    resample2(motion_init, E_F32, 3, q, m0 + q * nv, OUT_F32, div[q],
-                              prm->init_nx > 0 ? init_plan : down[L], tw, ti, nz, tmp2,
-                              dense(g.nx), s);
+                for (int f = 0; f < fa.n; ++f) {
+                    float* m0f = reinterpret_cast<float*>(at(o_m0, f));
+                    for (int q = 0; q < 3; ++q)
+                        resample2(motion_init + static_cast<size_t>(f) * fa.init_stride, E_F32, 3, q, m0f + q * nv,
+                                  OUT_F32, div[q], prm->init_nx > 0 ? init_plan : down[L], tw, ti, nz, tmp2,
+                                  dense(g.nx), s);
+                }
             } else {
-                fill(m0, 3 * nv, 0.0f, s);
+                fill_frames(m0, 3 * nv, 0.0f, fr, s);
             }
         } else if (start_from_cp(L)) {
-            launch_level_start(g, geo[L + 1], up[L].tx, up[L].ty, up[L].fu, tw, ti, c.ctl.as<Ctl>(),
-                               c.cp.as<float>(), starts_at_cp(L + 1) ? 1 : 0, mprev, m0, s);
+            launch_level_start(g, geo[L + 1], up[L].tx, up[L].ty, up[L].fu, tw, ti, ctl, cp0,
+                               starts_at_cp(L + 1) ? 1 : 0, mprev, m0, fr, s);
             c.stats.start_from_cp[L] = 1;
         } else {
             const size_t np = static_cast<size_t>(geo[L + 1].nx) * geo[L + 1].ny * nz;
-            for (int q = 0; q < 3; ++q)
-                resample2(mprev + q * np, E_F32, 1, 0, m0 + q * nv, OUT_F32, q < 2 ? 0.5 : 1.0, up[L], tw, ti, nz, tmp2,
-                          dense(g.nx), s);
+            for (int f = 0; f < fa.n; ++f) {
+                const float* pf = reinterpret_cast<const float*>(at(o_prev, f));
+                float* m0f = reinterpret_cast<float*>(at(o_m0, f));
+                for (int q = 0; q < 3; ++q)
+                    resample2(pf + q * np, E_F32, 1, 0, m0f + q * nv, OUT_F32, q < 2 ? 0.5 : 1.0, up[L], tw, ti, nz,
+                              tmp2, dense(g.nx), s);
+            }
         }
-        if (!cp_start) launch_cp_init(g, m0, c.cp.as<float>(), s);
+        if (!cp_start) launch_cp_init(g, m0, cp0, fr, s);
         const int frec = prm->forced_records ? prm->forced_records[L] : -1;
         const int fupd = prm->forced_updates ? prm->forced_updates[L] : -1;
-        launch_ctl_reset(c.ctl.as<Ctl>(), iter, frec, fupd, prm->tol, prm->smoothPenalty * p * p,
-                         static_cast<double>(nv), s);
+        launch_ctl_reset(ctl, iter, frec, fupd, prm->tol, prm->smoothPenalty * p * p, static_cast<double>(nv), fr,
+                         s);
 
         if (L - 2 >= 0) WR_TRY(build_level(L - 2));
 
         WR_HIP_CHECK(hipStreamWaitEvent(s, c.lvl_events[L], 0));
         LkBuffers b;
         b.store = st;
-        b.mov = c.mov_l.as<float>() + mov_off[L];
-        b.movq = st != ST_F32 ? c.movq.as<char>() + q_off[L] : nullptr;
+        b.mov = reinterpret_cast<float*>(at(o_mov, 0)) + mov_off[L];
+        b.movq = st != ST_F32 ? at(o_movq, 0) + q_off[L] : nullptr;
         b.ref = ref_lv[L];
         b.mref = mask_ref ? c.mref_l.as<uint8_t>() + ref_off[L] : nullptr;
-        b.mmov = mask_mov ? c.mmov_l.as<float>() + mov_off[L] : nullptr;
+        b.mmov = mask_mov ? reinterpret_cast<float*>(at(o_mmov, 0)) + mov_off[L] : nullptr;
         b.m0 = m0;
-        b.cp = c.cp.as<float>();
-        b.dnei = c.dnei.as<float>();
-        b.sums = c.sums.as<float>();
-        b.errpart = c.errpart.as<double>();
-        b.penpart = c.penpart.as<double>();
-        b.ctl = c.ctl.as<Ctl>();
-        b.log = c.log.as<double>() + static_cast<size_t>(L) * iter * 4;
-        b.counts = c.counts.as<int>() + L * 4;
-        b.mdpart = c.mdpart.as<float>();
+        b.cp = cp0;
+        b.dnei = reinterpret_cast<float*>(at(o_dnei, 0));
+        b.sums = reinterpret_cast<float*>(at(o_sums, 0));
+        b.errpart = reinterpret_cast<double*>(at(o_err, 0));
+        b.penpart = reinterpret_cast<double*>(at(o_pen, 0));
+        b.ctl = ctl;
+        b.log = reinterpret_cast<double*>(at(o_log, 0)) + static_cast<size_t>(L) * iter * 4;
+        b.counts = reinterpret_cast<int*>(at(o_cnt, 0)) + L * 4;
+        b.mdpart = reinterpret_cast<float*>(at(o_md, 0));
+        b.fr = fr;
         b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
         b.cp_start = cp_start ? 1 : 0;
         b.maxdiff_gather = prm->maxdiff_gather ? 1 : 0;
@@ -640,14 +698,20 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
     c.tmpl = key;   // every template level of this call is enqueued
 
     // ---- logs back to the host
-    std::vector<double> hlog(static_cast<size_t>(layer + 1) * iter * 4);
-    std::vector<int> hcnt(static_cast<size_t>(layer + 1) * 4);
-    WR_HIP_CHECK(hipMemcpyAsync(hlog.data(), c.log.p, hlog.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    WR_HIP_CHECK(hipMemcpyAsync(hcnt.data(), c.counts.p, hcnt.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+    std::vector<double> hlog(static_cast<size_t>(layer + 1) * iter * 4 * fa.n);
+    std::vector<int> hcnt(static_cast<size_t>(layer + 1) * 4 * fa.n);
+    const size_t nlog = static_cast<size_t>(layer + 1) * iter * 4, ncnt = static_cast<size_t>(layer + 1) * 4;
+    for (int f = 0; f < fa.n; ++f) {
+        WR_HIP_CHECK(hipMemcpyAsync(hlog.data() + f * nlog, at(o_log, f), nlog * sizeof(double), hipMemcpyDeviceToHost,
+                                    s));
+        WR_HIP_CHECK(hipMemcpyAsync(hcnt.data() + f * ncnt, at(o_cnt, f), ncnt * sizeof(int), hipMemcpyDeviceToHost,
+                                    s));
+    }
     WR_HIP_CHECK(hipStreamSynchronize(s));
     if (timing) {
         for (int L = 0; L <= layer; ++L) {
-            const int n = std::min(hcnt[L * 4 + 0], iter);
+            int n = 0;   // launches that did work for at least one frame
+            for (int f = 0; f < fa.n; ++f) n = std::max(n, std::min(hcnt[f * ncnt + L * 4 + 0], iter));
             double tot = 0.0;
             for (int k = 0; k < n; ++k) {
                 float ms = 0.0f;
@@ -659,25 +723,64 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
             c.stats.lk_ms[L] = tot;
         }
     }
-    for (int L = 0; L <= layer; ++L) c.stats.solve_fp64[L] = hcnt[L * 4 + 3];
-    if (log) {
-        for (int L = 0; L <= layer; ++L) {
-            log->records[L] = hcnt[L * 4 + 0];
-            log->updates[L] = hcnt[L * 4 + 1];
-            log->nans[L] = hcnt[L * 4 + 2];
-            for (int k = 0; k < iter; ++k) {
-                const double* row = &hlog[(static_cast<size_t>(L) * iter + k) * 4];
-                const size_t o = static_cast<size_t>(L) * iter + k;
-                if (log->diff_error) log->diff_error[o] = row[0];
-                if (log->penalty_error) log->penalty_error[o] = row[1];
-                if (log->current_error) log->current_error[o] = row[2];
-                if (log->max_diff_motion) log->max_diff_motion[o] = row[3];
+    for (int L = 0; L <= layer; ++L) {
+        int n64 = 0;
+        for (int f = 0; f < fa.n; ++f) n64 += hcnt[f * ncnt + L * 4 + 3];
+        c.stats.solve_fp64[L] = n64;
+    }
+    if (logs) {
+        for (int f = 0; f < fa.n; ++f) {
+            wr_log* log = logs + f;
+            const double* hl = hlog.data() + f * nlog;
+            const int* hc = hcnt.data() + f * ncnt;
+            for (int L = 0; L <= layer; ++L) {
+                log->records[L] = hc[L * 4 + 0];
+                log->updates[L] = hc[L * 4 + 1];
+                log->nans[L] = hc[L * 4 + 2];
+                for (int k = 0; k < iter; ++k) {
+                    const double* row = &hl[(static_cast<size_t>(L) * iter + k) * 4];
+                    const size_t o = static_cast<size_t>(L) * iter + k;
+                    if (log->diff_error) log->diff_error[o] = row[0];
+                    if (log->penalty_error) log->penalty_error[o] = row[1];
+                    if (log->current_error) log->current_error[o] = row[2];
+                    if (log->max_diff_motion) log->max_diff_motion[o] = row[3];
+                }
             }
+            const int n = hc[0];
+            log->final_error = n > 0 ? hl[static_cast<size_t>(n - 1) * 4 + 2] : NAN;
         }
-        const int n = hcnt[0];
-        log->final_error = n > 0 ? hlog[static_cast<size_t>(n - 1) * 4 + 2] : NAN;
     }
     return WR_OK;
+}
+
+}  // namespace
+
+int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
+                  const float* mask_ref, const float* mask_mov, const float* motion_init, const wr_params* prm,
+                  float* motion_out, float* coords_out, wr_log* log, void* stream) {
+    return get_motion_frames(ctx, mov, ref, dtype, nx, ny, nz, mask_ref, mask_mov, motion_init, prm, motion_out,
+                             coords_out, log, FrameArgs{}, stream);
+}
+
+int wr_get_motion_batch(wr_ctx* ctx, const void* mov, long long mov_stride, const void* ref, int dtype, int nx,
+                        int ny, int nz, int frames, const float* mask_ref, const float* mask_mov,
+                        long long mask_mov_stride, const float* motion_init, long long init_stride,
+                        const wr_params* prm, float* motion_out, wr_log* logs, void* stream) {
+    WR_ARG(frames >= 1, "wr_get_motion_batch: frames must be at least 1");
+    WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_get_motion: empty volume");
+    const long long nvox = static_cast<long long>(nx) * ny * nz;
+    WR_ARG(frames == 1 || mov_stride >= nvox, "wr_get_motion_batch: frames of mov overlap (mov_stride < nz*ny*nx)");
+    WR_ARG(!mask_mov || frames == 1 || mask_mov_stride >= nvox,
+           "wr_get_motion_batch: frames of mask_mov overlap (mask_mov_stride < nz*ny*nx)");
+    WR_ARG(!motion_init || frames == 1 || init_stride > 0, "wr_get_motion_batch: init_stride must be positive");
+    FrameArgs fa;
+    fa.n = frames;
+    fa.mov_stride = mov_stride;
+    fa.mmov_stride = mask_mov_stride;
+    fa.init_stride = init_stride;
+    fa.out_stride = 3 * nvox;
+    return get_motion_frames(ctx, mov, ref, dtype, nx, ny, nz, mask_ref, mask_mov, motion_init, prm, motion_out,
+                             nullptr, logs, fa, stream);
 }
 
 
@@ -694,7 +797,7 @@ struct MaskWs {
     double* out;
 };
 
-int mask_ws(Context& c, MaskWs& w) {
+static int mask_ws(Context& c, MaskWs& w) {
     const size_t o_hist = 256, o_part = o_hist + kHist16Bins * sizeof(unsigned);
     const size_t o_cpart = o_part + kStreamBlocks * sizeof(double);
     const size_t o_nc = o_cpart + kStreamBlocks * sizeof(long long);
@@ -820,7 +923,7 @@ int wr_channel_transform(wr_ctx* ctx, const void* in, int dtype, long long n, do
 }
 
 namespace {
-int harness_small(Context& c, double** part, float** ms) {
+static int harness_small(Context& c, double** part, float** ms) {
     WR_TRY(grow(c.hsmall, 256 + kStreamBlocks * sizeof(double)));
     *ms = c.hsmall.as<float>();
     *part = reinterpret_cast<double*>(c.hsmall.as<char>() + 256);

@@ -41,8 +41,9 @@ namespace {
 // ---------------------------------------------------------------- CP kernels
 
 __global__ __launch_bounds__(256) void k_ctl_reset(Ctl* c, int iter, int frec, int fupd, double tol,
-                                                    double s_pen, double n_vox) {
+                                                    double s_pen, double n_vox, long long fs) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c = at_frame(c, fs);
     c->done = 0;
     c->rec = 0;
     c->upd = 0;
@@ -59,7 +60,9 @@ __global__ __launch_bounds__(256) void k_ctl_reset(Ctl* c, int iter, int frec, i
 }
 
 __global__ __launch_bounds__(256) void k_cp_init(const float* __restrict__ m0, float* __restrict__ cp, int nx,
-                                                  int ny, int nz, int r, int p, int ncx, int ncy) {
+                                                  int ny, int nz, int r, int p, int ncx, int ncy, long long fs) {
+    m0 = at_frame(m0, fs);
+    cp = at_frame(cp, fs);
     const long long ncp = static_cast<long long>(ncx) * ncy * nz;
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     if (i >= ncp) return;
@@ -98,8 +101,17 @@ __global__ __launch_bounds__(256) void k_solve(Ctl* __restrict__ c, const float*
                                                 float* __restrict__ mdpart, int k, int fuse_err,
                                                 const double* __restrict__ errpart, long long nerr,
                                                 const double* __restrict__ penpart, long long npen,
-                                                double* __restrict__ log, int* __restrict__ counts) {
+                                                double* __restrict__ log, int* __restrict__ counts, long long fs) {
+    c = at_frame(c, fs);
     if (c->done) return;
+    sums = at_frame(sums, fs);
+    dn = at_frame(dn, fs);
+    cp = at_frame(cp, fs);
+    mdpart = at_frame(mdpart, fs);
+    errpart = at_frame(errpart, fs);
+    penpart = at_frame(penpart, fs);
+    log = at_frame(log, fs);
+    counts = at_frame(counts, fs);
     if (fuse_err) {
         __shared__ int s_stop;
         double diff, pen;
@@ -198,8 +210,13 @@ inline int dense_blocks(const LevelGeom& g) {
 
 __global__ __launch_bounds__(256) void k_dense_maxdiff(const Ctl* __restrict__ c, const float* __restrict__ m0,
                                                         const float* __restrict__ cp, int nx, int ny, int nz,
-                                                        int r, int p, int ncx, int ncy, float* __restrict__ mdpart) {
+                                                        int r, int p, int ncx, int ncy, float* __restrict__ mdpart,
+                                                        long long fs) {
+    c = at_frame(c, fs);
     if (c->done) return;
+    m0 = at_frame(m0, fs);
+    cp = at_frame(cp, fs);
+    mdpart = at_frame(mdpart, fs);
     const size_t nvox = static_cast<size_t>(nx) * ny * nz;
     const size_t ncp = static_cast<size_t>(ncx) * ncy * nz;
     const unsigned nrows = static_cast<unsigned>(ny) * static_cast<unsigned>(nz);
@@ -237,9 +254,13 @@ constexpr size_t kDenseLdsMax = 48 * 1024;
 __global__ __launch_bounds__(256) void k_dense_maxdiff_lds(const Ctl* __restrict__ c, const float* __restrict__ m0,
                                                             const float* __restrict__ cp, int nx, int ny, int nz,
                                                             int r, int p, int ncx, int ncy,
-                                                            float* __restrict__ mdpart) {
+                                                            float* __restrict__ mdpart, long long fs) {
     extern __shared__ __align__(16) unsigned char dense_smem[];
-    if (c->done) return;   // the same value for every thread of the launch
+    c = at_frame(c, fs);
+    if (c->done) return;   // the same value for every thread of the frame's workgroups
+    m0 = at_frame(m0, fs);
+    cp = at_frame(cp, fs);
+    mdpart = at_frame(mdpart, fs);
     int* sx0 = reinterpret_cast<int*>(dense_smem);
     float* stx = reinterpret_cast<float*>(sx0 + nx);
     float* srow = stx + nx;   // [component][row i0 | row i1][ncx]
@@ -288,8 +309,13 @@ __global__ __launch_bounds__(256) void k_dense_maxdiff_lds(const Ctl* __restrict
 __global__ __launch_bounds__(1024) void k_err_finish(Ctl* __restrict__ c, int k, const double* __restrict__ errpart,
                                                       long long nerr, const double* __restrict__ penpart,
                                                       long long npen, double* __restrict__ log,
-                                                      int* __restrict__ counts) {
+                                                      int* __restrict__ counts, long long fs) {
+    c = at_frame(c, fs);
     if (c->done) return;
+    errpart = at_frame(errpart, fs);
+    penpart = at_frame(penpart, fs);
+    log = at_frame(log, fs);
+    counts = at_frame(counts, fs);
     err_finish_block(c, k, errpart, nerr, penpart, npen, log, counts);
 }
 
@@ -297,8 +323,13 @@ __global__ __launch_bounds__(1024) void k_err_finish(Ctl* __restrict__ c, int k,
 // per-workgroup maxima from k_solve (and k_dense_maxdiff on a level's first
 // update) are reduced here.
 __global__ __launch_bounds__(1024) void k_check(Ctl* __restrict__ c, int k, const float* __restrict__ mdpart,
-                                                 int nparts, double* __restrict__ log, int* __restrict__ counts) {
+                                                 int nparts, double* __restrict__ log, int* __restrict__ counts,
+                                                 long long fs) {
+    c = at_frame(c, fs);
     if (c->done) return;
+    mdpart = at_frame(mdpart, fs);
+    log = at_frame(log, fs);
+    counts = at_frame(counts, fs);
     check_finish_block(c, k, mdpart, nparts, log, counts);
 }
 
@@ -312,7 +343,15 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
                                                     const float* __restrict__ cp, int nx, int ny, int nz, int r,
                                                     int p, int ncx, int ncy, int xblocks, float* __restrict__ soa,
                                                     float* __restrict__ aos, float* __restrict__ coords,
-                                                    int* __restrict__ counts, int cp_start) {
+                                                    int* __restrict__ counts, int cp_start, long long fs,
+                                                    long long ofs) {
+    c = at_frame(c, fs);
+    m0 = at_frame(m0, fs);
+    cp = at_frame(cp, fs);
+    soa = at_frame(soa, fs);
+    counts = at_frame(counts, fs);
+    aos = at_frame(aos, ofs);
+    coords = at_frame(coords, ofs);
     const unsigned b = blockIdx.x;
     const unsigned row = b / static_cast<unsigned>(xblocks);
     const int x = static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x);
@@ -358,7 +397,11 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
 // field), m0 copied to `prev`, since the next level's start overwrites m0.
 __global__ __launch_bounds__(256) void k_level_keep(const Ctl* __restrict__ c, const float* __restrict__ m0,
                                                      float* __restrict__ prev, long long n, int* __restrict__ counts,
-                                                     int cp_start) {
+                                                     int cp_start, long long fs) {
+    c = at_frame(c, fs);
+    m0 = at_frame(m0, fs);
+    prev = at_frame(prev, fs);
+    counts = at_frame(counts, fs);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         counts[2] = c->nan;
         counts[3] = c->solve64;
@@ -411,8 +454,12 @@ struct StartPass {
 template <int KX>
 __global__ __launch_bounds__(256) void k_level_start(const Ctl* __restrict__ c, const float* __restrict__ cp,
                                                       int cp_start, const float* __restrict__ prev,
-                                                      float* __restrict__ m0, StartPass u, int nz) {
+                                                      float* __restrict__ m0, StartPass u, int nz, long long fs) {
     __shared__ double xr[kUpRows][256];
+    c = at_frame(c, fs);
+    cp = at_frame(cp, fs);
+    prev = at_frame(prev, fs);
+    m0 = at_frame(m0, fs);
     const unsigned b = blockIdx.x;
     const int xb = static_cast<int>(b % static_cast<unsigned>(u.xblocks));
     const unsigned rest = b / static_cast<unsigned>(u.xblocks);
@@ -518,13 +565,14 @@ LkTile plan_tile(const LevelGeom& g, int np_x) {
 }
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen, double n_vox,
-                      hipStream_t s) {
-    k_ctl_reset<<<1, 64, 0, s>>>(ctl, iter, frec, fupd, tol, s_pen, n_vox);
+                      const Frames& fr, hipStream_t s) {
+    k_ctl_reset<<<dim3(1, fr.n), 64, 0, s>>>(ctl, iter, frec, fupd, tol, s_pen, n_vox, fr.fs);
 }
 
-void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, hipStream_t s) {
+void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, const Frames& fr, hipStream_t s) {
     const long long ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
-    k_cp_init<<<blocks_for(ncp, 256), 256, 0, s>>>(m0, cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy);
+    k_cp_init<<<dim3(blocks_for(ncp, 256), fr.n), 256, 0, s>>>(m0, cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
+                                                               fr.fs);
 }
 
 int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool separate_stop,
@@ -549,21 +597,25 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
     if (b.events) (void)hipEventRecord(b.events[2 * k], s);
     const long long np = launch_lk_pass(g, t, b, k, dense, check_now, s);
     if (b.events) (void)hipEventRecord(b.events[2 * k + 1], s);
-    if (!fuse_err) k_err_finish<<<1, 1024, 0, s>>>(b.ctl, k, b.errpart, nerr, b.penpart, np, b.log, b.counts);
-    k_solve<<<cpb, 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d,
-                                b.store != ST_F32 ? 1 : 0, b.mdpart, k,
-                                fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts);
+    const long long fs = b.fr.fs;
+    const unsigned nf = static_cast<unsigned>(b.fr.n);
+    if (!fuse_err)
+        k_err_finish<<<dim3(1, nf), 1024, 0, s>>>(b.ctl, k, b.errpart, nerr, b.penpart, np, b.log, b.counts, fs);
+    k_solve<<<dim3(cpb, nf), 256, 0, s>>>(b.ctl, b.sums, b.dnei, b.cp, ncp, g.mov_range, g.z_ratio_d,
+                                          b.store != ST_F32 ? 1 : 0, b.mdpart, k,
+                                          fuse_err ? 1 : 0, b.errpart, nerr, b.penpart, np, b.log, b.counts, fs);
     if (dense) {
         const size_t lds = dense_lds_bytes(g);
         if (lds <= kDenseLdsMax && !b.maxdiff_gather)
-            k_dense_maxdiff_lds<<<nd, 256, lds, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
-                                                     b.mdpart + cpb);
+            k_dense_maxdiff_lds<<<dim3(nd, nf), 256, lds, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx,
+                                                               g.ncy, b.mdpart + cpb, fs);
         else
-            k_dense_maxdiff<<<nd, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
-                                               b.mdpart + cpb);
+            k_dense_maxdiff<<<dim3(nd, nf), 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
+                                                         b.mdpart + cpb, fs);
     }
     if (!fuse_check(k))
-        k_check<<<1, 1024, 0, s>>>(b.ctl, k, b.mdpart, static_cast<int>(parts_of(k)), b.log, b.counts);
+        k_check<<<dim3(1, nf), 1024, 0, s>>>(b.ctl, k, b.mdpart, static_cast<int>(parts_of(k)), b.log, b.counts,
+                                             fs);
     return WR_OK;
 }
 
@@ -573,7 +625,7 @@ bool level_start_fits(const Table& tx, const Table& ty, bool x_first, const Fuse
 
 void launch_level_start(const LevelGeom& g, const LevelGeom& gp, const Table& tx, const Table& ty,
                         const FusedY& fu, const double* w, const int* idx, const Ctl* ctl, const float* cp,
-                        int cp_start, const float* prev, float* m0, hipStream_t s) {
+                        int cp_start, const float* prev, float* m0, const Frames& fr, hipStream_t s) {
     StartPass u;
     u.wx = w + tx.w_off;
     u.ix = idx + tx.i_off;
@@ -592,23 +644,24 @@ void launch_level_start(const LevelGeom& g, const LevelGeom& gp, const Table& tx
     u.pp = gp.p;
     u.ncxp = gp.ncx;
     u.ncyp = gp.ncy;
-    const unsigned blocks = static_cast<unsigned>(static_cast<long long>(u.xblocks) * fu.nblk * g.nz);
+    const dim3 grid(static_cast<unsigned>(static_cast<long long>(u.xblocks) * fu.nblk * g.nz),
+                    static_cast<unsigned>(fr.n));
     switch (tx.k) {
         case 2:
-            k_level_start<2><<<blocks, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz);
+            k_level_start<2><<<grid, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz, fr.fs);
             break;
         case 3:
-            k_level_start<3><<<blocks, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz);
+            k_level_start<3><<<grid, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz, fr.fs);
             break;
         default:
-            k_level_start<4><<<blocks, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz);
+            k_level_start<4><<<grid, 256, 0, s>>>(ctl, cp, cp_start, prev, m0, u, g.nz, fr.fs);
     }
 }
 
 void launch_level_keep(const LevelGeom& g, const LkBuffers& b, float* prev, hipStream_t s) {
     const long long n = 3LL * g.nx * g.ny * g.nz;
     const unsigned blocks = static_cast<unsigned>(std::min<long long>(blocks_for(n, 256), 4096));
-    k_level_keep<<<blocks, 256, 0, s>>>(b.ctl, b.m0, prev, n, b.counts, b.cp_start);
+    k_level_keep<<<dim3(blocks, b.fr.n), 256, 0, s>>>(b.ctl, b.m0, prev, n, b.counts, b.cp_start, b.fr.fs);
 }
 
 void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter, float* dense_soa,
@@ -617,8 +670,9 @@ void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int ite
     (void)iter;
     const int xblocks = (g.nx + 255) / 256;
     const unsigned blocks = static_cast<unsigned>(static_cast<long long>(xblocks) * g.ny * g.nz);
-    k_level_end<<<blocks, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, xblocks,
-                                       dense_soa, motion_aos, coords_aos, b.counts, b.cp_start);
+    k_level_end<<<dim3(blocks, b.fr.n), 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
+                                                     xblocks, dense_soa, motion_aos, coords_aos, b.counts,
+                                                     b.cp_start, b.fr.fs, b.fr.ofs);
 }
 
 }  // namespace wr

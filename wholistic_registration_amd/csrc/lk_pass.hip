@@ -453,6 +453,7 @@ struct PassArgs {
     long long ncp;
     float z_ratio;
     float hx, hy, hz;      // level extents - 1
+    long long fs;          // frame stride of the per-frame buffers (Frames)
 };
 
 // Decode the XCD-major, z-fastest tile order.
@@ -476,7 +477,21 @@ __device__ __forceinline__ void tile_of(long long nb, int nz, int nbx, int& bx, 
 // gathers are issued before the current row is evaluated.
 template <bool DENSE, bool MREF, bool MMOV, int S>
 __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
+    a.ctl = at_frame(a.ctl, a.fs);
     if (a.ctl->done) return;
+    // frame blockIdx.y's buffers (the template and mask_ref are shared)
+    a.mov.base = at_frame(a.mov.base, a.fs);
+    a.q.base = at_frame(a.q.base, a.fs);
+    a.mmov.base = at_frame(a.mmov.base, a.fs);
+    a.m0 = at_frame(a.m0, a.fs);
+    a.cp = at_frame(a.cp, a.fs);
+    a.sums = at_frame(a.sums, a.fs);
+    a.errpart = at_frame(a.errpart, a.fs);
+    a.mdpart = at_frame(a.mdpart, a.fs);
+    a.log = at_frame(a.log, a.fs);
+    a.counts = at_frame(a.counts, a.fs);
+    a.dnei = at_frame(a.dnei, a.fs);
+    a.penpart = at_frame(a.penpart, a.fs);
     if (a.check_parts > 0) {
         // the |delta motion| stop test of the previous update, decided alike
         // by every workgroup from the same maxima; workgroup 0 records it
@@ -917,9 +932,10 @@ long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b
     a.blocks = t.blocks;
     a.ncp = static_cast<long long>(g.ncx) * g.ncy * g.nz;
     a.z_ratio = g.z_ratio;
+    a.fs = b.fr.fs;
     const int threads = 64 * t.nwx;
     a.npen = (a.ncp + threads - 1) / threads;
-    const dim3 grid(static_cast<unsigned>(t.blocks + a.npen));
+    const dim3 grid(static_cast<unsigned>(t.blocks + a.npen), static_cast<unsigned>(b.fr.n));
     const bool mr = b.mref != nullptr, mm = b.mmov != nullptr;
     using Fn = void (*)(PassArgs);
 #define WR_LK_TABLE(S)                                                                                   \

@@ -211,6 +211,22 @@ void launch_moving_records(const float* mov_padded, int nx, int ny, int nz, int 
 void launch_moving_records_input(const void* in, Elem t, int nx, int ny, int nz, int store, void* out,
                                  hipStream_t s);
 
+// Frames of one launch sequence: the launches of an iteration run
+// gridDim.y = n rows of workgroups, row f registering frame f.  Every
+// per-frame buffer of frame f sits f * fs bytes after frame 0's (one
+// workspace per frame); the returned fields f * ofs bytes apart.
+struct Frames {
+    int n = 1;
+    long long fs = 0;
+    long long ofs = 0;
+};
+
+// Frame blockIdx.y's copy of a per-frame buffer `p` (frame stride fs bytes).
+template <class T>
+__device__ __forceinline__ T* at_frame(T* p, long long fs) {
+    return p == nullptr ? p : (T*)((const char*)p + static_cast<long long>(blockIdx.y) * fs);
+}
+
 struct LkBuffers {
     int store;               // element type of ref and of movq's records (Store)
     const void* mov;         // level moving image, float32 padded volume (padded_layout), allocation start
@@ -232,6 +248,7 @@ struct LkBuffers {
     int cp_start = 0;        // 1: the level starts from the CP field itself (no dense field m0):
                              // iteration 0 is a CP pass and its |delta motion| test uses CP maxima
     int maxdiff_gather = 0;  // 1: the first update's dense |delta motion| maximum uses the gather kernel
+    Frames fr;               // frames of the launches (every pointer above but ref and mref is per frame)
 };
 
 // Partial-maximum slots needed by one iteration with `ncp` control points.
@@ -246,8 +263,8 @@ long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b
                          int check_parts, hipStream_t s);
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
-                      double inv_n, hipStream_t s);
-void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, hipStream_t s);
+                      double inv_n, const Frames& fr, hipStream_t s);
+void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, const Frames& fr, hipStream_t s);
 int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool separate_stop,
                      hipStream_t s);
 void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter,
@@ -260,7 +277,7 @@ void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int ite
 bool level_start_fits(const Table& tx, const Table& ty, bool x_first, const FusedY& fu);
 void launch_level_start(const LevelGeom& g, const LevelGeom& gp, const Table& tx, const Table& ty,
                         const FusedY& fu, const double* w, const int* idx, const Ctl* ctl, const float* cp,
-                        int cp_start, const float* prev, float* m0, hipStream_t s);
+                        int cp_start, const float* prev, float* m0, const Frames& fr, hipStream_t s);
 // End of a level above 0 followed by launch_level_start: its counts, and its
 // start field copied to `prev` where it made no update and did not start
 // from its CP field.
@@ -416,8 +433,8 @@ void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int hs, int
 
 struct Context {
     int device = 0;
-    DevBuf mov_l, movq, ref_l, tmp, tmp2, mref_l, mmov_l, m0, mprev;   // movq: records of the moving levels
-    DevBuf cp, dnei, sums, errpart, penpart, ctl, log, counts, mdpart;
+    DevBuf fws;                          // per-frame workspaces of wr_get_motion(_batch) (Frames)
+    DevBuf ref_l, tmp, tmp2, mref_l;     // template and mask_ref levels; resampling intermediates
     DevBuf tab_w, tab_i, pen;
     DevBuf msel, mlab, mcnt, mtmp;       // mask statistics, component labels / areas, byte mask
     DevBuf hsmall, hbig;                 // harness: statistics / Gram scratch, sampled frame matrix

@@ -16,6 +16,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
+from ._util import is_tensor as _is_tensor
 from ._util import torch_module as _torch
 
 
@@ -87,6 +88,63 @@ class MotionResult:
 PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
 
 
+def _params(layer, iter, r, zRatio, movRange, smoothPenalty, tol, tile_patches_x=0, stop_kernels=False,
+            precision="fp32", reuse_template=False, timing=False, schedule=None, resample_start=False,
+            maxdiff_gather=False):
+    """wr_params of a getMotion call, and the ctypes arrays it points to."""
+    prm = _lib.WrParams()
+    prm.layer, prm.iter, prm.r = int(layer), int(iter), int(r)
+    prm.zRatio, prm.movRange = float(zRatio), float(movRange)
+    prm.smoothPenalty, prm.tol = float(smoothPenalty), float(tol)
+    keep = []
+    if isinstance(tile_patches_x, (list, tuple)):
+        if len(tile_patches_x) != layer + 1:
+            raise ValueError(f"tile_patches_x needs one entry per level (layer + 1 = {layer + 1})")
+        arr = (ctypes.c_int * (layer + 1))(*[int(v) for v in tile_patches_x])
+        prm.tile_patches_level = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int))
+        keep.append(arr)
+    else:
+        prm.tile_patches_x = int(tile_patches_x)
+    prm.stop_kernels = 1 if stop_kernels else 0
+    prm.resample_start = 1 if resample_start else 0
+    prm.maxdiff_gather = 1 if maxdiff_gather else 0
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
+    prm.precision = PRECISIONS[precision]
+    prm.reuse_template = 1 if reuse_template else 0
+    prm.timing = 1 if timing else 0
+    if schedule is not None:
+        rec = (ctypes.c_int * (layer + 1))(*[schedule[L][0] for L in range(layer + 1)])
+        upd = (ctypes.c_int * (layer + 1))(*[schedule[L][1] for L in range(layer + 1)])
+        prm.forced_records = ctypes.cast(rec, ctypes.POINTER(ctypes.c_int))
+        prm.forced_updates = ctypes.cast(upd, ctypes.POINTER(ctypes.c_int))
+        keep += [rec, upd]
+    return prm, keep
+
+
+def _log_struct(nlev, iter):
+    """A wr_log whose arrays point into fresh (nlev, iter) float64 buffers."""
+    bufs = {k: np.zeros((nlev, max(iter, 1)), dtype=np.float64) for k in ("diff", "pen", "cur", "md")}
+    log = _lib.WrLog()
+    dp = ctypes.POINTER(ctypes.c_double)
+    log.diff_error = bufs["diff"].ctypes.data_as(dp)
+    log.penalty_error = bufs["pen"].ctypes.data_as(dp)
+    log.current_error = bufs["cur"].ctypes.data_as(dp)
+    log.max_diff_motion = bufs["md"].ctypes.data_as(dp)
+    return log, bufs
+
+
+def _result(motion, coords, log, bufs, layer):
+    nlev = layer + 1
+    return MotionResult(
+        motion=motion, coords=coords,
+        records=[log.records[L] for L in range(nlev)],
+        updates=[log.updates[L] for L in range(nlev)],
+        nans=[log.nans[L] for L in range(nlev)],
+        diff_error=bufs["diff"], penalty_error=bufs["pen"], current_error=bufs["cur"],
+        max_diff_motion=bufs["md"], final_error=float(log.final_error), layer=layer)
+
+
 def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
                schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
@@ -144,49 +202,15 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
         motion_out = torch.empty((nz, ny, nx, 3), dtype=torch.float32, device=dev)
     coords = torch.empty_like(motion_out) if want_coords else None
 
-    prm = _lib.WrParams()
-    prm.layer, prm.iter, prm.r = int(layer), int(iter), int(r)
-    prm.zRatio, prm.movRange = float(zRatio), float(movRange)
-    prm.smoothPenalty, prm.tol = float(smoothPenalty), float(tol)
-    keep = []
-    if isinstance(tile_patches_x, (list, tuple)):
-        if len(tile_patches_x) != layer + 1:
-            raise ValueError(f"tile_patches_x needs one entry per level (layer + 1 = {layer + 1})")
-        arr = (ctypes.c_int * (layer + 1))(*[int(v) for v in tile_patches_x])
-        prm.tile_patches_level = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int))
-        keep.append(arr)
-    else:
-        prm.tile_patches_x = int(tile_patches_x)
-    prm.stop_kernels = 1 if stop_kernels else 0
-    prm.resample_start = 1 if resample_start else 0
-    prm.maxdiff_gather = 1 if maxdiff_gather else 0
-    if precision not in PRECISIONS:
-        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, got {precision!r}")
-    prm.precision = PRECISIONS[precision]
-    prm.reuse_template = 1 if reuse_template else 0
+    prm, keep = _params(layer, iter, r, zRatio, movRange, smoothPenalty, tol, tile_patches_x, stop_kernels,
+                        precision, reuse_template, timing, schedule, resample_start, maxdiff_gather)
     if motion_init_cp is not None:
         prm.init_cp = 1
         motion_init = motion_init_cp
     elif motion_init is not None and tuple(motion_init.shape[1:3]) != (ny, nx):
         # a warm start of another x-y size (full-resolution units)
         prm.init_ny, prm.init_nx = int(motion_init.shape[1]), int(motion_init.shape[2])
-    prm.timing = 1 if timing else 0
-    if schedule is not None:
-        rec = (ctypes.c_int * (layer + 1))(*[schedule[L][0] for L in range(layer + 1)])
-        upd = (ctypes.c_int * (layer + 1))(*[schedule[L][1] for L in range(layer + 1)])
-        prm.forced_records = ctypes.cast(rec, ctypes.POINTER(ctypes.c_int))
-        prm.forced_updates = ctypes.cast(upd, ctypes.POINTER(ctypes.c_int))
-        keep += [rec, upd]
-    nlev = layer + 1
-    bufs = {k: np.zeros((nlev, max(iter, 1)), dtype=np.float64)
-            for k in ("diff", "pen", "cur", "md")}
-    log = _lib.WrLog()
-    dp = ctypes.POINTER(ctypes.c_double)
-    log.diff_error = bufs["diff"].ctypes.data_as(dp)
-    log.penalty_error = bufs["pen"].ctypes.data_as(dp)
-    log.current_error = bufs["cur"].ctypes.data_as(dp)
-    log.max_diff_motion = bufs["md"].ctypes.data_as(dp)
-
+    log, bufs = _log_struct(layer + 1, iter)
     lib = _lib.load()
     ctx = context if context is not None else _lib.context(dev.index if dev.index is not None else 0)
     strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
@@ -194,13 +218,58 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
                            _ptr(mask_ref), _ptr(mask_mov), _ptr(motion_init), ctypes.byref(prm),
                            _ptr(motion_out), _ptr(coords), ctypes.byref(log), strm)
     _lib.check(rc, "wr_get_motion")
-    return MotionResult(
-        motion=motion_out, coords=coords,
-        records=[log.records[L] for L in range(nlev)],
-        updates=[log.updates[L] for L in range(nlev)],
-        nans=[log.nans[L] for L in range(nlev)],
-        diff_error=bufs["diff"], penalty_error=bufs["pen"], current_error=bufs["cur"],
-        max_diff_motion=bufs["md"], final_error=float(log.final_error), layer=layer)
+    return _result(motion_out, coords, log, bufs, layer)
+
+
+def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
+                     mask_ref=None, mask_mov=None, motion_init=None, motion_out=None, tile_patches_x=0,
+                     timing=False, stop_kernels=False, precision="fp32", reuse_template=False, context=None,
+                     stream=None, schedule=None):
+    """getMotion of every frame of ``movs`` (F, Z, Y, X) against ``ref`` in one
+    launch sequence (wr_get_motion_batch): one launch per kernel and
+    level-iteration over all frames, each frame with its own stop rules.
+    Every frame's field equals ``get_motion(movs[f], ref, ...)`` bit for bit.
+    ``mask_mov`` (F, Z, Y, X) float32 and ``motion_init`` (F, Z, Y, X, 3)
+    are per frame; ``motion_out`` is (F, Z, Y, X, 3).  Returns one
+    MotionResult per frame (its ``motion`` a view of ``motion_out``)."""
+    torch = _torch()
+    if not _is_tensor(movs) or movs.dim() != 4 or not movs.is_cuda or not movs.is_contiguous():
+        raise ValueError("movs must be a contiguous (F, Z, Y, X) device tensor")
+    nf, nz, ny, nx = (int(v) for v in movs.shape)
+    _check_vol(ref, "ref")
+    if tuple(ref.shape) != (nz, ny, nx) or ref.dtype != movs.dtype:
+        raise ValueError("ref must be (Z, Y, X) with the frames' shape and dtype")
+    _check_vol(mask_ref, "mask_ref")
+    if mask_ref is not None and (tuple(mask_ref.shape) != (nz, ny, nx) or mask_ref.dtype != torch.float32):
+        raise ValueError("mask_ref must be float32 with the volume's shape")
+    if mask_mov is not None and (tuple(mask_mov.shape) != (nf, nz, ny, nx) or mask_mov.dtype != torch.float32
+                                 or not mask_mov.is_contiguous()):
+        raise ValueError("mask_mov must be contiguous float32 (F, Z, Y, X)")
+    if motion_init is not None and (tuple(motion_init.shape) != (nf, nz, ny, nx, 3)
+                                    or motion_init.dtype != torch.float32 or not motion_init.is_contiguous()):
+        raise ValueError("motion_init must be contiguous float32 (F, Z, Y, X, 3)")
+    if motion_out is None:
+        motion_out = torch.empty((nf, nz, ny, nx, 3), dtype=torch.float32, device=movs.device)
+    elif tuple(motion_out.shape) != (nf, nz, ny, nx, 3) or not motion_out.is_contiguous():
+        raise ValueError("motion_out must be contiguous float32 (F, Z, Y, X, 3)")
+    prm, keep = _params(layer, iter, r, zRatio, movRange, smoothPenalty, tol, tile_patches_x, stop_kernels,
+                        precision, reuse_template, timing, schedule)
+    logs = (_lib.WrLog * nf)()
+    bufs = []
+    for f in range(nf):
+        lg, b = _log_struct(layer + 1, iter)
+        logs[f] = lg
+        bufs.append(b)
+    lib = _lib.load()
+    ctx = context if context is not None else _lib.context(movs.device.index or 0)
+    strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    nvox = nz * ny * nx
+    rc = lib.wr_get_motion_batch(ctx, _ptr(movs), nvox, _ptr(ref), dtype_code(movs), nx, ny, nz, nf,
+                                 _ptr(mask_ref), _ptr(mask_mov), nvox, _ptr(motion_init), 3 * nvox,
+                                 ctypes.byref(prm), _ptr(motion_out), logs, strm)
+    _lib.check(rc, "wr_get_motion_batch")
+    del keep
+    return [_result(motion_out[f], None, logs[f], bufs[f], layer) for f in range(nf)]
 
 
 def cp_grid(shape_zyx, r, level=0):
