@@ -165,7 +165,11 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype,
  *   motion_init   NULL, or frame f at motion_init + f * init_stride floats
  *                 (the layout params->init_nx / init_ny / init_cp select)
  *   motion_out    (frames, Z, Y, X, 3) float32
- *   logs          NULL or `frames` host structs
+ *   logs          `frames` host structs, or NULL: then the call returns once
+ *                 its work is enqueued, without waiting for the stream (no
+ *                 logs, no timing); a call with the geometry, frames and
+ *                 buffers of an earlier call on the context allocates and
+ *                 uploads nothing and can be captured in a HIP graph
  * ref, mask_ref, params and stream as for wr_get_motion (params->reuse_template
  * applies to the template).  Device memory: one workspace per frame, about
  * 21 bytes per voxel (11 more with 16-bit level storage). */

@@ -119,13 +119,27 @@ Plan2 make_plan(int nx_in, int nx_out, int ny_in, int ny_out, Kernel k, std::vec
 
 // The tables are replaced with host-synchronous copies, which are not ordered
 // after work on a non-blocking stream: wait for the caller's stream first, so
-// that no kernel of an earlier call on it still reads the old tables.
+// that no kernel of an earlier call on it still reads the old tables.  Tables
+// equal to the ones on the device are not uploaded again (no host wait: a
+// call with the same geometry can be captured in a graph).
 int upload_tables(Context& c, const std::vector<double>& wa, const std::vector<int>& ia, hipStream_t s) {
+    if (c.tab_valid && wa == c.tab_wh && ia == c.tab_ih) return WR_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    WR_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+        set_error("resampling tables differ from the previous call's on this context: make one call of this "
+                  "geometry before capturing it");
+        return WR_E_ARG;
+    }
+    c.tab_valid = false;
     WR_HIP_CHECK(hipStreamSynchronize(s));
     if (int rc = grow(c.tab_w, std::max<size_t>(wa.size(), 1) * sizeof(double))) return rc;
     if (int rc = grow(c.tab_i, std::max<size_t>(ia.size(), 1) * sizeof(int))) return rc;
     if (!wa.empty()) WR_HIP_CHECK(hipMemcpy(c.tab_w.p, wa.data(), wa.size() * sizeof(double), hipMemcpyHostToDevice));
     if (!ia.empty()) WR_HIP_CHECK(hipMemcpy(c.tab_i.p, ia.data(), ia.size() * sizeof(int), hipMemcpyHostToDevice));
+    c.tab_wh = wa;
+    c.tab_ih = ia;
+    c.tab_valid = true;
     return WR_OK;
 }
 
@@ -360,6 +374,7 @@ namespace {
 struct FrameArgs {
     int n = 1;
     long long mov_stride = 0, mmov_stride = 0, init_stride = 0, out_stride = 0;
+    bool async = false;   // no logs: return once everything is enqueued
 };
 
 static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
@@ -697,6 +712,8 @@ static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int 
 
     c.tmpl = key;   // every template level of this call is enqueued
 
+    if (fa.async) return WR_OK;
+
     // ---- logs back to the host
     std::vector<double> hlog(static_cast<size_t>(layer + 1) * iter * 4 * fa.n);
     std::vector<int> hcnt(static_cast<size_t>(layer + 1) * 4 * fa.n);
@@ -779,6 +796,8 @@ int wr_get_motion_batch(wr_ctx* ctx, const void* mov, long long mov_stride, cons
     fa.mmov_stride = mask_mov_stride;
     fa.init_stride = init_stride;
     fa.out_stride = 3 * nvox;
+    fa.async = logs == nullptr;
+    WR_ARG(!(fa.async && prm && prm->timing), "wr_get_motion_batch: timing needs logs (the call then waits)");
     return get_motion_frames(ctx, mov, ref, dtype, nx, ny, nz, mask_ref, mask_mov, motion_init, prm, motion_out,
                              nullptr, logs, fa, stream);
 }

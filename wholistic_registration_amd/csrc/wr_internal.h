@@ -436,6 +436,9 @@ struct Context {
     DevBuf fws;                          // per-frame workspaces of wr_get_motion(_batch) (Frames)
     DevBuf ref_l, tmp, tmp2, mref_l;     // template and mask_ref levels; resampling intermediates
     DevBuf tab_w, tab_i, pen;
+    std::vector<double> tab_wh;          // host copies of the tables on the device
+    std::vector<int> tab_ih;
+    bool tab_valid = false;
     DevBuf msel, mlab, mcnt, mtmp;       // mask statistics, component labels / areas, byte mask
     DevBuf hsmall, hbig;                 // harness: statistics / Gram scratch, sampled frame matrix
     DevBuf splat;                        // splats: shift scratch and the weighted sums

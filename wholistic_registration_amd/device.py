@@ -224,14 +224,18 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
 def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                      mask_ref=None, mask_mov=None, motion_init=None, motion_out=None, tile_patches_x=0,
                      timing=False, stop_kernels=False, precision="fp32", reuse_template=False, context=None,
-                     stream=None, schedule=None):
+                     stream=None, schedule=None, logs=True):
     """getMotion of every frame of ``movs`` (F, Z, Y, X) against ``ref`` in one
     launch sequence (wr_get_motion_batch): one launch per kernel and
     level-iteration over all frames, each frame with its own stop rules.
     Every frame's field equals ``get_motion(movs[f], ref, ...)`` bit for bit.
     ``mask_mov`` (F, Z, Y, X) float32 and ``motion_init`` (F, Z, Y, X, 3)
     are per frame; ``motion_out`` is (F, Z, Y, X, 3).  Returns one
-    MotionResult per frame (its ``motion`` a view of ``motion_out``)."""
+    MotionResult per frame (its ``motion`` a view of ``motion_out``).  With
+    ``logs=False`` the call only enqueues the work on ``stream`` and returns
+    ``motion_out`` without waiting: repeated with the same tensors after a
+    first call, it allocates nothing and can be captured in a CUDA/HIP graph
+    (``torch.cuda.graph``)."""
     torch = _torch()
     if not _is_tensor(movs) or movs.dim() != 4 or not movs.is_cuda or not movs.is_contiguous():
         raise ValueError("movs must be a contiguous (F, Z, Y, X) device tensor")
@@ -254,12 +258,15 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
         raise ValueError("motion_out must be contiguous float32 (F, Z, Y, X, 3)")
     prm, keep = _params(layer, iter, r, zRatio, movRange, smoothPenalty, tol, tile_patches_x, stop_kernels,
                         precision, reuse_template, timing, schedule)
-    logs = (_lib.WrLog * nf)()
     bufs = []
-    for f in range(nf):
-        lg, b = _log_struct(layer + 1, iter)
-        logs[f] = lg
-        bufs.append(b)
+    if logs:
+        logs = (_lib.WrLog * nf)()
+        for f in range(nf):
+            lg, b = _log_struct(layer + 1, iter)
+            logs[f] = lg
+            bufs.append(b)
+    else:
+        logs = None
     lib = _lib.load()
     ctx = context if context is not None else _lib.context(movs.device.index or 0)
     strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
@@ -269,6 +276,8 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
                                  ctypes.byref(prm), _ptr(motion_out), logs, strm)
     _lib.check(rc, "wr_get_motion_batch")
     del keep
+    if logs is None:
+        return motion_out
     return [_result(motion_out[f], None, logs[f], bufs[f], layer) for f in range(nf)]
 
 
