@@ -480,16 +480,14 @@ def learned_measure(n_templates=16, per_template=HELD_OUT_FRAMES, reps=3, weight
         base = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=T.penalty(tmpl), tol=1e-3)
         ref_l = D.pyramid_level(tmpl, Lr.LEVEL)
 
+        # getMotion + correctMotion of the frame (the warp formed with the field)
         def plain(mov, layer, it):
-            r = D.get_motion(mov, tmpl, layer=layer, iter=it, motion_out=motion, **base)
-            D.correct_motion(mov, motion, out=warped)
-            return r
+            return D.get_motion(mov, tmpl, layer=layer, iter=it, motion_out=motion, warp_src=mov,
+                                warped_out=warped, **base)
 
         def learned(mov, precision, fallback=None):
-            r, info = Lr.register_learned(mov, tmpl, net, ref_level=ref_l, precision=precision, motion_out=motion,
-                                          fallback=fallback, **Lr.SCHEDULE, **base)
-            D.correct_motion(mov, motion, out=warped)
-            return r, info
+            return Lr.register_learned(mov, tmpl, net, ref_level=ref_l, precision=precision, motion_out=motion,
+                                       fallback=fallback, warp_src=mov, warped_out=warped, **Lr.SCHEDULE, **base)
 
         for i in range(1, per_template + 1):
             mov, d = T.make_moving(clean, tseed + i, dev)
@@ -823,10 +821,10 @@ def main():
                   smoothPenalty=sp, tol=1e-3, tile_patches_x=args.tile_patches_x)
 
     def step(timing, precision="fp32", motion_out=motion):
-        res = D.get_motion(mov, tmpl, mask_ref=mask, motion_out=motion_out, timing=timing,
-                           precision=precision, **params)
-        D.correct_motion(mov, motion_out, out=warped)
-        return res
+        # getMotion + correctMotion of the moving volume, the warp formed in the
+        # kernel that writes the field (wr_get_motion_warp)
+        return D.get_motion(mov, tmpl, mask_ref=mask, motion_out=motion_out, timing=timing,
+                            precision=precision, warp_src=mov, warped_out=warped, **params)
 
     for i in range(args.warmup):
         res = step(False)

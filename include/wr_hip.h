@@ -165,6 +165,9 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype,
  *   motion_init   NULL, or frame f at motion_init + f * init_stride floats
  *                 (the layout params->init_nx / init_ny / init_cp select)
  *   motion_out    (frames, Z, Y, X, 3) float32
+ *   warp_src      NULL, or frame f's volume to warp (correctMotion) at
+ *                 warp_src + f * warp_stride elements of warp_dtype; the
+ *                 warps go to warped_out, (frames, Z, Y, X) float32
  *   logs          `frames` host structs, or NULL: then the call returns once
  *                 its work is enqueued, without waiting for the stream (no
  *                 logs, no timing); a call with the geometry, frames and
@@ -179,7 +182,23 @@ int wr_get_motion_batch(wr_ctx* ctx, const void* mov, long long mov_stride,
                         const float* mask_mov, long long mask_mov_stride,
                         const float* motion_init, long long init_stride,
                         const wr_params* params, float* motion_out,
+                        const void* warp_src, int warp_dtype,
+                        long long warp_stride, float* warped_out,
                         wr_log* logs, void* stream);
+
+/* wr_get_motion followed by wr_correct_motion(src, motion_out) -- the
+ * harness's getMotion / correctMotion pair (registration.py:320-325) -- with
+ * the warp formed in the kernel that writes the level-0 field, so the field
+ * is not read back.  `src` (Z,Y,X) of `src_dtype` (WR_F32 or WR_U16) is the
+ * volume correctMotion warps (the moving volume, or another channel of it);
+ * warped_out (Z,Y,X) float32.  Both outputs equal the two separate calls'
+ * bit for bit. */
+int wr_get_motion_warp(wr_ctx* ctx, const void* mov, const void* ref, int dtype,
+                       int nx, int ny, int nz,
+                       const float* mask_ref, const float* mask_mov,
+                       const float* motion_init, const wr_params* params,
+                       float* motion_out, const void* src, int src_dtype,
+                       float* warped_out, wr_log* log, void* stream);
 
 /* Replaces correctMotion(data_raw, motion_field)
  * (calFlow3d_Wei_v1.py:676-706): out(p) = trilinear(in, p + motion(p)),

@@ -192,3 +192,31 @@ def test_register_batched_equals_register(cuda):
     torch.cuda.synchronize()
     assert torch.equal(m1, m2) and torch.equal(w1, w2)
     assert [r.records for r in r1] == [r.records for r in r2]
+
+
+@pytest.mark.parametrize("src_u16", [True, False])
+def test_fused_warp_equals_separate_calls(cuda, src_u16):
+    """wr_get_motion_warp / the batched warp: the field and the warp of a
+    volume by it (formed in the kernel that writes the field) equal
+    getMotion then correctMotion, bit for bit; the warped volume may be
+    another channel than the registered one."""
+    import torch
+
+    from wholistic_registration_amd import device as D
+    tmpl, ref, stack, movs = _frames_case(cuda, 915, frames=3, uint16=True)
+    other = torch.from_numpy(np.ascontiguousarray(movs[::-1])).to(cuda)      # another channel
+    src = other if src_u16 else other.float().contiguous()
+    mref = torch.from_numpy((tmpl > 2000).astype(np.float32)).to(cuda)
+    prm = dict(layer=2, iter=8, r=3, zRatio=1.0, movRange=5.0, smoothPenalty=50.0, tol=1e-3, mask_ref=mref)
+    res = D.get_motion_batch(stack, ref, warp_src=src, **prm)
+    warped = torch.empty(stack.shape, dtype=torch.float32, device=cuda)
+    res2 = D.get_motion_batch(stack, ref, warp_src=src, warped_out=warped, **prm)
+    for f in range(stack.shape[0]):
+        one = D.get_motion(stack[f], ref, warp_src=src[f], **prm)
+        plain = D.get_motion(stack[f], ref, **prm)
+        sep = D.correct_motion(src[f], plain.motion)
+        torch.cuda.synchronize()
+        _same_result(one, plain)
+        assert torch.equal(one.extras["warped"], sep)
+        assert torch.equal(res[f].motion, plain.motion) and torch.equal(res2[f].motion, plain.motion)
+        assert torch.equal(warped[f], sep)

@@ -80,7 +80,10 @@ _SIGNATURES = {
     "wr_get_motion": ([_VP, _VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP, ctypes.POINTER(WrParams),
                        _VP, _VP, ctypes.POINTER(WrLog), _VP], _I),
     "wr_get_motion_batch": ([_VP, _VP, ctypes.c_longlong, _VP, _I, _I, _I, _I, _I, _VP, _VP, ctypes.c_longlong,
-                             _VP, ctypes.c_longlong, ctypes.POINTER(WrParams), _VP, ctypes.POINTER(WrLog), _VP], _I),
+                             _VP, ctypes.c_longlong, ctypes.POINTER(WrParams), _VP, _VP, _I, ctypes.c_longlong, _VP,
+                             ctypes.POINTER(WrLog), _VP], _I),
+    "wr_get_motion_warp": ([_VP, _VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP, ctypes.POINTER(WrParams),
+                            _VP, _VP, _I, _VP, ctypes.POINTER(WrLog), _VP], _I),
     "wr_correct_motion": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP, _VP], _I),
     "wr_pyramid_level": ([_VP, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_imresize": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),

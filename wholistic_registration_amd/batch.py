@@ -122,6 +122,8 @@ class FrameBatch:
                         # layer, precision and mask: the context's template levels stay valid
                         extra["tile_patches_x"] = self.tiles.tiles(frames[i], ref, context=ctx, stream=strm,
                                                                    **params)
+                    if warped_out is not None:   # correctMotion formed with the field
+                        extra.update(warp_src=frames[i], warped_out=warped_out[i])
                     if forks is not None:
                         results[i], info = _learned.register_learned(
                             frames[i], ref, forks[k], ref_level=ref_level, motion_out=motion_out[i],
@@ -133,8 +135,6 @@ class FrameBatch:
                             frames[i], ref, motion_out=motion_out[i], context=ctx, stream=strm,
                             reuse_template=reuse, mask_mov=None if masks_mov is None else masks_mov[i], **extra)
                         self._ref_seen[k] = key
-                    if warped_out is not None:
-                        _dev.correct_motion(frames[i], motion_out[i], out=warped_out[i], context=ctx, stream=strm)
                 strm.synchronize()
             except BaseException as e:          # re-raised on the calling thread
                 self._ref_seen[k] = None
@@ -185,13 +185,11 @@ class FrameBatch:
                     res = _dev.get_motion_batch(
                         frames[c0:c1], ref, motion_out=motion_out[c0:c1], context=ctx, stream=strm,
                         reuse_template=self._ref_seen[k] == key,
-                        mask_mov=None if masks_mov is None else masks_mov[c0:c1], **params)
+                        mask_mov=None if masks_mov is None else masks_mov[c0:c1],
+                        warp_src=None if warped_out is None else frames[c0:c1],
+                        warped_out=None if warped_out is None else warped_out[c0:c1], **params)
                     self._ref_seen[k] = key
                     results[c0:c1] = res
-                    if warped_out is not None:
-                        for i in range(c0, c1):
-                            _dev.correct_motion(frames[i], motion_out[i], out=warped_out[i], context=ctx,
-                                                stream=strm)
                 strm.synchronize()
             except BaseException as e:          # re-raised on the calling thread
                 self._ref_seen[k] = None

@@ -375,6 +375,7 @@ struct FrameArgs {
     int n = 1;
     long long mov_stride = 0, mmov_stride = 0, init_stride = 0, out_stride = 0;
     bool async = false;   // no logs: return once everything is enqueued
+    WarpOut warp;         // warp.src set: correctMotion of it in the level-0 end
 };
 
 static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
@@ -706,6 +707,7 @@ static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int 
         }
         if (L > 0 && start_from_cp(L - 1)) launch_level_keep(g, b, mprev, s);
         else if (L > 0) launch_level_end(g, b, L, iter, mprev, nullptr, nullptr, s);
+        else if (fa.warp.src) launch_level_end_warp(g, b, motion_out, coords_out, fa.warp, s);
         else launch_level_end(g, b, L, iter, nullptr, motion_out, coords_out, s);
         WR_HIP_CHECK(hipGetLastError());
     }
@@ -779,10 +781,25 @@ int wr_get_motion(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int 
                              coords_out, log, FrameArgs{}, stream);
 }
 
+int wr_get_motion_warp(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
+                       const float* mask_ref, const float* mask_mov, const float* motion_init, const wr_params* prm,
+                       float* motion_out, const void* src, int src_dtype, float* warped_out, wr_log* log,
+                       void* stream) {
+    WR_ARG(src && warped_out, "wr_get_motion_warp: null argument");
+    WR_ARG(src_dtype == WR_F32 || src_dtype == WR_U16, "wr_get_motion_warp: src_dtype must be WR_F32 or WR_U16");
+    FrameArgs fa;
+    fa.warp.src = src;
+    fa.warp.src_t = elem_of(src_dtype);
+    fa.warp.out = warped_out;
+    return get_motion_frames(ctx, mov, ref, dtype, nx, ny, nz, mask_ref, mask_mov, motion_init, prm, motion_out,
+                             nullptr, log, fa, stream);
+}
+
 int wr_get_motion_batch(wr_ctx* ctx, const void* mov, long long mov_stride, const void* ref, int dtype, int nx,
                         int ny, int nz, int frames, const float* mask_ref, const float* mask_mov,
                         long long mask_mov_stride, const float* motion_init, long long init_stride,
-                        const wr_params* prm, float* motion_out, wr_log* logs, void* stream) {
+                        const wr_params* prm, float* motion_out, const void* warp_src, int warp_dtype,
+                        long long warp_stride, float* warped_out, wr_log* logs, void* stream) {
     WR_ARG(frames >= 1, "wr_get_motion_batch: frames must be at least 1");
     WR_ARG(nx > 0 && ny > 0 && nz > 0, "wr_get_motion: empty volume");
     const long long nvox = static_cast<long long>(nx) * ny * nz;
@@ -796,6 +813,16 @@ int wr_get_motion_batch(wr_ctx* ctx, const void* mov, long long mov_stride, cons
     fa.mmov_stride = mask_mov_stride;
     fa.init_stride = init_stride;
     fa.out_stride = 3 * nvox;
+    if (warp_src) {
+        WR_ARG(warped_out, "wr_get_motion_batch: warp_src needs warped_out");
+        WR_ARG(warp_dtype == WR_F32 || warp_dtype == WR_U16, "wr_get_motion_batch: warp_dtype must be WR_F32 or WR_U16");
+        WR_ARG(frames == 1 || warp_stride >= nvox, "wr_get_motion_batch: frames of warp_src overlap");
+        fa.warp.src = warp_src;
+        fa.warp.src_t = elem_of(warp_dtype);
+        fa.warp.src_fs = warp_stride * (warp_dtype == WR_U16 ? 2 : 4);
+        fa.warp.out = warped_out;
+        fa.warp.out_fs = nvox * static_cast<long long>(sizeof(float));
+    }
     fa.async = logs == nullptr;
     WR_ARG(!(fa.async && prm && prm->timing), "wr_get_motion_batch: timing needs logs (the call then waits)");
     return get_motion_frames(ctx, mov, ref, dtype, nx, ny, nz, mask_ref, mask_mov, motion_init, prm, motion_out,

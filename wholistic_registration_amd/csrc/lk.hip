@@ -336,15 +336,22 @@ __global__ __launch_bounds__(1024) void k_check(Ctl* __restrict__ c, int k, cons
 // Dense field at the end of a level: the CP interpolant when the level made
 // at least one update, else its initial field.  Writes the 3-plane layout
 // (for the next level's upsample) and/or the interleaved (Z,Y,X,3) layout
-// (the returned motion), and optionally motion + grid.
+// (the returned motion), and optionally motion + grid; with WARP_T >= 0
+// (the element type of `src`) also the warp of `src` by the field
+// (correctMotion, warp_taps / warp_value as k_warp forms it), so the field is
+// not read back.  A thread owns one column of kWarpRows consecutive rows
+// (the rows of a workgroup are uniform); the x interpolation of the CP rows
+// (the inner lerps of cp_bilinear) is formed once per CP row and z.
 typedef float f3u __attribute__((ext_vector_type(3), aligned(4)));
 
+template <int WARP_T>
 __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, const float* __restrict__ m0,
                                                     const float* __restrict__ cp, int nx, int ny, int nz, int r,
                                                     int p, int ncx, int ncy, int xblocks, float* __restrict__ soa,
                                                     float* __restrict__ aos, float* __restrict__ coords,
-                                                    int* __restrict__ counts, int cp_start, long long fs,
-                                                    long long ofs) {
+                                                    int* __restrict__ counts, int cp_start,
+                                                    const void* __restrict__ src, float* __restrict__ wout,
+                                                    long long fs, long long ofs, long long sfs, long long wfs) {
     c = at_frame(c, fs);
     m0 = at_frame(m0, fs);
     cp = at_frame(cp, fs);
@@ -352,42 +359,76 @@ __global__ __launch_bounds__(256) void k_level_end(const Ctl* __restrict__ c, co
     counts = at_frame(counts, fs);
     aos = at_frame(aos, ofs);
     coords = at_frame(coords, ofs);
+    src = at_frame(src, sfs);
+    wout = at_frame(wout, wfs);
     const unsigned b = blockIdx.x;
-    const unsigned row = b / static_cast<unsigned>(xblocks);
-    const int x = static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x);
+    const unsigned rg = b / static_cast<unsigned>(xblocks);
+    const int x = static_cast<int>(b - rg * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x);
     if (b == 0 && threadIdx.x == 0) {
         counts[2] = c->nan;
         counts[3] = c->solve64;
     }
     if (x >= nx) return;
-    const int y = static_cast<int>(row % static_cast<unsigned>(ny));
-    const int z = static_cast<int>(row / static_cast<unsigned>(ny));
+    const unsigned nrows = static_cast<unsigned>(ny) * static_cast<unsigned>(nz);
     const size_t nvox = static_cast<size_t>(nx) * ny * nz;
     const size_t ncp = static_cast<size_t>(ncx) * ncy * nz;
-    const size_t v = static_cast<size_t>(row) * nx + x;
-    float m[3];
-    if (c->upd > 0 || cp_start) {   // the CP interpolant (a level without updates keeps its start)
-        const Axis1 ax = cp_axis(x, r, p, ncx), ay = cp_axis(y, r, p, ncy);
-        const float* base = cp + static_cast<size_t>(z) * ncy * ncx;
-        for (int q = 0; q < 3; ++q) m[q] = cp_bilinear(base + q * ncp, ncx, ax, ay);
-    } else {
-        for (int q = 0; q < 3; ++q) m[q] = m0[v + q * nvox];
+    const bool interp = c->upd > 0 || cp_start;   // the CP interpolant (a level without updates keeps its start)
+    const Axis1 ax = cp_axis(x, r, p, ncx);
+    float m[kWarpRows][3];
+    WarpTaps w[kWarpRows];
+    int cz = -1, c0 = -1, c1 = -1;
+    float h0[3], h1[3];
+#pragma unroll
+    for (int j = 0; j < kWarpRows; ++j) {
+        const unsigned row = min(rg * kWarpRows + j, nrows - 1);
+        const int y = static_cast<int>(row % static_cast<unsigned>(ny));
+        const int z = static_cast<int>(row / static_cast<unsigned>(ny));
+        if (interp) {
+            const Axis1 ay = cp_axis(y, r, p, ncy);
+            if (z != cz || ay.i0 != c0) {   // uniform: the rows of a workgroup are shared
+                const float* base = cp + static_cast<size_t>(z) * ncy * ncx;
+                const bool keep = z == cz && ay.i0 == c1;
+                for (int q = 0; q < 3; ++q) {
+                    const float* a0 = base + q * ncp + ay.i0 * ncx;
+                    const float* a1 = base + q * ncp + ay.i1 * ncx;
+                    h0[q] = keep ? h1[q] : lerpf(a0[ax.i0], a0[ax.i1], ax.t);
+                    h1[q] = lerpf(a1[ax.i0], a1[ax.i1], ax.t);
+                }
+                cz = z;
+                c0 = ay.i0;
+                c1 = ay.i1;
+            }
+            for (int q = 0; q < 3; ++q) m[j][q] = lerpf(h0[q], h1[q], ay.t);
+        } else {
+            const size_t v = static_cast<size_t>(row) * nx + x;
+            for (int q = 0; q < 3; ++q) m[j][q] = m0[v + q * nvox];
+        }
+        if constexpr (WARP_T >= 0) w[j] = warp_taps<WARP_T>(src, nx, ny, nz, x, y, z, m[j][0], m[j][1], m[j][2]);
     }
-    if (soa)
-        for (int q = 0; q < 3; ++q) soa[v + q * nvox] = m[q];
-    if (aos) {
-        f3u o;
-        o.x = m[0];
-        o.y = m[1];
-        o.z = m[2];
-        *reinterpret_cast<f3u*>(aos + 3 * v) = o;
-    }
-    if (coords) {
-        f3u o;
-        o.x = static_cast<float>(x) + m[0];
-        o.y = static_cast<float>(y) + m[1];
-        o.z = static_cast<float>(z) + m[2];
-        *reinterpret_cast<f3u*>(coords + 3 * v) = o;
+#pragma unroll
+    for (int j = 0; j < kWarpRows; ++j) {
+        const unsigned row = rg * kWarpRows + j;
+        if (row >= nrows) break;
+        const size_t v = static_cast<size_t>(row) * nx + x;
+        if (soa)
+            for (int q = 0; q < 3; ++q) soa[v + q * nvox] = m[j][q];
+        if (aos) {
+            f3u o;
+            o.x = m[j][0];
+            o.y = m[j][1];
+            o.z = m[j][2];
+            *reinterpret_cast<f3u*>(aos + 3 * v) = o;
+        }
+        if (coords) {
+            const int y = static_cast<int>(row % static_cast<unsigned>(ny));
+            const int z = static_cast<int>(row / static_cast<unsigned>(ny));
+            f3u g;
+            g.x = static_cast<float>(x) + m[j][0];
+            g.y = static_cast<float>(y) + m[j][1];
+            g.z = static_cast<float>(z) + m[j][2];
+            *reinterpret_cast<f3u*>(coords + 3 * v) = g;
+        }
+        if constexpr (WARP_T >= 0) wout[v] = warp_value(w[j]);
     }
 }
 
@@ -664,15 +705,27 @@ void launch_level_keep(const LevelGeom& g, const LkBuffers& b, float* prev, hipS
     k_level_keep<<<dim3(blocks, b.fr.n), 256, 0, s>>>(b.ctl, b.m0, prev, n, b.counts, b.cp_start, b.fr.fs);
 }
 
+void launch_level_end_warp(const LevelGeom& g, const LkBuffers& b, float* motion_aos, float* coords_aos,
+                           const WarpOut& w, hipStream_t s) {
+    const int xblocks = (g.nx + 255) / 256;
+    const long long groups = (static_cast<long long>(g.ny) * g.nz + kWarpRows - 1) / kWarpRows;
+    const dim3 grid(static_cast<unsigned>(static_cast<long long>(xblocks) * groups), static_cast<unsigned>(b.fr.n));
+    auto kern = w.src == nullptr ? k_level_end<-1> : (w.src_t == E_U16 ? k_level_end<E_U16> : k_level_end<E_F32>);
+    kern<<<grid, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, xblocks, nullptr,
+                              motion_aos, coords_aos, b.counts, b.cp_start, w.src, w.out, b.fr.fs, b.fr.ofs,
+                              w.src_fs, w.out_fs);
+}
+
 void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter, float* dense_soa,
                       float* motion_aos, float* coords_aos, hipStream_t s) {
     (void)level;
     (void)iter;
     const int xblocks = (g.nx + 255) / 256;
-    const unsigned blocks = static_cast<unsigned>(static_cast<long long>(xblocks) * g.ny * g.nz);
-    k_level_end<<<dim3(blocks, b.fr.n), 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy,
-                                                     xblocks, dense_soa, motion_aos, coords_aos, b.counts,
-                                                     b.cp_start, b.fr.fs, b.fr.ofs);
+    const long long groups = (static_cast<long long>(g.ny) * g.nz + kWarpRows - 1) / kWarpRows;
+    const dim3 grid(static_cast<unsigned>(static_cast<long long>(xblocks) * groups), static_cast<unsigned>(b.fr.n));
+    k_level_end<-1><<<grid, 256, 0, s>>>(b.ctl, b.m0, b.cp, g.nx, g.ny, g.nz, g.r, g.p, g.ncx, g.ncy, xblocks,
+                                         dense_soa, motion_aos, coords_aos, b.counts, b.cp_start, nullptr, nullptr,
+                                         b.fr.fs, b.fr.ofs, 0, 0);
 }
 
 }  // namespace wr

@@ -269,6 +269,11 @@ int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, in
                      hipStream_t s);
 void launch_level_end(const LevelGeom& g, const LkBuffers& b, int level, int iter,
                       float* dense_soa, float* motion_aos, float* coords_aos, hipStream_t s);
+struct WarpOut;
+// The level-0 end (the returned field and, optionally, motion + grid) with
+// the warp of `w.src` by the same field (correctMotion) in the same kernel.
+void launch_level_end_warp(const LevelGeom& g, const LkBuffers& b, float* motion_aos, float* coords_aos,
+                           const WarpOut& w, hipStream_t s);
 // Start field of level L (geometry g) from the end of level L + 1 (gp): the
 // bilinear upsample (tables tx, ty, blocking fu of plan_up_y) of that level's
 // CP interpolant, formed from `cp` on the fly, or of `prev` where that level
@@ -285,8 +290,74 @@ void launch_level_keep(const LevelGeom& g, const LkBuffers& b, float* prev, hipS
 
 // ---------------------------------------------------------------- warp
 
+// Pull warp of one voxel, correctMotion (calFlow3d_Wei_v1.py:676-706):
+// trilinear interpolation of `in` at (x, y, z) + m with map_coordinates'
+// nearest mode (the coordinate clamped to the volume).  warp_taps issues the
+// eight corner loads, warp_value combines them; k_warp and the fused level-0
+// end (launch_level_end with a warp) share both, so their results agree bit
+// for bit.
+struct WarpTaps {
+    float v[8];
+    float t[3];
+};
+
+template <int IN_T>
+__device__ __forceinline__ float warp_ld(const void* p, size_t i) {
+    if constexpr (IN_T == E_F32) return static_cast<const float*>(p)[i];
+    else return static_cast<float>(static_cast<const uint16_t*>(p)[i]);
+}
+
+template <int IN_T>
+__device__ __forceinline__ WarpTaps warp_taps(const void* __restrict__ in, int nx, int ny, int nz, int x, int y,
+                                              int z, float mx, float my, float mz) {
+    WarpTaps w;
+    const float cx = fminf(fmaxf(static_cast<float>(x) + mx, 0.0f), static_cast<float>(nx - 1));
+    const float cy = fminf(fmaxf(static_cast<float>(y) + my, 0.0f), static_cast<float>(ny - 1));
+    const float cz = fminf(fmaxf(static_cast<float>(z) + mz, 0.0f), static_cast<float>(nz - 1));
+    const int fx = static_cast<int>(floorf(cx)), fy = static_cast<int>(floorf(cy)), fz = static_cast<int>(floorf(cz));
+    w.t[0] = cx - fx;
+    w.t[1] = cy - fy;
+    w.t[2] = cz - fz;
+    const int x1 = min(fx + 1, nx - 1), y1 = min(fy + 1, ny - 1), z1 = min(fz + 1, nz - 1);
+    const size_t r00 = (static_cast<size_t>(fz) * ny + fy) * nx, r01 = (static_cast<size_t>(fz) * ny + y1) * nx;
+    const size_t r10 = (static_cast<size_t>(z1) * ny + fy) * nx, r11 = (static_cast<size_t>(z1) * ny + y1) * nx;
+    w.v[0] = warp_ld<IN_T>(in, r00 + fx);
+    w.v[1] = warp_ld<IN_T>(in, r00 + x1);
+    w.v[2] = warp_ld<IN_T>(in, r01 + fx);
+    w.v[3] = warp_ld<IN_T>(in, r01 + x1);
+    w.v[4] = warp_ld<IN_T>(in, r10 + fx);
+    w.v[5] = warp_ld<IN_T>(in, r10 + x1);
+    w.v[6] = warp_ld<IN_T>(in, r11 + fx);
+    w.v[7] = warp_ld<IN_T>(in, r11 + x1);
+    return w;
+}
+
+__device__ __forceinline__ float warp_value(const WarpTaps& w) {
+    const float a = __builtin_fmaf(w.t[1], __builtin_fmaf(w.t[0], w.v[3] - w.v[2], w.v[2]) -
+                                               __builtin_fmaf(w.t[0], w.v[1] - w.v[0], w.v[0]),
+                                   __builtin_fmaf(w.t[0], w.v[1] - w.v[0], w.v[0]));
+    const float c = __builtin_fmaf(w.t[1], __builtin_fmaf(w.t[0], w.v[7] - w.v[6], w.v[6]) -
+                                               __builtin_fmaf(w.t[0], w.v[5] - w.v[4], w.v[4]),
+                                   __builtin_fmaf(w.t[0], w.v[5] - w.v[4], w.v[4]));
+    return __builtin_fmaf(w.t[2], c - a, a);
+}
+
+// Rows of the warp handled by one thread (k_warp and the fused level-0 end).
+constexpr int kWarpRows = 4;
+
 void launch_warp(const void* in, Elem in_t, int nx, int ny, int nz, const float* motion,
                  float* out, hipStream_t s);
+
+// The warp the level-0 end forms with the field it writes: frame f's source
+// at src + f * src_fs bytes (element type src_t), its result at
+// out + f * out_fs bytes.
+struct WarpOut {
+    const void* src = nullptr;
+    Elem src_t = E_F32;
+    long long src_fs = 0;
+    float* out = nullptr;
+    long long out_fs = 0;
+};
 
 // ---------------------------------------------------------------- penalty
 

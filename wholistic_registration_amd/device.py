@@ -149,7 +149,8 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
                mask_ref=None, mask_mov=None, motion_init=None, want_coords=False,
                schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
                stop_kernels=False, precision="fp32", reuse_template=False, context=None, stream=None,
-               motion_init_cp=None, resample_start=False, maxdiff_gather=False):
+               motion_init_cp=None, resample_start=False, maxdiff_gather=False, warp_src=None,
+               warped_out=None):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
@@ -166,7 +167,11 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     default to the device's shared context and torch's current stream.
     ``tile_patches_x``: patches per LK workgroup along x, one value for every
     level or a list of ``layer + 1`` values (level 0 first; 0 = the built-in
-    rule), as ``autotune.tune_tiles`` picks them.  ``motion_init`` may have
+    rule), as ``autotune.tune_tiles`` picks them.  ``warp_src`` (a (Z, Y, X)
+    float32 or uint16 volume, e.g. ``mov``) is also warped by the returned
+    field (``correctMotion``, calFlow3d_Wei_v1.py:676-706) into ``warped_out``
+    (float32, allocated when None; ``result.extras["warped"]``) in the kernel
+    that writes the field (wr_get_motion_warp).  ``motion_init`` may have
     another x-y size than the volume (full-resolution voxel units; the
     reference's warm start resamples it to the coarsest level).
     ``motion_init_cp`` (instead of ``motion_init``): the coarsest level's
@@ -214,6 +219,25 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     lib = _lib.load()
     ctx = context if context is not None else _lib.context(dev.index if dev.index is not None else 0)
     strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    if warp_src is not None:
+        _check_vol(warp_src, "warp_src")
+        if tuple(warp_src.shape) != (nz, ny, nx):
+            raise ValueError("warp_src must have the volume's shape")
+        if want_coords:
+            raise ValueError("want_coords and warp_src together are not supported")
+        if warped_out is None:
+            warped_out = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+        _check_vol(warped_out, "warped_out")
+        if tuple(warped_out.shape) != (nz, ny, nx) or warped_out.dtype != torch.float32:
+            raise ValueError("warped_out must be float32 with the volume's shape")
+        rc = lib.wr_get_motion_warp(ctx, _ptr(mov), _ptr(ref), dtype_code(mov), nx, ny, nz,
+                                    _ptr(mask_ref), _ptr(mask_mov), _ptr(motion_init), ctypes.byref(prm),
+                                    _ptr(motion_out), _ptr(warp_src), dtype_code(warp_src), _ptr(warped_out),
+                                    ctypes.byref(log), strm)
+        _lib.check(rc, "wr_get_motion_warp")
+        res = _result(motion_out, coords, log, bufs, layer)
+        res.extras["warped"] = warped_out
+        return res
     rc = lib.wr_get_motion(ctx, _ptr(mov), _ptr(ref), dtype_code(mov), nx, ny, nz,
                            _ptr(mask_ref), _ptr(mask_mov), _ptr(motion_init), ctypes.byref(prm),
                            _ptr(motion_out), _ptr(coords), ctypes.byref(log), strm)
@@ -224,7 +248,7 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
 def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                      mask_ref=None, mask_mov=None, motion_init=None, motion_out=None, tile_patches_x=0,
                      timing=False, stop_kernels=False, precision="fp32", reuse_template=False, context=None,
-                     stream=None, schedule=None, logs=True):
+                     stream=None, schedule=None, logs=True, warp_src=None, warped_out=None):
     """getMotion of every frame of ``movs`` (F, Z, Y, X) against ``ref`` in one
     launch sequence (wr_get_motion_batch): one launch per kernel and
     level-iteration over all frames, each frame with its own stop rules.
@@ -232,6 +256,9 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
     ``mask_mov`` (F, Z, Y, X) float32 and ``motion_init`` (F, Z, Y, X, 3)
     are per frame; ``motion_out`` is (F, Z, Y, X, 3).  Returns one
     MotionResult per frame (its ``motion`` a view of ``motion_out``).  With
+    ``warp_src`` (F, Z, Y, X) float32 or uint16 (e.g. ``movs``) is warped by
+    each frame's field into ``warped_out`` (F, Z, Y, X) float32 in the kernel
+    that writes the field.  With
     ``logs=False`` the call only enqueues the work on ``stream`` and returns
     ``motion_out`` without waiting: repeated with the same tensors after a
     first call, it allocates nothing and can be captured in a CUDA/HIP graph
@@ -258,6 +285,15 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
         raise ValueError("motion_out must be contiguous float32 (F, Z, Y, X, 3)")
     prm, keep = _params(layer, iter, r, zRatio, movRange, smoothPenalty, tol, tile_patches_x, stop_kernels,
                         precision, reuse_template, timing, schedule)
+    if warp_src is not None:
+        if (not _is_tensor(warp_src) or tuple(warp_src.shape) != (nf, nz, ny, nx) or not warp_src.is_contiguous()
+                or warp_src.dtype not in (torch.float32, torch.uint16)):
+            raise ValueError("warp_src must be contiguous float32 or uint16 (F, Z, Y, X)")
+        if warped_out is None:
+            warped_out = torch.empty((nf, nz, ny, nx), dtype=torch.float32, device=movs.device)
+        elif (tuple(warped_out.shape) != (nf, nz, ny, nx) or warped_out.dtype != torch.float32
+              or not warped_out.is_contiguous()):
+            raise ValueError("warped_out must be contiguous float32 (F, Z, Y, X)")
     bufs = []
     if logs:
         logs = (_lib.WrLog * nf)()
@@ -273,12 +309,18 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
     nvox = nz * ny * nx
     rc = lib.wr_get_motion_batch(ctx, _ptr(movs), nvox, _ptr(ref), dtype_code(movs), nx, ny, nz, nf,
                                  _ptr(mask_ref), _ptr(mask_mov), nvox, _ptr(motion_init), 3 * nvox,
-                                 ctypes.byref(prm), _ptr(motion_out), logs, strm)
+                                 ctypes.byref(prm), _ptr(motion_out), _ptr(warp_src),
+                                 dtype_code(warp_src) if warp_src is not None else 0, nvox, _ptr(warped_out),
+                                 logs, strm)
     _lib.check(rc, "wr_get_motion_batch")
     del keep
     if logs is None:
         return motion_out
-    return [_result(motion_out[f], None, logs[f], bufs[f], layer) for f in range(nf)]
+    out = [_result(motion_out[f], None, logs[f], bufs[f], layer) for f in range(nf)]
+    if warp_src is not None:
+        for f, res in enumerate(out):
+            res.extras["warped"] = warped_out[f]
+    return out
 
 
 def cp_grid(shape_zyx, r, level=0):

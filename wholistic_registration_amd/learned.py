@@ -434,7 +434,8 @@ def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothP
     res = _dev.get_motion(mov, ref, iter=iter, **init, **params)
     info = {"used": "learned", "learned_error": res.final_error}
     if fallback == "plain":
-        kw_plain = {k: v for k, v in params.items() if k not in ("motion_out", "layer", "reuse_template")}
+        kw_plain = {k: v for k, v in params.items()
+                    if k not in ("motion_out", "layer", "reuse_template", "warp_src", "warped_out")}
         if isinstance(kw_plain.get("tile_patches_x"), (list, tuple)):    # per-level widths of the learned run
             kw_plain.pop("tile_patches_x")
         plain = _dev.get_motion(mov, ref, layer=plain_layer, iter=plain_iter, **kw_plain)
@@ -445,6 +446,10 @@ def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothP
                 with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(mov.device)):
                     out.copy_(plain.motion)
                 plain.motion = out
+            if params.get("warp_src") is not None:   # the warp of the kept field
+                plain.extras["warped"] = _dev.correct_motion(params["warp_src"], plain.motion,
+                                                             out=res.extras["warped"], context=context,
+                                                             stream=stream)
             res, info["used"] = plain, "plain"
     elif fallback is not None:
         raise ValueError("fallback must be None or 'plain'")
