@@ -271,12 +271,18 @@ def precision_measure(w, n0, step, motion_fp32, field, mask, steps, prec):
     ek = e[keep].float()
     bytes_per_launch = (2 * 2 + (1 if mask is not None else 0)) * n0
     avg_ms = lk_ms / max(lk_n, 1)
+    roof = {"kernel": "k_lk_pass (level 0)", "achieved": bytes_per_launch / (avg_ms * 1e-3) / 1e9,
+            "unit": "GB/s", "frac": bytes_per_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch}
+    name = next(k for k, v in WORKLOADS.items() if v is w)
+    counted = (measured_traffic(name) or {}).get("bf16") if prec == "bf16" else None
+    if counted:
+        roof.update(traffic=counted["bytes_per_launch"], ta_busy=counted["ta_busy"],
+                    traffic_note="records of 8 B per voxel (the image and its central differences)")
     return {
         "value": n0 / dt, "ms_per_step": 1e3 * dt,
         "records_per_level": {f"L{L}": res.records[L] for L in range(w["layer"], -1, -1)},
-        "roofline": {"kernel": "k_lk_pass (level 0)", "achieved": bytes_per_launch / (avg_ms * 1e-3) / 1e9,
-                     "unit": "GB/s", "frac": bytes_per_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "avg_launch_ms": avg_ms, "bytes_per_launch": bytes_per_launch},
+        "roofline": roof,
         "epe_vs_fp32_field": {"epe_mean": float(ek.mean()), "epe_max": float(ek.max()),
                               "sample": "full volume outside the mask"},
         "epe_vs_truth": truth_epe(motion, field, mask),
@@ -867,8 +873,10 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,
                      "bytes_per_launch": bytes_per_launch,
-                     "bound_note": "the pass is VALU-issue bound (about 250 FP32 VALU instructions "
-                                   "per wave-row); HBM is the roofline it is priced against"},
+                     "bound_note": "the pass is bound by the vector-memory issue path (texture-address unit "
+                                   "busy 0.83 of the cycles, the bfloat16 record layout with half the "
+                                   "vector-memory instructions runs 1.6x faster at the same VALU count); HBM "
+                                   "is the roofline it is priced against"},
         "pipeline_gbs": alg / (total / args.steps) / 1e9,
         "algorithmic_bytes_per_step": alg,
     }
@@ -876,6 +884,11 @@ def main():
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
         out["roofline"]["traffic_source"] = traffic["source"]
+        # issue fractions of the same kernel from the committed counter runs
+        out["roofline"]["issue"] = {"ta_busy": traffic["ta_busy"],
+                                    "valu_per_wave": traffic["valu_instructions"] / traffic["waves"],
+                                    "vmem_read_per_wave": traffic["vmem_read_instructions"] / traffic["waves"],
+                                    "source": traffic["source"]}
     out["epe_vs_truth"] = truth_epe(motion, field, mask)
     precisions = [] if args.precisions == "none" else args.precisions.split(",")
     if precisions:
