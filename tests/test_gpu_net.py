@@ -64,6 +64,7 @@ def _within_bf16_step(got, want, mag=None):
     ((2, 4, 5, 12), 8, 16, True),       # 240 voxels: partial 64-voxel tiles
     ((1, 3, 4, 16), 48, 32, True),      # 27 * 48 = 1296: partial 32-wide k step
     ((2, 2, 3, 8), 96, 64, False),
+    ((1, 2, 2, 8), 128, 128, True),     # the bottleneck's channels on a tiny volume (one-tile waves)
 ])
 def test_conv_forward_matches_oracle(cuda, shape, cin, cout, leaky):
     import torch
