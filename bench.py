@@ -256,15 +256,17 @@ def precision_measure(w, n0, step, motion_fp32, field, mask, steps, prec):
     from wholistic_registration_amd import device as D
     res = step(False, prec)          # warm-up (and workspace) for this storage
     torch.cuda.synchronize()
-    lk_ms = lk_n = 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
-        res = step(True, prec)
+        res = step(False, prec)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    lk_ms = lk_n = 0.0
+    for _ in range(steps):           # with HIP events around the LK passes (not timed)
+        step(True, prec)
         st = D.stats()
         lk_ms += st.lk_ms[0]
         lk_n += st.lk_exec[0]
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
     motion = res.motion
     e = torch.sqrt(((motion - motion_fp32) ** 2).sum(-1))
     keep = mask <= 0 if mask is not None else torch.ones_like(e, dtype=torch.bool)
@@ -837,23 +839,32 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] warmup {i}: records {res.records[::-1]} (levels {w['layer']}..0)")
     torch.cuda.synchronize()
-    times, lk_ms, lk_n, recs = [], 0.0, 0, None
+    times, recs = [], None
     t_all = time.perf_counter()
     for i in range(args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = step(True)
+        res = step(False)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
-        st = D.stats()
-        lk_ms += st.lk_ms[0]
-        lk_n += st.lk_exec[0]
         recs = res.records
         upds = res.updates
         log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms, records {res.records[::-1]}")
     total = time.perf_counter() - t_all
     ms_per_step = 1e3 * total / args.steps
     value = n0 / (total / args.steps)
+    # the same registration with HIP events around every LK pass (outside the
+    # timed region): per-level LK time and the level-0 pass duration
+    layers = range(w["layer"], -1, -1)
+    lk_lvl = {L: [0.0, 0] for L in layers}
+    for i in range(args.steps):
+        step(True)
+        torch.cuda.synchronize()
+        st = D.stats()
+        for L in layers:
+            lk_lvl[L][0] += st.lk_ms[L]
+            lk_lvl[L][1] += st.lk_exec[L]
+    lk_ms, lk_n = lk_lvl[0]
 
     # roofline of the level-0 LK pass
     bytes_per_launch = (2 * 4 + (1 if mask is not None else 0)) * n0
@@ -868,7 +879,9 @@ def main():
         "config": {"workload": args.workload, "zyx": list(w["zyx"]), "input_dtype": str(tmpl.dtype),
                    "layer": w["layer"], "r": w["r"], "iter": w["iter"], "mask": mask is not None,
                    "records_per_level": {f"L{L}": recs[L] for L in range(w["layer"], -1, -1)},
-                   "tile_cols": D.stats().tile_cols},
+                   "tile_cols": D.stats().tile_cols,
+                   "lk_ms_per_level": {f"L{L}": {"ms": lk_lvl[L][0] / args.steps,
+                                                "passes": lk_lvl[L][1] / args.steps} for L in layers}},
         "roofline": {"bound": "hbm", "kernel": "k_lk_pass (level 0)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ms,

@@ -626,7 +626,13 @@ __global__ __launch_bounds__(kMaxThreads) void k_lk_pass(PassArgs a) {
             } else {
                 g = eval_rec(cc, tr);
             }
-            const float gz = a.z_ratio == 1.0f ? g.gz : __fdiv_rn(g.gz, a.z_ratio);
+            // a uniform branch, not a select (the empty volatile asm keeps the
+            // compiler from speculating the division): skipped at zRatio 1
+            float gz = g.gz;
+            if (a.z_ratio != 1.0f) {
+                __asm__ volatile("");
+                gz = __fdiv_rn(gz, a.z_ratio);
+            }
             const float d = rc - g.w;
             if (s >= 2) {
                 const float vsum = d0 + d1 + d;
