@@ -44,7 +44,8 @@ extern "C" {
 
 #define WR_F32 0         /* input volume dtypes */
 #define WR_U16 1
-#define WR_F64 2         /* accepted by wr_imresize only */
+#define WR_F64 2         /* accepted by wr_imresize and wr_spline_coef */
+#define WR_U8 3          /* byte masks: wr_spline_coef only */
 
 #define WR_MAX_LEVELS 16
 
@@ -447,6 +448,117 @@ int wr_net_output(wr_ctx* ctx, const void* p, int cp, long long n, double scale,
  * wr_params.init_cp takes. */
 int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int hs, int ws, int q, int ncx,
                      int ncy, int r, int level, double scale, float* cp_out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Cross-resolution registration (getMotion_v2,
+ * src/wholistic_registration/utils/calFlowCrossResolution.py:2470-2663).
+ *
+ * A moving volume on a coarse-z grid is registered to a reference volume on
+ * its own (finer) grid: moving voxel p maps to reference coordinates
+ * phase(p) = phase0(p) + motion(p).  Volumes are dense (z, y, x) device
+ * arrays (the reference's (x, y, z) arrays transposed); fields (phase,
+ * motion) are 3 planes (x, y, z components) of nz * ny * nx float32.  Level
+ * images and residuals are float64, as in the reference.  The Python drop-in
+ * (wholistic_registration_amd/calFlowCrossResolution.py) composes these
+ * calls exactly as the reference composes its functions.
+ * ------------------------------------------------------------------------ */
+
+/* Cubic B-spline coefficients of a volume for map_coordinates(order=3,
+ * mode="nearest") (the interpolator of generate_continuous_H_gpu, :463-477):
+ * the volume padded by 12 replicated voxels on every side, then the cubic
+ * B-spline prefilter along x, y and z in float64 (gain 6, pole sqrt(3) - 2,
+ * half-sample symmetric initial conditions).  coef: (nz + 24) * (ny + 24) * (nx + 24)
+ * doubles.  dtype: WR_F64, WR_F32 or WR_U8. */
+int wr_spline_coef(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int nz, double* coef, void* stream);
+
+/* Values of that interpolant at n points (n, 3) float32 (x, y, z), the z
+ * coordinate divided by z_ratio first (generate_continuous_H_gpu's H and
+ * apply_H_to_matrix_gpu, :463-477, :1144-1153); out: n doubles. */
+int wr_spline_sample(wr_ctx* ctx, const double* coef, int nx, int ny, int nz, const float* pts, long long n,
+                     double z_ratio, double* out, void* stream);
+
+/* One pyramid layer's optimisation (optimize_layer_cross_resolution,
+ * :2031-2237). */
+typedef struct wr_xres_layer {
+    int nx;                      /* moving level (x, y, z extents)                */
+    int ny;
+    int nz;
+    int rx;                      /* reference level                               */
+    int ry;
+    int rz;
+    int r;                       /* patch half-width                              */
+    int iters;                   /* option["iter"]                                */
+    double smooth_penalty_sum;   /* smoothPenalty * (2r + 1)^2                     */
+    double mov_range;            /* option["movRange"]                             */
+    double z_ratio_hr;           /* zRatio_HR / 2^layer                            */
+    double tol;                  /* option["tol"]                                  */
+} wr_xres_layer;
+
+/* Iteration record of one optimisation.  reason: 0 last iteration, 1 error
+ * increased repeatedly, 2 error change below tol, 3 motion update below tol,
+ * 4 forced schedule.  The three arrays (host, `iters` entries each, may be
+ * NULL) receive each iteration's diffError, penaltyError, currentError. */
+typedef struct wr_xres_trace {
+    int records;
+    int updates;
+    int reason;
+    double* diff_error;
+    double* penalty_error;
+    double* current_error;
+    double final_diff;           /* the final recompute (:2198-2222)               */
+    double final_penalty;
+    double final_error;
+} wr_xres_trace;
+
+/* mov_l (nz, ny, nx) and ref_l (rz, ry, rx) float64 level images; ref_coef /
+ * mask_coef their spline coefficients (mask_coef: of the reference
+ * exclusion mask, NULL = nothing excluded); mask_mov (nz, ny, nx) bytes,
+ * nonzero = excluded (NULL = none); phase0 / motion_init 3-plane fields.
+ * forced: NULL, or {records, updates} replacing the data-dependent stop tests.
+ * Outputs: motion, phase (3 planes), sampled (reference at the phase),
+ * residual (the masked 3x3 mean residual), valid (1 = voxel counts), and
+ * first_residual (NULL or the residual of the first iteration, for the
+ * 'highresidual' exclusion).  The host waits for each iteration's errors. */
+int wr_xres_optimize(wr_ctx* ctx, const wr_xres_layer* layer, const double* mov_l, const double* ref_l,
+                     const double* ref_coef, const double* mask_coef, const uint8_t* mask_mov,
+                     const float* phase0, const float* motion_init, const int* forced, float* motion,
+                     float* phase, double* sampled, double* residual, uint8_t* valid, double* first_residual,
+                     wr_xres_trace* trace, void* stream);
+
+/* compute_valid_mask_on_moving_grid (:2006-2028) at a given phase: valid
+ * (nullable) and its count (host). */
+int wr_xres_valid(wr_ctx* ctx, const float* phase, int nx, int ny, int nz, const double* mask_coef, int rx,
+                  int ry, int rz, const uint8_t* mask_mov, uint8_t* valid, long long* count, void* stream);
+
+/* Wrong-region detection of one layer (correct_wrong_regions_one_layer
+ * :2306-2351 with get_local_error_on_control_points :1190-1222, metric 0
+ * 'mse' or 1 'mae',
+ * build_bad_region_mask_from_cp_error :1570-1787, the 'highresidual'
+ * selection :2329-2347 when mode = 1 (first_residual and q used), and
+ * & valid): bad (nz, ny, nx) bytes; *n_regions = kept components. */
+int wr_xres_bad_regions(wr_ctx* ctx, const double* residual, const uint8_t* valid, const double* first_residual,
+                        int nx, int ny, int nz, int r, int metric, double mad_threshold, int min_size, int mode,
+                        double q, uint8_t* bad, int* n_regions, void* stream);
+
+/* build_reference_trap_mask_from_bad_moving_fast_roi (:1413-1567) united with
+ * the reference level's exclusion mask: corrected (rz, ry, rx) bytes =
+ * mask_ref_l (NULL = none) | trap; *trap_count = voxels of the trap. */
+int wr_xres_trap_mask(wr_ctx* ctx, const uint8_t* bad, const float* phase, int nx, int ny, int nz,
+                      const double* ref_l, int rx, int ry, int rz, const uint8_t* mask_ref_l, double z_ratio_ref,
+                      double expand_radius_xy, double sigma, double intensity_k, uint8_t* corrected,
+                      long long* trap_count, void* stream);
+
+/* Number of nonzero bytes of m (int(cp.sum(valid_mask)), :2375-2378). */
+int wr_count_u8(wr_ctx* ctx, const uint8_t* m, long long n, long long* count, void* stream);
+
+/* Element-wise helpers of the layer set-up (:2540-2558, :1917-1992):
+ * out[i] = float32(in[i] * mul / div) (the level's start fields);
+ * mask[i] = in[i] > thr (resample_exclude_mask); the identity phase
+ * (x, y, z * z_mul) of a level. */
+int wr_xres_scale(wr_ctx* ctx, const double* in, long long n, double mul, double div, float* out, void* stream);
+int wr_xres_threshold(wr_ctx* ctx, const double* in, long long n, double thr, uint8_t* out, void* stream);
+int wr_xres_phase_identity(wr_ctx* ctx, int nx, int ny, int nz, double z_ratio, double z_ratio_hr, float* phase,
+                           void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,90 @@
+"""Inputs of the cross-resolution (``getMotion_v2``) fixtures, regenerated
+bit-identically from seeds.
+
+A cross-resolution pair is a moving volume on a coarse-z grid and a reference
+volume on a finer-z grid (calFlowCrossResolution.py:2470-2534): moving voxel
+``(x, y, k)`` sits at reference coordinates ``(x, y, k * zRatio / zRatio_HR)``
+before motion.  The moving volume here is the noise-free reference sampled
+there plus a smooth displacement (in reference index units), with Poisson
+noise; optional planted artefacts make the wrong-region detector fire.
+
+Arrays are in the solver's ``(x, y, z)`` layout (the ``transpose(2, 1, 0)``
+view of ``(Z, Y, X)`` C-order volumes).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy import ndimage as ndi
+
+from wholistic_registration_amd import synth
+
+# name -> dict of the case
+XRES_CASES = {
+    # no wrong-region pass, no masks, two levels
+    "plain": dict(seed=301, mov_xyz=(64, 48, 8), ref_xyz=(64, 48, 24), z_ratio=3.0, z_ratio_hr=1.0,
+                  n_blobs=90, dmax=1.5, layer=1, r=3, iter=8, wrong=False),
+    # wrong-region correction on, a bright artefact in the moving volume that the
+    # reference does not hold (detected, trap mask built, corrected run kept or not)
+    "wrong": dict(seed=302, mov_xyz=(72, 56, 6), ref_xyz=(72, 56, 18), z_ratio=3.0, z_ratio_hr=1.0,
+                  n_blobs=110, dmax=1.5, layer=1, r=3, iter=8, wrong=True, artefacts=3),
+    # masks on both sides, a warm start, three levels, reference wider in y
+    "masked": dict(seed=303, mov_xyz=(60, 52, 6), ref_xyz=(60, 60, 12), z_ratio=2.0, z_ratio_hr=1.0,
+                   n_blobs=100, dmax=1.2, layer=2, r=2, iter=7, wrong=True, artefacts=2,
+                   mask_ref=0.12, mask_mov=True, init=0.6),
+    # the 'highresidual' exclusion mode, non-unit zRatio_HR
+    "highres": dict(seed=304, mov_xyz=(64, 64, 5), ref_xyz=(64, 64, 10), z_ratio=4.0, z_ratio_hr=2.0,
+                    n_blobs=120, dmax=1.5, layer=1, r=3, iter=6, wrong=True, artefacts=3,
+                    exclude_mode="highresidual"),
+}
+
+
+def _option(c):
+    opt = {
+        "layer": c["layer"], "iter": c["iter"], "r": c["r"],
+        "zRatio": c["z_ratio"], "zRatio_HR": c["z_ratio_hr"],
+        "smoothPenalty": 0.05, "movRange": 3.0, "tol": 1e-3,
+        "wrong_region_enable": bool(c["wrong"]),
+    }
+    if "exclude_mode" in c:
+        opt["wrong_region_exclude_mode"] = c["exclude_mode"]
+    return opt
+
+
+def xres_inputs(name):
+    """``(mov_xyz, ref_xyz, option)`` of a case; ``option`` holds numpy masks."""
+    c = XRES_CASES[name]
+    rng = np.random.default_rng(c["seed"])
+    rx, ry, rz = c["ref_xyz"]
+    mx, my, mz = c["mov_xyz"]
+    ref_zyx = (rz, ry, rx)
+    blobs = synth.draw_blobs(rng, ref_zyx, c["n_blobs"])
+    clean = synth.render_template_np(ref_zyx, blobs).astype(np.float64)
+    modes = synth.draw_modes(rng, 3)
+    d = synth.field_np((mz, my, mx), modes, c["dmax"]).astype(np.float64)   # (Z, Y, X, 3)
+    k, yy, xx = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    zs = k * (c["z_ratio"] / c["z_ratio_hr"])
+    coords = np.stack([zs + d[..., 2], yy + d[..., 1], xx + d[..., 0]])
+    mov_clean = ndi.map_coordinates(clean, coords, order=1, mode="nearest")
+    for _ in range(c.get("artefacts", 0)):
+        cz, cy, cx = rng.integers(0, mz), rng.integers(6, my - 6), rng.integers(6, mx - 6)
+        zz, yv, xv = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+        rr = ((yv - cy) ** 2 + (xv - cx) ** 2) / 9.0 + (zz - cz) ** 2
+        mov_clean = mov_clean + 900.0 * np.exp(-rr)
+    ref = rng.poisson(clean).astype(np.float32)
+    mov = rng.poisson(np.maximum(mov_clean, 0)).astype(np.float32)
+    mov_xyz = np.ascontiguousarray(mov.transpose(2, 1, 0))
+    ref_xyz = np.ascontiguousarray(ref.transpose(2, 1, 0))
+    opt = _option(c)
+    mref = np.zeros(c["ref_xyz"], dtype=bool)
+    if c.get("mask_ref"):
+        mref = np.ascontiguousarray(synth.ellipsoid_mask_np(rng, ref_zyx, c["mask_ref"]).transpose(2, 1, 0))
+    mmov = np.zeros(c["mov_xyz"], dtype=bool)
+    if c.get("mask_mov"):
+        mmov[mx // 4: mx // 4 + 8, 5:13, :] = True
+    opt["mask_ref"] = mref
+    opt["mask_mov"] = mmov
+    if c.get("init"):
+        f = synth.field_np((mz, my, mx), synth.draw_modes(rng, 2), c["init"])
+        opt["motion"] = np.ascontiguousarray(f.transpose(2, 1, 0, 3))
+    return mov_xyz, ref_xyz, opt

@@ -38,7 +38,8 @@ def _c_struct_fields(struct_name):
 
 
 @pytest.mark.parametrize("struct,py", [("wr_params", _lib.WrParams), ("wr_log", _lib.WrLog),
-                                       ("wr_stats", _lib.WrStats)])
+                                       ("wr_stats", _lib.WrStats), ("wr_xres_layer", _lib.WrXresLayer),
+                                       ("wr_xres_trace", _lib.WrXresTrace)])
 def test_ctypes_structs_match_header(struct, py):
     assert _c_struct_fields(struct) == [f[0] for f in py._fields_]
 

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG, "libwrhip.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "wr_hip.h")
 
 WR_OK, WR_E_ARG, WR_E_KEY, WR_E_HIP, WR_E_MEM = 0, -1, -2, -3, -4
-WR_F32, WR_U16, WR_F64 = 0, 1, 2
+WR_F32, WR_U16, WR_F64, WR_U8 = 0, 1, 2, 3
 WR_MAX_LEVELS = 16
 
 
@@ -71,8 +71,35 @@ class WrStats(ctypes.Structure):
     ]
 
 
+class WrXresLayer(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("rx", ctypes.c_int), ("ry", ctypes.c_int), ("rz", ctypes.c_int),
+        ("r", ctypes.c_int),
+        ("iters", ctypes.c_int),
+        ("smooth_penalty_sum", ctypes.c_double),
+        ("mov_range", ctypes.c_double),
+        ("z_ratio_hr", ctypes.c_double),
+        ("tol", ctypes.c_double),
+    ]
+
+
+class WrXresTrace(ctypes.Structure):
+    _fields_ = [
+        ("records", ctypes.c_int), ("updates", ctypes.c_int), ("reason", ctypes.c_int),
+        ("diff_error", ctypes.POINTER(ctypes.c_double)),
+        ("penalty_error", ctypes.POINTER(ctypes.c_double)),
+        ("current_error", ctypes.POINTER(ctypes.c_double)),
+        ("final_diff", ctypes.c_double),
+        ("final_penalty", ctypes.c_double),
+        ("final_error", ctypes.c_double),
+    ]
+
+
 _VP = ctypes.c_void_p
 _I = ctypes.c_int
+_D = ctypes.c_double
+_LL = ctypes.c_longlong
 _SIGNATURES = {
     "wr_create": ([_I], _VP),
     "wr_destroy": ([_VP], None),
@@ -123,6 +150,20 @@ _SIGNATURES = {
     "wr_net_input_padded": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_net_output": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, _VP, _VP], _I),
     "wr_net_output_cp": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double, _VP, _VP], _I),
+    "wr_spline_coef": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
+    "wr_spline_sample": ([_VP, _VP, _I, _I, _I, _VP, _LL, _D, _VP, _VP], _I),
+    "wr_xres_optimize": ([_VP, ctypes.POINTER(WrXresLayer), _VP, _VP, _VP, _VP, _VP, _VP, _VP,
+                          ctypes.POINTER(ctypes.c_int), _VP, _VP, _VP, _VP, _VP, _VP,
+                          ctypes.POINTER(WrXresTrace), _VP], _I),
+    "wr_xres_valid": ([_VP, _VP, _I, _I, _I, _VP, _I, _I, _I, _VP, _VP, ctypes.POINTER(_LL), _VP], _I),
+    "wr_xres_bad_regions": ([_VP, _VP, _VP, _VP, _I, _I, _I, _I, _I, _D, _I, _I, _D, _VP,
+                             ctypes.POINTER(ctypes.c_int), _VP], _I),
+    "wr_xres_trap_mask": ([_VP, _VP, _VP, _I, _I, _I, _VP, _I, _I, _I, _VP, _D, _D, _D, _D, _VP,
+                           ctypes.POINTER(_LL), _VP], _I),
+    "wr_count_u8": ([_VP, _VP, _LL, ctypes.POINTER(_LL), _VP], _I),
+    "wr_xres_scale": ([_VP, _VP, _LL, _D, _D, _VP, _VP], _I),
+    "wr_xres_threshold": ([_VP, _VP, _LL, _D, _VP, _VP], _I),
+    "wr_xres_phase_identity": ([_VP, _I, _I, _I, _D, _D, _VP, _VP], _I),
 }
 
 _lib = None

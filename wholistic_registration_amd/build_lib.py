@@ -17,7 +17,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libwrhip.so")
-SOURCES = ["api.hip", "resample.hip", "lk.hip", "lk_pass.hip", "warp.hip", "penalty.hip", "mask.hip", "harness.hip", "splat.hip", "net.hip"]
+SOURCES = ["api.hip", "resample.hip", "lk.hip", "lk_pass.hip", "warp.hip", "penalty.hip", "mask.hip", "harness.hip", "splat.hip", "net.hip", "xres.hip"]
 ARCH = os.environ.get("WR_OFFLOAD_ARCH", "gfx950")
 
 

@@ -208,23 +208,6 @@ void release(DevBuf& b) {
 
 using namespace wr;
 
-#define WR_TRY(expr)             \
-    do {                         \
-        int rc_ = (expr);        \
-        if (rc_ != WR_OK) return rc_; \
-    } while (0)
-
-#define WR_ARG(cond, msg)         \
-    do {                          \
-        if (!(cond)) {            \
-            set_error(msg);       \
-            return WR_E_ARG;      \
-        }                         \
-    } while (0)
-
-struct wr_ctx {
-    Context c;
-};
 
 extern "C" {
 
@@ -250,7 +233,8 @@ void wr_destroy(wr_ctx* ctx) {
     Context& c = ctx->c;
     (void)hipSetDevice(c.device);
     for (DevBuf* b : {&c.fws, &c.ref_l, &c.tmp, &c.tmp2, &c.mref_l, &c.tab_w, &c.tab_i, &c.pen,
-                      &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig, &c.splat, &c.netw})
+                      &c.msel, &c.mlab, &c.mcnt, &c.mtmp, &c.hsmall, &c.hbig, &c.splat, &c.netw, &c.xws,
+                      &c.xsel, &c.xroi})
         release(*b);
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c.lvl_events) (void)hipEventDestroy(e);

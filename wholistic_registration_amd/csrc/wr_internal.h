@@ -514,6 +514,7 @@ struct Context {
     DevBuf hsmall, hbig;                 // harness: statistics / Gram scratch, sampled frame matrix
     DevBuf splat;                        // splats: shift scratch and the weighted sums
     DevBuf netw;                         // learned initialisation: flipped weights, input statistics
+    DevBuf xws, xsel, xroi;              // cross-resolution solver: per-iteration scratch, selection, trap ROI
     std::vector<hipEvent_t> events;
     hipStream_t side = nullptr;          // pyramid builds, ahead of the iterations
     std::vector<hipEvent_t> lvl_events;  // [L]: level L's images are ready; [layer+1]: fork
@@ -528,3 +529,22 @@ struct Context {
 };
 
 }  // namespace wr
+
+// The opaque handle of the C ABI.
+struct wr_ctx {
+    wr::Context c;
+};
+
+#define WR_TRY(expr)                  \
+    do {                              \
+        int rc_ = (expr);             \
+        if (rc_ != WR_OK) return rc_; \
+    } while (0)
+
+#define WR_ARG(cond, msg)          \
+    do {                           \
+        if (!(cond)) {             \
+            ::wr::set_error(msg);  \
+            return WR_E_ARG;       \
+        }                          \
+    } while (0)
