@@ -389,6 +389,72 @@ def splat_measure(reps=3):
             "weighted_gbs": (16 * n + 2 * out_bytes) / (t_w * 1e-3) / 1e9}
 
 
+XRES = dict(seed=41, mov_xyz=(512, 512, 12), ref_xyz=(512, 512, 48), z_ratio=4.0, z_ratio_hr=1.0, n_blobs=6000,
+            dmax=2.0, artefacts=24, layer=2, r=8, iter=10)
+
+
+def xres_option(c):
+    return {"layer": c["layer"], "iter": c["iter"], "r": c["r"], "zRatio": c["z_ratio"],
+            "zRatio_HR": c["z_ratio_hr"], "smoothPenalty": 0.05, "movRange": 3.0, "tol": 1e-3,
+            "mask_ref": np.zeros(c["ref_xyz"], np.float32), "mask_mov": np.zeros(c["mov_xyz"], np.float32)}
+
+
+def xres_measure(reps=3, cpu=True, cpu_xy=128):
+    """Cross-resolution registration (getMotion_v2, wrong-region correction
+    on): moving 12 x 512 x 512 (z, y, x) at zRatio 4 against a 48 x 512 x 512
+    reference at zRatio_HR 1, 6000 blobs, a 3-mode field of max |d| = 2, 24
+    planted artefacts, layer 2, r 8, 10 iterations.  Inputs resident on the
+    device; wall time of the whole call (its host waits per iteration
+    included).  CPU: the oracle (bit-identical to the reference's numpy path)
+    on the same generator at 12 x 128 x 128 / 48 x 128 x 128, one thread."""
+    import time
+
+    import torch
+
+    from wholistic_registration_amd import calFlowCrossResolution as X
+    from wholistic_registration_amd import synth
+    c = XRES
+    mov, ref, _ = synth.make_xres_pair_np(c["seed"], c["mov_xyz"], c["ref_xyz"], c["z_ratio"], c["z_ratio_hr"],
+                                          c["n_blobs"], c["dmax"], c["artefacts"])
+    dev = torch.device("cuda", 0)
+    mov_t = torch.from_numpy(np.ascontiguousarray(mov.transpose(2, 1, 0))).to(dev).permute(2, 1, 0)
+    ref_t = torch.from_numpy(np.ascontiguousarray(ref.transpose(2, 1, 0))).to(dev).permute(2, 1, 0)
+    times, trace, info = [], [], {}
+    for k in range(reps + 1):
+        tr, inf = [], {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X.getMotion_v2(mov_t, ref_t, xres_option(c), trace=tr, info=inf, output="device")
+        torch.cuda.synchronize()
+        if k:
+            times.append(time.perf_counter() - t0)
+        trace, info = tr, inf
+    n = int(np.prod(c["mov_xyz"]))
+    ms = 1e3 * float(np.median(times))
+    out = {"moving_xyz": list(c["mov_xyz"]), "reference_xyz": list(c["ref_xyz"]), "layer": c["layer"],
+           "r": c["r"], "iter": c["iter"], "ms": ms, "moving_voxels_per_s": n / (ms * 1e-3),
+           "optimisations": [(t["records"], t["updates"]) for t in trace],
+           "wrong_regions": {f"L{L}": (v or {}).get("regions") for L, v in info.items()},
+           "accepted": {f"L{L}": (v or {}).get("accepted") for L, v in info.items()}}
+    if cpu:
+        from oracle import xres_oracle as xo
+        small_m, small_r, _ = synth.make_xres_pair_np(c["seed"], (cpu_xy, cpu_xy, c["mov_xyz"][2]),
+                                                      (cpu_xy, cpu_xy, c["ref_xyz"][2]), c["z_ratio"],
+                                                      c["z_ratio_hr"], c["n_blobs"] * cpu_xy * cpu_xy // (512 * 512),
+                                                      c["dmax"], c["artefacts"] * cpu_xy * cpu_xy // (512 * 512))
+        o = xres_option(c)
+        o["mask_ref"] = np.zeros(small_r.shape, np.float32)
+        o["mask_mov"] = np.zeros(small_m.shape, np.float32)
+        t0 = time.perf_counter()
+        xo.get_motion_v2(small_m, small_r, o)
+        sec = time.perf_counter() - t0
+        nn = int(np.prod(small_m.shape))
+        out["cpu_reference_path"] = {"moving_xyz": list(small_m.shape), "seconds": sec,
+                                     "moving_voxels_per_s": nn / sec, "cores": 1,
+                                     "kind": "port (bit-identical to the reference's numpy path)"}
+    return out
+
+
 def mask_measure(vol, crop, reps=3):
     """The harness's automatic mask of one moving volume (getMask +
     bwareafilt3_wei with the default config's thresFactor 5 and maskRange
@@ -773,6 +839,9 @@ def main():
     ap.add_argument("--net-layers", default=None, metavar="D,H,W",
                     help="only time each convolution of the learned start's network for a (D, H, W) input "
                          "(64,64,128: the level-3 image of a 64x512x1024 frame) and print one line per layer")
+    ap.add_argument("--no-xres", action="store_true", help="skip the cross-resolution (getMotion_v2) measurement")
+    ap.add_argument("--xres-only", action="store_true",
+                    help="only run the cross-resolution measurement (prints its JSON)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -806,6 +875,9 @@ def main():
     from wholistic_registration_amd import _lib
     from wholistic_registration_amd import device as D
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.xres_only:
+        print(json.dumps(xres_measure(cpu=not args.no_cpu_baseline)), flush=True)
+        return
     if args.learned_only:
         print(json.dumps(learned_measure(max(args.learned_templates, 1), weights=args.learned_weights)),
               flush=True)
@@ -923,6 +995,9 @@ def main():
     if args.learned_templates > 0:
         log(f"[bench] plain vs learned start on {16 * args.learned_templates} held-out frames")
         out["learned_init"] = learned_measure(args.learned_templates, weights=args.learned_weights)
+    if not args.no_xres:
+        log("[bench] cross-resolution registration (getMotion_v2)")
+        out["cross_resolution"] = xres_measure(cpu=not args.no_cpu_baseline)
     if not args.no_mask_measure:
         log("[bench] fixed-plane projections")
         out["plane_projections"] = splat_measure()

@@ -15,8 +15,6 @@ view of ``(Z, Y, X)`` C-order volumes).
 from __future__ import annotations
 
 import numpy as np
-from scipy import ndimage as ndi
-
 from wholistic_registration_amd import synth
 
 # name -> dict of the case
@@ -54,27 +52,12 @@ def _option(c):
 def xres_inputs(name):
     """``(mov_xyz, ref_xyz, option)`` of a case; ``option`` holds numpy masks."""
     c = XRES_CASES[name]
-    rng = np.random.default_rng(c["seed"])
     rx, ry, rz = c["ref_xyz"]
     mx, my, mz = c["mov_xyz"]
     ref_zyx = (rz, ry, rx)
-    blobs = synth.draw_blobs(rng, ref_zyx, c["n_blobs"])
-    clean = synth.render_template_np(ref_zyx, blobs).astype(np.float64)
-    modes = synth.draw_modes(rng, 3)
-    d = synth.field_np((mz, my, mx), modes, c["dmax"]).astype(np.float64)   # (Z, Y, X, 3)
-    k, yy, xx = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
-    zs = k * (c["z_ratio"] / c["z_ratio_hr"])
-    coords = np.stack([zs + d[..., 2], yy + d[..., 1], xx + d[..., 0]])
-    mov_clean = ndi.map_coordinates(clean, coords, order=1, mode="nearest")
-    for _ in range(c.get("artefacts", 0)):
-        cz, cy, cx = rng.integers(0, mz), rng.integers(6, my - 6), rng.integers(6, mx - 6)
-        zz, yv, xv = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
-        rr = ((yv - cy) ** 2 + (xv - cx) ** 2) / 9.0 + (zz - cz) ** 2
-        mov_clean = mov_clean + 900.0 * np.exp(-rr)
-    ref = rng.poisson(clean).astype(np.float32)
-    mov = rng.poisson(np.maximum(mov_clean, 0)).astype(np.float32)
-    mov_xyz = np.ascontiguousarray(mov.transpose(2, 1, 0))
-    ref_xyz = np.ascontiguousarray(ref.transpose(2, 1, 0))
+    mov_xyz, ref_xyz, rng = synth.make_xres_pair_np(c["seed"], c["mov_xyz"], c["ref_xyz"], c["z_ratio"],
+                                                    c["z_ratio_hr"], c["n_blobs"], c["dmax"],
+                                                    c.get("artefacts", 0))
     opt = _option(c)
     mref = np.zeros(c["ref_xyz"], dtype=bool)
     if c.get("mask_ref"):

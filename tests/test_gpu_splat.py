@@ -59,10 +59,10 @@ def test_weighted_projection(cuda, mode, ds):
     assert np.allclose(out, w_out.transpose(0, 2, 1), rtol=1e-5, atol=1e-4)
 
 
-def test_values_are_required(cuda):
+def test_reference_volume_is_required(cuda):
+    """Without values the reference volume is sampled (tests/test_gpu_xres.py);
+    the volume itself stays required, as in the reference."""
     from wholistic_registration_amd import calFlowCrossResolution as X
     coords, _ = _samples(1)
-    with pytest.raises(NotImplementedError):
-        X.project_coords_to_fixed_planes_gpu(coords, np.zeros((24, 18, 11)), [1.0])
     with pytest.raises(ValueError):
         X.project_coords_to_fixed_planes_gpu(coords, None, [1.0], values_xyk=np.zeros(coords.shape[:-1]))

@@ -354,14 +354,16 @@ def _count(mask):
 
 
 class _Layer:
-    """Device images of one pyramid layer."""
+    """Device images of one pyramid layer.  An empty exclusion mask has an
+    all-zero interpolant, so its coefficients are left out (None: nothing
+    excluded) and the kernels skip its reads."""
 
     def __init__(self, mov_l, ref_l, mmov_l, mref_l):
         self.mov_l, self.ref_l, self.mmov_l, self.mref_l = mov_l, ref_l, mmov_l, mref_l
         self.nz, self.ny, self.nx = mov_l.shape
         self.rz, self.ry, self.rx = ref_l.shape
         self.ref_coef = _spline_coef(ref_l)
-        self.mask_coef = _spline_coef(mref_l)
+        self.mask_coef = _spline_coef(mref_l) if _count(mref_l) else None
 
 
 def compute_valid_mask_on_moving_grid(phase_new, H_mask_ref_layer, mask_mov_layer, lay=None):
@@ -522,7 +524,8 @@ def correct_wrong_regions_one_layer(lay, phase_current, motion_init_layer, r, it
     return res0
 
 
-def getMotion_v2(data_mov, data_ref, option, verbose=False, *, schedules=None, trace=None, info=None):
+def getMotion_v2(data_mov, data_ref, option, verbose=False, *, schedules=None, trace=None, info=None,
+                 output="numpy"):
     """Cross-resolution registration with wrong-region correction
     (calFlowCrossResolution.py:2470-2663).
 
@@ -531,11 +534,16 @@ def getMotion_v2(data_mov, data_ref, option, verbose=False, *, schedules=None, t
     ``(phase_new, motion_current, data_ref_sampled)`` as numpy arrays
     ((x, y, z, 3) float32, (x, y, z, 3) float32, (x, y, z) float64).
 
-    Beyond the reference (testing): ``schedules`` (list of (records, updates)
-    per optimisation, in call order) forces the iteration schedules;
-    ``trace`` (list) and ``info`` (dict) receive every optimisation's record
-    and each layer's wrong-region outcome.
+    Beyond the reference: ``output="device"`` returns the device tensors
+    instead (phase and motion as (3, z, y, x) float32 planes, the sampled
+    reference (z, y, x) float64), for callers that keep working on the GPU.
+    Testing: ``schedules`` (list of (records, updates) per optimisation, in
+    call order) forces the iteration schedules; ``trace`` (list) and
+    ``info`` (dict) receive every optimisation's record and each layer's
+    wrong-region outcome.
     """
+    if output not in ("numpy", "device"):
+        raise ValueError("output must be 'numpy' or 'device'")
     torch = _torch()
     dev = _device()
     option["mask_ref"] = np.asarray(option["mask_ref"], dtype=np.float32) if not _is_tensor(option["mask_ref"]) \
@@ -595,6 +603,8 @@ def getMotion_v2(data_mov, data_ref, option, verbose=False, *, schedules=None, t
                                                   schedule=next(sched) if sched else None, trace=trace)
         if info is not None:
             info[layer] = lvl
+    if output == "device":
+        return res["phase_new"], res["motion"], res["data_ref_sampled"]
     phase_np = _planes_to_field(res["phase_new"])
     motion_np = _planes_to_field(res["motion"])
     sampled_np = res["data_ref_sampled"].permute(2, 1, 0).cpu().numpy()

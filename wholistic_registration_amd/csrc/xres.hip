@@ -50,6 +50,7 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kSplinePad = 12;
+constexpr long long kMaxElems = 4294967295ll;   // the kernels' 32-bit index arithmetic
 
 unsigned blocks_for(long long n, int t = kT) {
     const long long b = (n + t - 1) / t;
@@ -61,6 +62,25 @@ __device__ __forceinline__ long long gtid() {
 }
 __device__ __forceinline__ long long gstride() { return static_cast<long long>(gridDim.x) * blockDim.x; }
 
+// grids of kernels that finish with an atomic per workgroup on one address:
+// few workgroups, grid-stride loops (the atomics to one address serialise)
+unsigned blocks_atomic(long long n) { return std::min(blocks_for(n), 1024u); }
+
+// the index of this lane's entry when the lanes with `pred` append to a list
+// (one atomic per wave), -1 for the others
+__device__ __forceinline__ long long wave_append(unsigned long long* count, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    if (!m) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(count, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, leader, 64);
+    if (!pred) return -1;
+    const unsigned long long below = m & ((1ull << lane) - 1ull);
+    return static_cast<long long>(base + __popcll(below));
+}
+
 __device__ __forceinline__ int clampi_(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // floor of a coordinate as an int, bounded so that non-finite or huge values
@@ -71,6 +91,18 @@ __device__ __forceinline__ int floor_i(double x, double* frac) {
     if (f > 1.0e8) f = 1.0e8;
     *frac = x - f;
     return static_cast<int>(f);
+}
+
+// (x, y, z) of flat index i of a (nz, ny, nx) array, in 32-bit arithmetic
+// (every volume here holds fewer than 2^32 elements, checked at the entry
+// points; 64-bit division is many times slower)
+__device__ __forceinline__ void unflat(long long i, int nx, int ny, int& x, int& y, int& z) {
+    const unsigned u = static_cast<unsigned>(i);
+    const unsigned t = u / static_cast<unsigned>(nx);
+    x = static_cast<int>(u - t * static_cast<unsigned>(nx));
+    const unsigned zz = t / static_cast<unsigned>(ny);
+    y = static_cast<int>(t - zz * static_cast<unsigned>(ny));
+    z = static_cast<int>(zz);
 }
 
 // ------------------------------------------------------------ block reductions
@@ -155,10 +187,13 @@ __global__ void k_spline_line(double* __restrict__ c, long long nlines, int len,
     double* p = c + (l / inner) * outer_stride + (l % inner) * inner_stride;
     for (int i = 0; i < len; ++i) p[i * stride] *= gain;
     if (len == 1) return;
+    // the causal initial value; terms past 48 are below 1e-27 of the first
+    // (|z|^48) and the mirrored terms vanish with z^len for long lines
     const double c0 = p[0];
     double s = c0 + zn * p[(len - 1) * stride];
     double zi = z;
-    for (int i = 1; i < len; ++i) {
+    const int nsum = len < 48 ? len : 48;
+    for (int i = 1; i < nsum; ++i) {
         s += zi * (p[i * stride] + zn * p[(len - 1 - i) * stride]);
         zi *= z;
     }
@@ -181,7 +216,12 @@ __device__ __forceinline__ void bspline_w(double t, double w[4]) {
     w[3] = t3 / 6.0;
 }
 
-// map_coordinates(order=3, mode="nearest") at (x, y, z) in unpadded voxel units
+typedef double d2v __attribute__((ext_vector_type(2), aligned(8)));
+
+// map_coordinates(order=3, mode="nearest") at (x, y, z) in unpadded voxel
+// units.  The four x taps of a row are two 16-byte loads when none of them is
+// clamped (every voxel within 10 of the volume), four 8-byte loads otherwise;
+// the sum is the same either way.
 __device__ double spline_at(const SplineVol& v, float fx, float fy, float fz) {
     double tx, ty, tz;
     const int x0 = floor_i(static_cast<double>(fx) + kSplinePad, &tx) - 1;
@@ -191,6 +231,7 @@ __device__ double spline_at(const SplineVol& v, float fx, float fy, float fz) {
     bspline_w(tx, wx);
     bspline_w(ty, wy);
     bspline_w(tz, wz);
+    const bool inner = x0 >= 0 && x0 + 3 <= v.px - 1;
     int ix[4];
     for (int i = 0; i < 4; ++i) ix[i] = clampi_(x0 + i, 0, v.px - 1);
     double acc = 0.0;
@@ -199,7 +240,18 @@ __device__ double spline_at(const SplineVol& v, float fx, float fy, float fz) {
         for (int j = 0; j < 4; ++j) {
             const double* row = v.c + (zo + clampi_(y0 + j, 0, v.py - 1)) * v.px;
             const double wyz = wz[k] * wy[j];
-            for (int i = 0; i < 4; ++i) acc += wyz * wx[i] * row[ix[i]];
+            double t[4];
+            if (inner) {
+                const d2v a = *reinterpret_cast<const d2v*>(row + x0);
+                const d2v b = *reinterpret_cast<const d2v*>(row + x0 + 2);
+                t[0] = a.x;
+                t[1] = a.y;
+                t[2] = b.x;
+                t[3] = b.y;
+            } else {
+                for (int i = 0; i < 4; ++i) t[i] = row[ix[i]];
+            }
+            for (int i = 0; i < 4; ++i) acc += wyz * wx[i] * t[i];
         }
     }
     return acc;
@@ -275,10 +327,9 @@ __global__ __launch_bounds__(kT) void k_xr_residual(Geo g, const double* __restr
     for (long long i = gtid(); i < g.n; i += gstride()) {
         double v = 0.0;
         if (valid[i]) {
-            const int x = static_cast<int>(i % g.nx);
-            const long long t = i / g.nx;
-            const int y = static_cast<int>(t % g.ny);
-            const long long zo = (t / g.ny) * g.plane;
+            int x, y, z;
+            unflat(i, g.nx, g.ny, x, y, z);
+            const long long zo = static_cast<long long>(z) * g.plane;
             for (int dy = -1; dy <= 1; ++dy) {
                 const double* row = raw + zo + static_cast<long long>(clampi_(y + dy, 0, g.ny - 1)) * g.nx;
                 for (int dx = -1; dx <= 1; ++dx) v += row[clampi_(x + dx, 0, g.nx - 1)] * w9;
@@ -307,10 +358,8 @@ __global__ __launch_bounds__(kT) void k_xr_nei(Geo g, Cps cp, const float* __res
     __shared__ double sh[16];
     double acc = 0.0;
     for (long long q = gtid(); q < cp.n; q += gstride()) {
-        const int i = static_cast<int>(q % cp.ncx);
-        const long long t = q / cp.ncx;
-        const int j = static_cast<int>(t % cp.ncy);
-        const int z = static_cast<int>(t / cp.ncy);
+        int i, j, z;
+        unflat(q, cp.ncx, cp.ncy, i, j, z);
         float d[3];
         for (int c = 0; c < 3; ++c) {
             const float* m = motion + c * g.n + static_cast<long long>(z) * g.plane;
@@ -372,10 +421,8 @@ __global__ __launch_bounds__(256) void k_xr_cp_solve(Geo g, Cps cp, const float*
     const long long q = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (q >= cp.n) return;
-    const int i = static_cast<int>(q % cp.ncx);
-    const long long t = q / cp.ncx;
-    const int j = static_cast<int>(t % cp.ncy);
-    const int z = static_cast<int>(t / cp.ncy);
+    int i, j, z;
+    unflat(q, cp.ncx, cp.ncy, i, j, z);
     const int xc = cp.r + i * cp.p, yc = cp.r + j * cp.p;
     const int w = 2 * cp.r + 1, nw = w * w;
     const float fxmax = static_cast<float>(g.rx - 1), fymax = static_cast<float>(g.ry - 1),
@@ -442,10 +489,8 @@ __global__ __launch_bounds__(kT) void k_xr_dense(Geo g, Cps cp, const double* __
     float mx = 0.0f;
     const float fp = static_cast<float>(cp.p), fr = static_cast<float>(cp.r);
     for (long long i = gtid(); i < g.n; i += gstride()) {
-        const int x = static_cast<int>(i % g.nx);
-        const long long t = i / g.nx;
-        const int y = static_cast<int>(t % g.ny);
-        const int z = static_cast<int>(t / g.ny);
+        int x, y, z;
+        unflat(i, g.nx, g.ny, x, y, z);
         float cx = __fdiv_rn(static_cast<float>(x) - fr, fp);
         float cy = __fdiv_rn(static_cast<float>(y) - fr, fp);
         cx = fminf(fmaxf(cx, 0.0f), static_cast<float>(cp.ncx));
@@ -478,10 +523,8 @@ __global__ __launch_bounds__(256) void k_xr_cp_err(Geo g, Cps cp, const double* 
     const long long q = static_cast<long long>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (q >= cp.n) return;
-    const int i = static_cast<int>(q % cp.ncx);
-    const long long t = q / cp.ncx;
-    const int j = static_cast<int>(t % cp.ncy);
-    const int z = static_cast<int>(t / cp.ncy);
+    int i, j, z;
+    unflat(q, cp.ncx, cp.ncy, i, j, z);
     const int xc = cp.r + i * cp.p, yc = cp.r + j * cp.p;
     const int w = 2 * cp.r + 1, nw = w * w;
     double s = 0.0;
@@ -625,10 +668,8 @@ __global__ __launch_bounds__(kT) void k_xr_boxes(Geo g, int npl, const Box* __re
                                                  const int* __restrict__ nbox, const uint8_t* __restrict__ valid,
                                                  int and_valid, uint8_t* __restrict__ bad) {
     for (long long i = gtid(); i < g.n; i += gstride()) {
-        const int x = static_cast<int>(i % g.nx);
-        const long long t = i / g.nx;
-        const int y = static_cast<int>(t % g.ny);
-        const int z = static_cast<int>(t / g.ny);
+        int x, y, z;
+        unflat(i, g.nx, g.ny, x, y, z);
         const Box* bz = boxes + static_cast<long long>(z) * npl;
         const int nb = nbox[z];
         uint8_t b = 0;
@@ -643,13 +684,14 @@ __global__ __launch_bounds__(kT) void k_xr_dense_err(Geo g, const double* __rest
                                                      const double* __restrict__ last, const uint8_t* __restrict__ bad,
                                                      double w9, double* __restrict__ dense, double* __restrict__ list,
                                                      unsigned long long* __restrict__ nlist) {
-    for (long long i = gtid(); i < g.n; i += gstride()) {
+    for (long long i0 = static_cast<long long>(blockIdx.x) * blockDim.x; i0 < g.n; i0 += gstride()) {
+        const long long i = i0 + threadIdx.x;   // whole waves iterate together (wave_append)
+        const bool in = i < g.n;
         double v = 0.0;
-        if (bad[i]) {
-            const int x = static_cast<int>(i % g.nx);
-            const long long t = i / g.nx;
-            const int y = static_cast<int>(t % g.ny);
-            const long long zo = (t / g.ny) * g.plane;
+        if (in && bad[i]) {
+            int x, y, z;
+            unflat(i, g.nx, g.ny, x, y, z);
+            const long long zo = static_cast<long long>(z) * g.plane;
             for (int dy = -1; dy <= 1; ++dy) {
                 const long long ro = zo + static_cast<long long>(clampi_(y + dy, 0, g.ny - 1)) * g.nx;
                 for (int dx = -1; dx <= 1; ++dx) {
@@ -657,9 +699,10 @@ __global__ __launch_bounds__(kT) void k_xr_dense_err(Geo g, const double* __rest
                     v += (first[k] * first[k] - last[k] * last[k]) * w9;
                 }
             }
-            list[atomicAdd(nlist, 1ull)] = v;
         }
-        dense[i] = v;
+        const long long at = wave_append(nlist, in && bad[i]);
+        if (at >= 0) list[at] = v;
+        if (in) dense[i] = v;
     }
 }
 
@@ -699,25 +742,37 @@ __global__ void k_sel_init(SelState* st, const unsigned long long* n, double q_o
     st->rank = static_cast<unsigned long long>(k);
 }
 
+constexpr int kSelBins = 2048;   // 11-bit digits: 64 = 5 x 11 + 9
+
+// per-workgroup LDS histogram of the digit at `shift` (`bits` wide) over the
+// values whose higher digits equal the prefix, added to the global one
 __global__ __launch_bounds__(kT) void k_sel_hist(const double* __restrict__ v, const unsigned long long* __restrict__ n,
-                                                 const SelState* __restrict__ st, int shift,
+                                                 const SelState* __restrict__ st, int shift, int bits,
                                                  unsigned* __restrict__ hist) {
+    __shared__ unsigned h[kSelBins];
+    for (int b = threadIdx.x; b < kSelBins; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
     const long long nn = static_cast<long long>(*n);
-    const unsigned long long pre = st->prefix, msk = st->mask;
+    const unsigned long long pre = st->prefix, msk = st->mask, dm = (1ull << bits) - 1ull;
     for (long long i = gtid(); i < nn; i += gstride()) {
         const unsigned long long k = key64(v[i]);
-        if ((k & msk) == pre) atomicAdd(&hist[(k >> shift) & 0xffffull], 1u);
+        if ((k & msk) == pre) atomicAdd(&h[(k >> shift) & dm], 1u);
     }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kSelBins; b += blockDim.x)
+        if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
-// one workgroup of 1024: 64 bins per thread, prefix over the threads, the bin
+// one workgroup of 1024: 2 bins per thread, prefix over the threads, the bin
 // holding the rank; clears the histogram for the next pass
-__global__ __launch_bounds__(1024) void k_sel_pick(unsigned* __restrict__ hist, int shift, SelState* st, int last) {
+__global__ __launch_bounds__(1024) void k_sel_pick(unsigned* __restrict__ hist, int shift, int bits, SelState* st,
+                                                   int last) {
+    constexpr int kPer = kSelBins / 1024;
     __shared__ unsigned long long sc[1024];
     const int t = threadIdx.x;
     const unsigned long long rank = st->rank;   // read before any thread can update it
     unsigned long long local = 0;
-    for (int b = 0; b < 64; ++b) local += hist[t * 64 + b];
+    for (int b = 0; b < kPer; ++b) local += hist[t * kPer + b];
     sc[t] = local;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
@@ -729,12 +784,12 @@ __global__ __launch_bounds__(1024) void k_sel_pick(unsigned* __restrict__ hist, 
     const unsigned long long before = t > 0 ? sc[t - 1] : 0ull;
     if (rank >= before && rank < sc[t]) {
         unsigned long long acc = before;
-        for (int b = 0; b < 64; ++b) {
-            const unsigned h = hist[t * 64 + b];
+        for (int b = 0; b < kPer; ++b) {
+            const unsigned h = hist[t * kPer + b];
             if (rank < acc + h) {
-                const unsigned long long bin = static_cast<unsigned long long>(t * 64 + b);
+                const unsigned long long bin = static_cast<unsigned long long>(t * kPer + b);
                 st->prefix |= bin << shift;
-                st->mask |= 0xffffull << shift;
+                st->mask |= ((1ull << bits) - 1ull) << shift;
                 st->rank = rank - acc;
                 if (last) st->value = unkey64(st->prefix);
                 break;
@@ -743,19 +798,19 @@ __global__ __launch_bounds__(1024) void k_sel_pick(unsigned* __restrict__ hist, 
         }
     }
     __syncthreads();
-    for (int b = 0; b < 64; ++b) hist[t * 64 + b] = 0u;
+    for (int b = 0; b < kPer; ++b) hist[t * kPer + b] = 0u;
 }
 
 // the rank-th smallest of the n (device count) values into st->value; the
-// histogram (65536 uints) must be zero on entry and is zero on exit
+// histogram (kSelBins uints) must be zero on entry and is zero on exit
 void select_rank(const double* v, const unsigned long long* n, long long rank, double q, SelState* st,
                  unsigned* hist, long long n_bound, hipStream_t s) {
     k_sel_init<<<1, 1, 0, s>>>(st, n, q, rank);
-    const unsigned g = blocks_for(n_bound);
-    for (int p = 0; p < 4; ++p) {
-        const int shift = 48 - 16 * p;
-        k_sel_hist<<<g, kT, 0, s>>>(v, n, st, shift, hist);
-        k_sel_pick<<<1, 1024, 0, s>>>(hist, shift, st, p == 3);
+    const unsigned g = std::min(blocks_for(n_bound), 512u);
+    const int shifts[6] = {53, 42, 31, 20, 9, 0}, bits[6] = {11, 11, 11, 11, 11, 9};
+    for (int p = 0; p < 6; ++p) {
+        k_sel_hist<<<g, kT, 0, s>>>(v, n, st, shifts[p], bits[p], hist);
+        k_sel_pick<<<1, 1024, 0, s>>>(hist, shifts[p], bits[p], st, p == 5);
     }
 }
 
@@ -766,21 +821,56 @@ struct Roi {
     long long n;
 };
 
+// the seeds' bounding box and count: per-thread, then per-workgroup in LDS,
+// then one atomic per quantity and workgroup
 __global__ __launch_bounds__(kT) void k_trap_bbox(Geo g, const uint8_t* __restrict__ bad, const float* __restrict__ phase,
                                                   int* __restrict__ bb) {
+    int v[7] = {INT32_MAX, -1, INT32_MAX, -1, INT32_MAX, -1, 0};
     for (long long i = gtid(); i < g.n; i += gstride()) {
         if (!bad[i]) continue;
         const int sx = static_cast<int>(rintf(phase[i]));
         const int sy = static_cast<int>(rintf(phase[g.n + i]));
         const int sz = static_cast<int>(rintf(phase[2 * g.n + i]));
         if (sx < 0 || sx >= g.rx || sy < 0 || sy >= g.ry || sz < 0 || sz >= g.rz) continue;
-        atomicMin(&bb[0], sx);
-        atomicMax(&bb[1], sx);
-        atomicMin(&bb[2], sy);
-        atomicMax(&bb[3], sy);
-        atomicMin(&bb[4], sz);
-        atomicMax(&bb[5], sz);
-        atomicAdd(&bb[6], 1);
+        v[0] = min(v[0], sx);
+        v[1] = max(v[1], sx);
+        v[2] = min(v[2], sy);
+        v[3] = max(v[3], sy);
+        v[4] = min(v[4], sz);
+        v[5] = max(v[5], sz);
+        v[6] += 1;
+    }
+    __shared__ int sh[7];
+    if (threadIdx.x < 6) sh[threadIdx.x] = (threadIdx.x & 1) ? -1 : INT32_MAX;
+    if (threadIdx.x == 6) sh[6] = 0;
+    __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {   // wave reduction first: one LDS update per wave
+        v[0] = min(v[0], __shfl_down(v[0], o, 64));
+        v[1] = max(v[1], __shfl_down(v[1], o, 64));
+        v[2] = min(v[2], __shfl_down(v[2], o, 64));
+        v[3] = max(v[3], __shfl_down(v[3], o, 64));
+        v[4] = min(v[4], __shfl_down(v[4], o, 64));
+        v[5] = max(v[5], __shfl_down(v[5], o, 64));
+        v[6] += __shfl_down(v[6], o, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && v[6] > 0) {
+        atomicMin(&sh[0], v[0]);
+        atomicMax(&sh[1], v[1]);
+        atomicMin(&sh[2], v[2]);
+        atomicMax(&sh[3], v[3]);
+        atomicMin(&sh[4], v[4]);
+        atomicMax(&sh[5], v[5]);
+        atomicAdd(&sh[6], v[6]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sh[6] > 0) {
+        atomicMin(&bb[0], sh[0]);
+        atomicMax(&bb[1], sh[1]);
+        atomicMin(&bb[2], sh[2]);
+        atomicMax(&bb[3], sh[3]);
+        atomicMin(&bb[4], sh[4]);
+        atomicMax(&bb[5], sh[5]);
+        atomicAdd(&bb[6], sh[6]);
     }
 }
 
@@ -811,10 +901,8 @@ __global__ __launch_bounds__(kT) void k_morph(Roi roi, const uint8_t* __restrict
                                               int noff, int erode, const uint8_t* __restrict__ allowed,
                                               uint8_t* __restrict__ out, int* __restrict__ changed) {
     for (long long i = gtid(); i < roi.n; i += gstride()) {
-        const int x = static_cast<int>(i % roi.wx);
-        const long long t = i / roi.wx;
-        const int y = static_cast<int>(t % roi.wy);
-        const int z = static_cast<int>(t / roi.wy);
+        int x, y, z;
+        unflat(i, roi.wx, roi.wy, x, y, z);
         bool r = erode ? true : false;
         for (int k = 0; k < noff; ++k) {
             const int xx = x + off[3 * k], yy = y + off[3 * k + 1], zz = z + off[3 * k + 2];
@@ -832,7 +920,11 @@ __global__ __launch_bounds__(kT) void k_morph(Roi roi, const uint8_t* __restrict
         }
         if (allowed) r = r && allowed[i];
         const uint8_t o = r ? 1 : 0;
-        if (changed && o != in[i]) atomicOr(changed, 1);
+        if (changed) {
+            const unsigned long long m = __ballot(o != in[i]);
+            if (m && (threadIdx.x & 63) == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1))
+                atomicOr(changed, 1);
+        }
         out[i] = o;
     }
 }
@@ -840,10 +932,8 @@ __global__ __launch_bounds__(kT) void k_morph(Roi roi, const uint8_t* __restrict
 __global__ __launch_bounds__(kT) void k_trap_crop(Roi roi, const double* __restrict__ ref_l, int rx, int ry,
                                                   float* __restrict__ out) {
     for (long long i = gtid(); i < roi.n; i += gstride()) {
-        const int x = static_cast<int>(i % roi.wx);
-        const long long t = i / roi.wx;
-        const int y = static_cast<int>(t % roi.wy);
-        const int z = static_cast<int>(t / roi.wy);
+        int x, y, z;
+        unflat(i, roi.wx, roi.wy, x, y, z);
         out[i] = static_cast<float>(
             ref_l[(static_cast<long long>(z + roi.z0) * ry + (y + roi.y0)) * rx + (x + roi.x0)]);
     }
@@ -866,10 +956,8 @@ __global__ __launch_bounds__(kT) void k_gauss_axis(Roi roi, const float* __restr
     const int n = dims[axis];
     const long long st = strides[axis];
     for (long long i = gtid(); i < roi.n; i += gstride()) {
-        const int x = static_cast<int>(i % roi.wx);
-        const long long t = i / roi.wx;
-        const int y = static_cast<int>(t % roi.wy);
-        const int z = static_cast<int>(t / roi.wy);
+        int x, y, z;
+        unflat(i, roi.wx, roi.wy, x, y, z);
         const int pos = axis == 0 ? x : (axis == 1 ? y : z);
         const long long base = i - static_cast<long long>(pos) * st;
         double acc = static_cast<double>(in[i]) * w[rad];
@@ -884,8 +972,12 @@ __global__ __launch_bounds__(kT) void k_gauss_axis(Roi roi, const float* __restr
 // values (as float64) of `vals` where `m` is set, listed in any order
 __global__ __launch_bounds__(kT) void k_list_masked(Roi roi, const float* __restrict__ vals, const uint8_t* __restrict__ m,
                                                     double* __restrict__ list, unsigned long long* __restrict__ n) {
-    for (long long i = gtid(); i < roi.n; i += gstride())
-        if (m[i]) list[atomicAdd(n, 1ull)] = static_cast<double>(vals[i]);
+    for (long long i0 = static_cast<long long>(blockIdx.x) * blockDim.x; i0 < roi.n; i0 += gstride()) {
+        const long long i = i0 + threadIdx.x;   // whole waves iterate together (wave_append)
+        const bool set = i < roi.n && m[i];
+        const long long at = wave_append(n, set);
+        if (at >= 0) list[at] = static_cast<double>(vals[i]);
+    }
 }
 
 // |v - med| in float32 for the listed values, in place
@@ -922,10 +1014,8 @@ __global__ __launch_bounds__(kT) void k_trap_write(Roi roi, const uint8_t* __res
     double c = 0.0;
     for (long long i = gtid(); i < roi.n; i += gstride()) {
         if (!(region[i] || seed[i])) continue;
-        const int x = static_cast<int>(i % roi.wx);
-        const long long t = i / roi.wx;
-        const int y = static_cast<int>(t % roi.wy);
-        const int z = static_cast<int>(t / roi.wy);
+        int x, y, z;
+        unflat(i, roi.wx, roi.wy, x, y, z);
         out[(static_cast<long long>(z + roi.z0) * ry + (y + roi.y0)) * rx + (x + roi.x0)] = 1;
         c += 1.0;
     }
@@ -948,10 +1038,8 @@ __global__ __launch_bounds__(kT) void k_xr_threshold(const double* __restrict__ 
 __global__ __launch_bounds__(kT) void k_xr_phase(int nx, int ny, int nz, double zr, double zr_hr, float* __restrict__ ph) {
     const long long n = static_cast<long long>(nx) * ny * nz;
     for (long long i = gtid(); i < n; i += gstride()) {
-        const int x = static_cast<int>(i % nx);
-        const long long t = i / nx;
-        const int y = static_cast<int>(t % ny);
-        const int z = static_cast<int>(t / ny);
+        int x, y, z;
+        unflat(i, nx, ny, x, y, z);
         ph[i] = static_cast<float>(x);
         ph[n + i] = static_cast<float>(y);
         ph[2 * n + i] = static_cast<float>(static_cast<double>(z) * zr / zr_hr);
@@ -1042,6 +1130,7 @@ int wr_spline_coef(wr_ctx* ctx, const void* in, int dtype, int nx, int ny, int n
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int px = nx + 2 * kSplinePad, py = ny + 2 * kSplinePad, pz = nz + 2 * kSplinePad;
     const long long n = static_cast<long long>(px) * py * pz;
+    WR_ARG(n <= kMaxElems, "wr_spline_coef: padded volume above 2^32 elements");
     const unsigned g = blocks_for(n);
     if (dtype == WR_F64)
         k_spline_pad<double><<<g, kT, 0, s>>>(static_cast<const double*>(in), nx, ny, nz, coef);
@@ -1102,6 +1191,7 @@ int wr_xres_optimize(wr_ctx* ctx, const wr_xres_layer* L, const double* mov_l, c
           static_cast<long long>(L->nx) * L->ny};
     Cps cp{n_cp_axis(L->nx, L->r), n_cp_axis(L->ny, L->r), L->r, 2 * L->r + 1, 0};
     WR_ARG(cp.ncx > 0 && cp.ncy > 0, "wr_xres_optimize: the level holds no control point (x or y below r + 2)");
+    WR_ARG(g.n <= kMaxElems, "wr_xres_optimize: level above 2^32 voxels");
     cp.n = static_cast<long long>(cp.ncx) * cp.ncy * L->nz;
     const unsigned gv = blocks_for(g.n), gc = blocks_for(cp.n);
     const unsigned gw = static_cast<unsigned>((cp.n + 3) / 4);
@@ -1219,11 +1309,12 @@ int wr_xres_valid(wr_ctx* ctx, const float* phase, int nx, int ny, int nz, const
     WR_HIP_CHECK(hipSetDevice(c.device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     Geo g{nx, ny, nz, rx, ry, rz, static_cast<long long>(nx) * ny * nz, static_cast<long long>(nx) * ny};
+    WR_ARG(g.n <= kMaxElems, "wr_xres_valid: volume above 2^32 voxels");
     WR_TRY(grow(c.xsel, 256));
     auto* cnt = c.xsel.as<unsigned long long>();
     k_zero_u64<<<1, 64, 0, s>>>(cnt, 1);
     const SplineVol mv = spline_vol(mask_coef ? mask_coef : nullptr, rx, ry, rz);
-    k_xr_valid<<<blocks_for(g.n), kT, 0, s>>>(g, phase, mv, mask_coef ? 1 : 0, mask_mov, valid, cnt);
+    k_xr_valid<<<blocks_atomic(g.n), kT, 0, s>>>(g, phase, mv, mask_coef ? 1 : 0, mask_mov, valid, cnt);
     WR_HIP_CHECK(hipGetLastError());
     unsigned long long h = 0;
     WR_HIP_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1244,6 +1335,7 @@ int wr_xres_bad_regions(wr_ctx* ctx, const double* residual, const uint8_t* vali
     WR_HIP_CHECK(hipSetDevice(c.device));
     hipStream_t s = static_cast<hipStream_t>(stream);
     Geo g{nx, ny, nz, 0, 0, 0, static_cast<long long>(nx) * ny * nz, static_cast<long long>(nx) * ny};
+    WR_ARG(g.n <= kMaxElems, "wr_xres_bad_regions: volume above 2^32 voxels");
     Cps cp{n_cp_axis(nx, r), n_cp_axis(ny, r), r, 2 * r + 1, 0};
     *n_regions = 0;
     if (cp.ncx == 0 || cp.ncy == 0) {
@@ -1258,7 +1350,7 @@ int wr_xres_bad_regions(wr_ctx* ctx, const double* residual, const uint8_t* vali
     const size_t bytes = align256(cp.n * sizeof(double)) + align256(static_cast<size_t>(nz) * pow2 * sizeof(double)) +
                          3 * align256(static_cast<size_t>(cp.n) * 4 * sizeof(int)) +
                          align256(static_cast<size_t>(cp.n) * sizeof(Box)) + align256((nz + 2) * sizeof(int)) +
-                         (mode == 1 ? 2 * align256(g.n * sizeof(double)) + align256(65536 * sizeof(unsigned)) +
+                         (mode == 1 ? 2 * align256(g.n * sizeof(double)) + align256(kSelBins * sizeof(unsigned)) +
                                           align256(2 * sizeof(SelState)) + align256(64)
                                     : 0);
     WR_TRY(grow(c.xsel, bytes));
@@ -1285,10 +1377,10 @@ int wr_xres_bad_regions(wr_ctx* ctx, const double* residual, const uint8_t* vali
     if (mode == 1 && total > 0) {
         double* dense = cv.take<double>(g.n);
         double* list = cv.take<double>(g.n);
-        unsigned* hist = cv.take<unsigned>(65536);
+        unsigned* hist = cv.take<unsigned>(kSelBins);
         SelState* st = cv.take<SelState>(2);
         auto* nl = cv.take<unsigned long long>(4);
-        WR_HIP_CHECK(hipMemsetAsync(hist, 0, 65536 * sizeof(unsigned), s));
+        WR_HIP_CHECK(hipMemsetAsync(hist, 0, kSelBins * sizeof(unsigned), s));
         k_zero_u64<<<1, 64, 0, s>>>(nl, 1);
         k_xr_dense_err<<<blocks_for(g.n), kT, 0, s>>>(g, first_residual, residual, bad,
                                                        static_cast<double>(1.0f / 9.0f), dense, list, nl);
@@ -1322,12 +1414,13 @@ int wr_xres_trap_mask(wr_ctx* ctx, const uint8_t* bad, const float* phase, int n
     hipStream_t s = static_cast<hipStream_t>(stream);
     Geo g{nx, ny, nz, rx, ry, rz, static_cast<long long>(nx) * ny * nz, static_cast<long long>(nx) * ny};
     const long long nref = static_cast<long long>(rx) * ry * rz;
+    WR_ARG(g.n <= kMaxElems && nref <= kMaxElems, "wr_xres_trap_mask: volume above 2^32 voxels");
     *trap_count = 0;
     k_copy_mask<<<blocks_for(nref), kT, 0, s>>>(nref, mask_ref_l, corrected);
     WR_TRY(grow(c.xsel, 4096));
     int* bb = c.xsel.as<int>();
     k_trap_bbox_init<<<1, 64, 0, s>>>(bb);
-    k_trap_bbox<<<blocks_for(g.n), kT, 0, s>>>(g, bad, phase, bb);
+    k_trap_bbox<<<blocks_atomic(g.n), kT, 0, s>>>(g, bad, phase, bb);
     WR_HIP_CHECK(hipGetLastError());
     int h[7];
     WR_HIP_CHECK(hipMemcpyAsync(h, bb, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1359,7 +1452,7 @@ int wr_xres_trap_mask(wr_ctx* ctx, const uint8_t* bad, const float* phase, int n
     const size_t nb = static_cast<size_t>(roi.n);
     const size_t bytes = 5 * align256(nb) + 3 * align256(nb * sizeof(float)) + align256(nb * sizeof(double)) +
                          align256((ball1.size() + ballE.size()) * sizeof(int)) + align256(gw.size() * sizeof(double)) +
-                         align256(65536 * sizeof(unsigned)) + align256(4 * sizeof(SelState)) + align256(256);
+                         align256(kSelBins * sizeof(unsigned)) + align256(4 * sizeof(SelState)) + align256(256);
     WR_TRY(grow(c.xroi, bytes));
     Carve cv{c.xroi.as<char>()};
     uint8_t* seed = cv.take<uint8_t>(nb);
@@ -1373,7 +1466,7 @@ int wr_xres_trap_mask(wr_ctx* ctx, const uint8_t* bad, const float* phase, int n
     double* list = cv.take<double>(nb);
     int* offs = cv.take<int>(ball1.size() + ballE.size());
     double* dgw = cv.take<double>(gw.size());
-    unsigned* hist = cv.take<unsigned>(65536);
+    unsigned* hist = cv.take<unsigned>(kSelBins);
     SelState* st = cv.take<SelState>(4);
     auto* misc = cv.take<unsigned long long>(8);   // [0] list count, [1] region count, [2] changed, [3..4] medians
     int* changed = reinterpret_cast<int*>(misc + 2);
@@ -1382,7 +1475,7 @@ int wr_xres_trap_mask(wr_ctx* ctx, const uint8_t* bad, const float* phase, int n
     WR_HIP_CHECK(hipMemcpyAsync(offs + ball1.size(), ballE.data(), ballE.size() * sizeof(int), hipMemcpyHostToDevice, s));
     if (!gw.empty()) WR_HIP_CHECK(hipMemcpyAsync(dgw, gw.data(), gw.size() * sizeof(double), hipMemcpyHostToDevice, s));
     WR_HIP_CHECK(hipMemsetAsync(seed, 0, nb, s));
-    WR_HIP_CHECK(hipMemsetAsync(hist, 0, 65536 * sizeof(unsigned), s));
+    WR_HIP_CHECK(hipMemsetAsync(hist, 0, kSelBins * sizeof(unsigned), s));
     k_zero_u64<<<1, 64, 0, s>>>(misc, 8);
     const unsigned gr = blocks_for(roi.n);
     const int n1 = static_cast<int>(ball1.size() / 3), nE = static_cast<int>(ballE.size() / 3);
@@ -1436,7 +1529,7 @@ int wr_xres_trap_mask(wr_ctx* ctx, const uint8_t* bad, const float* phase, int n
         if (!ch) break;
         std::swap(region, tmp);
     }
-    k_trap_write<<<gr, kT, 0, s>>>(roi, region, seed, rx, ry, corrected, misc + 1);
+    k_trap_write<<<blocks_atomic(roi.n), kT, 0, s>>>(roi, region, seed, rx, ry, corrected, misc + 1);
     WR_HIP_CHECK(hipGetLastError());
     unsigned long long tc = 0;
     WR_HIP_CHECK(hipMemcpyAsync(&tc, misc + 1, sizeof(tc), hipMemcpyDeviceToHost, s));
@@ -1453,7 +1546,7 @@ int wr_count_u8(wr_ctx* ctx, const uint8_t* m, long long n, long long* count, vo
     WR_TRY(grow(c.xsel, 256));
     auto* cnt = c.xsel.as<unsigned long long>();
     k_zero_u64<<<1, 64, 0, s>>>(cnt, 1);
-    if (n > 0) k_count_u8<<<blocks_for(n), kT, 0, s>>>(m, n, cnt);
+    if (n > 0) k_count_u8<<<blocks_atomic(n), kT, 0, s>>>(m, n, cnt);
     WR_HIP_CHECK(hipGetLastError());
     unsigned long long h = 0;
     WR_HIP_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));

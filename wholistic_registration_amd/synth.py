@@ -159,6 +159,38 @@ def make_pair_np(seed, shape_zyx, n_blobs, n_modes, dmax, uint16=False, mask_fra
     return Pair(tmpl, mov, d, mask)
 
 
+def make_xres_pair_np(seed, mov_xyz, ref_xyz, z_ratio, z_ratio_hr, n_blobs, dmax, artefacts=0,
+                      artefact_amp=900.0):
+    """Host-side cross-resolution pair in the solver's ``(x, y, z)`` layout:
+    a blob reference on its own z grid, and a moving volume on a coarser z
+    grid that samples the noise-free reference at ``(x, y, k zRatio /
+    zRatio_HR) + d`` (trilinear) for a smooth 3-mode field ``d`` of max
+    |d| = dmax (reference index units), plus ``artefacts`` bright flat blobs
+    that the reference does not hold, then Poisson noise on both.  Returns
+    ``(mov_xyz, ref_xyz, rng)`` (float32; the generator continues for masks
+    and warm starts)."""
+    from scipy import ndimage as ndi
+    rng = np.random.default_rng(seed)
+    rx, ry, rz = ref_xyz
+    mx, my, mz = mov_xyz
+    ref_zyx = (rz, ry, rx)
+    blobs = draw_blobs(rng, ref_zyx, n_blobs)
+    clean = render_template_np(ref_zyx, blobs).astype(np.float64)
+    modes = draw_modes(rng, 3)
+    d = field_np((mz, my, mx), modes, dmax).astype(np.float64)
+    k, yy, xx = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    zs = k * (z_ratio / z_ratio_hr)
+    coords = np.stack([zs + d[..., 2], yy + d[..., 1], xx + d[..., 0]])
+    mov_clean = ndi.map_coordinates(clean, coords, order=1, mode="nearest")
+    for _ in range(artefacts):
+        cz, cy, cx = rng.integers(0, mz), rng.integers(6, my - 6), rng.integers(6, mx - 6)
+        rr = ((yy - cy) ** 2 + (xx - cx) ** 2) / 9.0 + (k - cz) ** 2
+        mov_clean = mov_clean + artefact_amp * np.exp(-rr)
+    ref = rng.poisson(clean).astype(np.float32)
+    mov = rng.poisson(np.maximum(mov_clean, 0)).astype(np.float32)
+    return np.ascontiguousarray(mov.transpose(2, 1, 0)), np.ascontiguousarray(ref.transpose(2, 1, 0)), rng
+
+
 # ---------------------------------------------------------------- torch side
 
 
