@@ -21,6 +21,14 @@ Contents
       float64; it is the FP64 oracle the endpoint-error gate is quoted
       against.
 
+``xres_oracle``
+    numpy/scipy restatement of the cross-resolution solver ``getMotion_v2``
+    with its wrong-region correction (``calFlowCrossResolution.py:
+    463-477,1144-2663``), making the reference's own numpy/scipy calls with
+    its cast points; pinned bit for bit by fixtures written by running the
+    reference (``tests/golden/make_golden_xres.py``) and by the known answers
+    of the reference's geometry regression tests.
+
 ``metrics``
     endpoint-error statistics restated from
     ``src/wholistic_registration/utils/simulation.py:303-339``.
