@@ -548,6 +548,19 @@ int wr_xres_trap_mask(wr_ctx* ctx, const uint8_t* bad, const float* phase, int n
                       double expand_radius_xy, double sigma, double intensity_k, uint8_t* corrected,
                       long long* trap_count, void* stream);
 
+/* FindInitZ_stack_global_fixed_spacing (:370-460): the reference z of each of
+ * nk moving slices spaced delta * direction apart, chosen by the summed
+ * weighted ZNCC (Hann window when use_hann) of the slices' Sobel gradient
+ * magnitudes (raw values when use_gradient = 0) against the nz reference
+ * slices.  mov (nk, w, h) and ref (nz, w, h) float32 device stacks (the
+ * reference's (h, w, k) arrays transposed); z_init (host, nk) the chosen
+ * z indices truncated to uint16 as the reference returns them; scores_out
+ * (host, nk * nz, nullable) the ZNCC matrix; best_score / best_z0 (host,
+ * nullable).  WR_E_ARG with the reference's message when no start fits. */
+int wr_find_init_z(wr_ctx* ctx, const float* mov, int nk, const float* ref, int nz, int h, int w, double delta,
+                   int use_gradient, int use_hann, int direction, double eps, uint16_t* z_init, float* scores_out,
+                   double* best_score, int* best_z0, void* stream);
+
 /* Number of nonzero bytes of m (int(cp.sum(valid_mask)), :2375-2378). */
 int wr_count_u8(wr_ctx* ctx, const uint8_t* m, long long n, long long* count, void* stream);
 

@@ -476,3 +476,82 @@ def get_motion_v2(data_mov, data_ref, option, schedules=None, trace=None, info=N
             info[level] = lvl_info
         motion, phase, sampled = res["motion"], res["phase"], res["sampled"]
     return np.asarray(phase), np.asarray(motion), np.asarray(sampled)
+
+
+# ---------------------------------------------------------------- z-offset search
+
+
+def grad_mag_2d(img):
+    """``calculate.grad_mag_2d`` (calculate.py:109-112): Sobel magnitude with
+    replicated edges, float32."""
+    gx = ndi.sobel(img, axis=1, mode="nearest")
+    gy = ndi.sobel(img, axis=0, mode="nearest")
+    return np.sqrt(gx * gx + gy * gy).astype(np.float32)
+
+
+def hann2d(h, w):
+    """``calculate.hann2d`` (calculate.py:115-121)."""
+    wy = np.hanning(h) if h > 1 else np.ones(1, dtype=np.float32)
+    wx = np.hanning(w) if w > 1 else np.ones(1, dtype=np.float32)
+    return np.maximum(np.outer(wy, wx).astype(np.float32), 1e-3)
+
+
+def zncc_scores(mov_feat, ref_feat, use_hann_weight=True, eps=1e-6):
+    """``_global_zncc_scores`` (:332-367): (K, Z) weighted ZNCC of every moving
+    slice with every reference slice, clipped to [-1, 1]."""
+    h, w, k_n = mov_feat.shape
+    mov_feat = mov_feat.astype(np.float32, copy=False)
+    ref_feat = ref_feat.astype(np.float32, copy=False)
+    weight = hann2d(h, w).astype(np.float32) if use_hann_weight else np.ones((h, w), dtype=np.float32)
+    weight = weight / (np.sum(weight) + eps)
+    scores = np.zeros((k_n, ref_feat.shape[2]), dtype=np.float32)
+    ref_mean = np.sum(weight[:, :, None] * ref_feat, axis=(0, 1))
+    ref_c = ref_feat - ref_mean[None, None, :]
+    ref_var = np.sum(weight[:, :, None] * ref_c * ref_c, axis=(0, 1))
+    for k in range(k_n):
+        m = mov_feat[:, :, k]
+        mm = np.sum(weight * m)
+        mc = m - mm
+        mv = np.sum(weight * mc * mc)
+        num = np.sum(weight[:, :, None] * mc[:, :, None] * ref_c, axis=(0, 1))
+        scores[k, :] = num / (np.sqrt(mv * ref_var) + eps)
+    return np.clip(scores, -1.0, 1.0)
+
+
+def find_init_z(data_mov, data_ref, delta_ref_idx, use_gradient=True, use_hann_weight=True, direction=1,
+                weight_eps=1e-6, return_debug=False):
+    """``FindInitZ_stack_global_fixed_spacing`` (:370-460): the reference z of
+    each moving slice, slices a fixed spacing apart, chosen by the summed
+    ZNCC of their gradient magnitudes."""
+    mov = np.asarray(data_mov, dtype=np.float32)
+    ref = np.asarray(data_ref, dtype=np.float32)
+    if mov.ndim != 3 or ref.ndim != 3:
+        raise ValueError("data_ref must be 3D (H,W,Z)")
+    h, w, k_n = mov.shape
+    if ref.shape[:2] != (h, w):
+        raise ValueError(f"XY size mismatch: data_mov has {(h, w)}, data_ref has {ref.shape[:2]}")
+    z_n = ref.shape[2]
+    if use_gradient:
+        mf = np.stack([grad_mag_2d(mov[:, :, k]) for k in range(k_n)], axis=2)
+        rf = np.stack([grad_mag_2d(ref[:, :, z]) for z in range(z_n)], axis=2)
+    else:
+        mf, rf = mov, ref
+    scores = zncc_scores(mf, rf, use_hann_weight, weight_eps)
+    offsets = np.arange(k_n, dtype=np.float32) * float(delta_ref_idx) * float(direction)
+    z0_min = int(np.ceil(-np.min(offsets)).item())
+    z0_max = int(np.floor((z_n - 1) - np.max(offsets)).item())
+    if z0_max < z0_min:
+        raise ValueError(f"No valid z0 exists. Z={z_n}, K={k_n}, delta_ref_idx={delta_ref_idx}. "
+                         f"Need reference z range >= {(k_n - 1) * abs(delta_ref_idx) + 1}.")
+    best, best_init, best_z0 = -np.inf, None, None
+    for z0 in range(z0_min, z0_max + 1):
+        z_init = z0 + offsets
+        idx = np.rint(z_init).astype(np.int32)
+        sc = np.sum(scores[np.arange(k_n), idx])
+        if sc > best:
+            best, best_init, best_z0 = sc, z_init.copy(), z0
+    out = best_init.astype(np.uint16)
+    if return_debug:
+        return out, {"scores": scores, "best_score": float(best), "best_z0": int(best_z0),
+                     "delta_ref_idx": float(delta_ref_idx), "direction": int(direction)}
+    return out

@@ -71,3 +71,26 @@ def xres_inputs(name):
         f = synth.field_np((mz, my, mx), synth.draw_modes(rng, 2), c["init"])
         opt["motion"] = np.ascontiguousarray(f.transpose(2, 1, 0, 3))
     return mov_xyz, ref_xyz, opt
+
+
+# z-offset search (FindInitZ_stack_global_fixed_spacing): name -> (seed, (H, W),
+# K moving slices, Z reference slices, true z0, spacing, direction, gradient, hann)
+FINDZ_CASES = {
+    "grad_hann": (311, (96, 80), 6, 40, 9, 3.0, 1, True, True),
+    "raw_flat": (312, (64, 72), 5, 30, 17, 2.5, 1, False, False),
+    "backward": (313, (80, 64), 4, 36, 28, 4.0, -1, True, True),
+}
+
+
+def findz_inputs(name):
+    """``(mov (H, W, K), ref (H, W, Z), delta, direction, use_gradient,
+    use_hann)``: moving slices are reference slices z0 + k * spacing *
+    direction (rounded) with fresh Poisson noise."""
+    seed, (h, w), k_n, z_n, z0, delta, direction, grad, hann = FINDZ_CASES[name]
+    rng = np.random.default_rng(seed)
+    blobs = synth.draw_blobs(rng, (z_n, w, h), 6 * z_n)
+    clean = synth.render_template_np((z_n, w, h), blobs).astype(np.float64)      # (Z, W, H)
+    ref = rng.poisson(clean).astype(np.float32).transpose(2, 1, 0)
+    zs = [int(np.rint(z0 + k * delta * direction)) for k in range(k_n)]
+    mov = np.stack([rng.poisson(clean[z]).astype(np.float32).T for z in zs], axis=2)
+    return np.ascontiguousarray(mov), np.ascontiguousarray(ref), delta, direction, grad, hann

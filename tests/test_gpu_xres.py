@@ -257,3 +257,30 @@ def test_projection_samples_the_reference_when_values_are_none(cuda):
     want = want.transpose(0, 2, 1)              # the default output order is (Nplanes, Yout, Xout)
     assert got.shape == want.shape
     assert np.abs(got - want).max() <= 1e-3
+
+
+@pytest.mark.parametrize("name", ["grad_hann", "raw_flat", "backward"])
+def test_find_init_z_matches_reference(cuda, name):
+    """The z-offset search: the score matrix within 1e-5 of the reference's
+    (its float32 sums against float64 here), the planted start recovered and
+    the chosen indices equal to the oracle's."""
+    from tests.golden.xres_cases import FINDZ_CASES, findz_inputs
+    X = _X()
+    mov, ref, delta, direction, grad, hann = findz_inputs(name)
+    z, dbg = X.FindInitZ_stack_global_fixed_spacing(mov, ref, delta, use_gradient=grad, use_hann_weight=hann,
+                                                    direction=direction, return_debug=True)
+    g = np.load(os.path.join(HERE, f"findz_{name}.npz"))
+    assert np.abs(dbg["scores"] - g["scores"]).max() <= 1e-5
+    want, wdbg = xo.find_init_z(mov, ref, delta, grad, hann, direction, return_debug=True)
+    assert np.array_equal(z, want) and z.dtype == np.uint16
+    assert dbg["best_z0"] == wdbg["best_z0"] == FINDZ_CASES[name][4]
+    assert abs(dbg["best_score"] - wdbg["best_score"]) <= 1e-4
+
+
+def test_find_init_z_rejects_impossible_spacing(cuda):
+    X = _X()
+    rng = np.random.default_rng(0)
+    with pytest.raises(ValueError, match="No valid z0 exists"):
+        X.FindInitZ_stack_global_fixed_spacing(rng.uniform(size=(16, 16, 5)), rng.uniform(size=(16, 16, 8)), 3.0)
+    with pytest.raises(ValueError, match="XY size mismatch"):
+        X.FindInitZ_stack_global_fixed_spacing(rng.uniform(size=(16, 16, 2)), rng.uniform(size=(16, 12, 8)), 1.0)

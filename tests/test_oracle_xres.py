@@ -210,3 +210,26 @@ def test_spline_prefilter_restatement_matches_scipy():
                        for i in range(4) for j in range(4) for k in range(4)))
     want = ndi.map_coordinates(vol, pts.T, order=3, mode="nearest")
     assert np.abs(np.array(got) - want).max() < 1e-8
+
+
+@pytest.mark.parametrize("name", ["grad_hann", "raw_flat", "backward"])
+def test_oracle_find_init_z_against_reference_scores_and_planted_start(name):
+    """The slice features and ZNCC scores equal the reference's own
+    (grad_mag_2d, _global_zncc_scores) bit for bit; the search recovers the
+    planted start, and brute force over all starts agrees."""
+    from tests.golden.xres_cases import FINDZ_CASES, findz_inputs
+    mov, ref, delta, direction, grad, hann = findz_inputs(name)
+    g = np.load(os.path.join(HERE, f"findz_{name}.npz"))
+    z, dbg = xo.find_init_z(mov, ref, delta, grad, hann, direction, return_debug=True)
+    assert np.array_equal(dbg["scores"], g["scores"])
+    if grad:
+        assert np.array_equal(xo.grad_mag_2d(mov[:, :, 0]), g["mov_feat"])
+        assert np.array_equal(xo.grad_mag_2d(ref[:, :, 0]), g["ref_feat"])
+    z0 = FINDZ_CASES[name][4]
+    assert dbg["best_z0"] == z0
+    k = np.arange(mov.shape[2])
+    offs = (k.astype(np.float32) * np.float32(delta) * np.float32(direction))
+    assert np.array_equal(z, (z0 + offs).astype(np.uint16))
+    starts = [s for s in range(ref.shape[2]) if 0 <= s + offs.min() and s + offs.max() <= ref.shape[2] - 1]
+    sums = [np.sum(dbg["scores"][k, np.rint(s + offs).astype(int)]) for s in starts]
+    assert starts[int(np.argmax(sums))] == z0

@@ -161,6 +161,8 @@ _SIGNATURES = {
     "wr_xres_trap_mask": ([_VP, _VP, _VP, _I, _I, _I, _VP, _I, _I, _I, _VP, _D, _D, _D, _D, _VP,
                            ctypes.POINTER(_LL), _VP], _I),
     "wr_count_u8": ([_VP, _VP, _LL, ctypes.POINTER(_LL), _VP], _I),
+    "wr_find_init_z": ([_VP, _VP, _I, _VP, _I, _I, _I, _D, _I, _I, _I, _D, _VP, _VP, ctypes.POINTER(_D),
+                        ctypes.POINTER(_I), _VP], _I),
     "wr_xres_scale": ([_VP, _VP, _LL, _D, _D, _VP, _VP], _I),
     "wr_xres_threshold": ([_VP, _VP, _LL, _D, _VP, _VP], _I),
     "wr_xres_phase_identity": ([_VP, _I, _I, _I, _D, _D, _VP, _VP], _I),

@@ -609,3 +609,51 @@ def getMotion_v2(data_mov, data_ref, option, verbose=False, *, schedules=None, t
     motion_np = _planes_to_field(res["motion"])
     sampled_np = res["data_ref_sampled"].permute(2, 1, 0).cpu().numpy()
     return phase_np, motion_np, sampled_np
+
+
+# ============================================================================
+# z-offset search (:370-460)
+# ============================================================================
+
+
+def _stack_3d(a, what, dev):
+    torch = _torch()
+    t = a.to(dev, torch.float32) if _is_tensor(a) else torch.from_numpy(np.asarray(a, dtype=np.float32)).to(dev)
+    if t.dim() != 3:
+        raise ValueError(f"{what} must be 3D (H,W,Z), got shape {tuple(t.shape)}")
+    return t
+
+
+def FindInitZ_stack_global_fixed_spacing(data_mov, data_ref, delta_ref_idx, use_gradient=True, use_hann_weight=True,
+                                         direction=1, weight_eps=1e-6, return_debug=False):
+    """The reference z of each moving slice (slices ``delta_ref_idx *
+    direction`` apart), chosen by the summed Hann-weighted ZNCC of the
+    slices' Sobel gradient magnitudes (calFlowCrossResolution.py:370-460).
+    ``data_mov`` (H, W, K), ``data_ref`` (H, W, Z); returns the z indices as
+    uint16 (and the reference's debug dict with ``return_debug``)."""
+    dev = _device()
+    mov = _stack_3d(data_mov, "data_ref", dev)
+    ref = _stack_3d(data_ref, "data_ref", dev)
+    h, w, k_n = mov.shape
+    hr, wr_, z_n = ref.shape
+    if (hr, wr_) != (h, w):
+        raise ValueError(f"XY size mismatch: data_mov has {(h, w)}, data_ref has {(hr, wr_)}")
+    if k_n < 1:
+        raise ValueError("Moving input must have at least one slice.")
+    if z_n < 1:
+        raise ValueError("Reference input must have at least one slice.")
+    m = mov.permute(2, 1, 0).contiguous()
+    r = ref.permute(2, 1, 0).contiguous()
+    z_init = np.zeros(k_n, dtype=np.uint16)
+    scores = np.zeros((k_n, z_n), dtype=np.float32)
+    best = ctypes.c_double(0.0)
+    z0 = ctypes.c_int(0)
+    ctx, st = _ctx_stream(dev)
+    _lib.check(_lib.load().wr_find_init_z(
+        ctx, _vp(m), k_n, _vp(r), z_n, h, w, float(delta_ref_idx), int(bool(use_gradient)),
+        int(bool(use_hann_weight)), int(direction), float(weight_eps), z_init.ctypes.data_as(ctypes.c_void_p),
+        scores.ctypes.data_as(ctypes.c_void_p), ctypes.byref(best), ctypes.byref(z0), st), "FindInitZ")
+    if return_debug:
+        return z_init, {"scores": scores, "best_score": float(best.value), "best_z0": int(z0.value),
+                        "delta_ref_idx": float(delta_ref_idx), "direction": int(direction)}
+    return z_init

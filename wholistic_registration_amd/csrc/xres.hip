@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <vector>
 
 #include "wr_internal.h"
@@ -1046,6 +1047,81 @@ __global__ __launch_bounds__(kT) void k_xr_phase(int nx, int ny, int nz, double 
     }
 }
 
+// ------------------------------------------------------------ z-offset search
+
+// Sobel gradient magnitude of every slice (calculate.py:109-112): slices of h
+// (fastest) x w, each step rounded to float32 as scipy writes it: the
+// difference along one axis, then the [1, 2, 1] smoothing along the other
+// (replicated edges), then sqrt(gx^2 + gy^2).
+__global__ __launch_bounds__(kT) void k_fz_gradmag(const float* __restrict__ in, int h, int w, long long n,
+                                                   float* __restrict__ out) {
+    for (long long i = gtid(); i < n; i += gstride()) {
+        int a, b, sl;
+        unflat(i, h, w, a, b, sl);   // a along h (axis 0), b along w (axis 1)
+        const float* p = in + static_cast<long long>(sl) * h * w;
+        auto at = [&](int aa, int bb) { return p[static_cast<long long>(clampi_(bb, 0, w - 1)) * h + clampi_(aa, 0, h - 1)]; };
+        // derivative along w, smoothed along h
+        float d1[3];
+        for (int k = -1; k <= 1; ++k) d1[k + 1] = at(a + k, b + 1) - at(a + k, b - 1);
+        const float gx = static_cast<float>(static_cast<double>(d1[1]) * 2.0 +
+                                            (static_cast<double>(d1[0]) + static_cast<double>(d1[2])));
+        // derivative along h, smoothed along w
+        float d0[3];
+        for (int k = -1; k <= 1; ++k) d0[k + 1] = at(a + 1, b + k) - at(a - 1, b + k);
+        const float gy = static_cast<float>(static_cast<double>(d0[1]) * 2.0 +
+                                            (static_cast<double>(d0[0]) + static_cast<double>(d0[2])));
+        out[i] = sqrtf(gx * gx + gy * gy);
+    }
+}
+
+// per slice: weighted mean and weighted variance about it (float64 sums)
+__global__ __launch_bounds__(1024) void k_fz_stats(const float* __restrict__ f, const float* __restrict__ wt, long long hw,
+                                                   double* __restrict__ mean, double* __restrict__ var) {
+    __shared__ double sh[16];
+    const float* p = f + static_cast<long long>(blockIdx.x) * hw;
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < hw; i += blockDim.x) s += static_cast<double>(wt[i]) * p[i];
+    s = block_sum(s, sh);
+    __shared__ double m;
+    if (threadIdx.x == 0) m = s;
+    __syncthreads();
+    const float mf = static_cast<float>(m);
+    double v = 0.0;
+    for (long long i = threadIdx.x; i < hw; i += blockDim.x) {
+        const float c = p[i] - mf;
+        v += static_cast<double>(wt[i]) * c * c;
+    }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+        mean[blockIdx.x] = m;
+        var[blockIdx.x] = v;
+    }
+}
+
+// one workgroup per (moving slice k, reference slice z): the weighted ZNCC
+__global__ __launch_bounds__(1024) void k_fz_zncc(const float* __restrict__ mf, const float* __restrict__ rf,
+                                                  const float* __restrict__ wt, long long hw, int nz,
+                                                  const double* __restrict__ mmean, const double* __restrict__ mvar,
+                                                  const double* __restrict__ rmean, const double* __restrict__ rvar,
+                                                  double eps, float* __restrict__ scores) {
+    __shared__ double sh[16];
+    const int k = blockIdx.x / nz, z = blockIdx.x % nz;
+    const float* a = mf + static_cast<long long>(k) * hw;
+    const float* b = rf + static_cast<long long>(z) * hw;
+    const float ma = static_cast<float>(mmean[k]), mb = static_cast<float>(rmean[z]);
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < hw; i += blockDim.x)
+        s += static_cast<double>(wt[i]) * static_cast<double>(a[i] - ma) * static_cast<double>(b[i] - mb);
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        const float mv = static_cast<float>(mvar[k]), rv = static_cast<float>(rvar[z]);
+        const float num = static_cast<float>(s);
+        float sc = num / (sqrtf(mv * rv) + static_cast<float>(eps));
+        sc = fminf(fmaxf(sc, -1.0f), 1.0f);
+        scores[blockIdx.x] = sc;
+    }
+}
+
 // ------------------------------------------------------------ host helpers
 
 int n_cp_axis(int n, int r) {
@@ -1552,6 +1628,100 @@ int wr_count_u8(wr_ctx* ctx, const uint8_t* m, long long n, long long* count, vo
     WR_HIP_CHECK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, s));
     WR_HIP_CHECK(hipStreamSynchronize(s));
     *count = static_cast<long long>(h);
+    return WR_OK;
+}
+
+int wr_find_init_z(wr_ctx* ctx, const float* mov, int nk, const float* ref, int nz, int h, int w, double delta,
+                    int use_gradient, int use_hann, int direction, double eps, uint16_t* z_init, float* scores_out,
+                    double* best_score, int* best_z0, void* stream) {
+    WR_ARG(ctx && mov && ref && z_init, "wr_find_init_z: null argument");
+    WR_ARG(nk >= 1, "Moving input must have at least one slice.");
+    WR_ARG(nz >= 1, "Reference input must have at least one slice.");
+    WR_ARG(h >= 1 && w >= 1, "wr_find_init_z: empty slice");
+    const long long hw = static_cast<long long>(h) * w;
+    WR_ARG(hw * (nk + nz) <= kMaxElems, "wr_find_init_z: stacks above 2^32 voxels");
+    // offsets (float32 as the reference forms them) and the admissible starts
+    std::vector<float> off(nk);
+    for (int k = 0; k < nk; ++k) off[k] = static_cast<float>(k) * static_cast<float>(delta) * static_cast<float>(direction);
+    const float omin = *std::min_element(off.begin(), off.end()), omax = *std::max_element(off.begin(), off.end());
+    const int z0_min = static_cast<int>(std::ceil(-omin));
+    const int z0_max = static_cast<int>(std::floor(static_cast<float>(nz - 1) - omax));
+    if (z0_max < z0_min) {
+        char buf[256];
+        std::snprintf(buf, sizeof(buf),
+                      "No valid z0 exists. Z=%d, K=%d, delta_ref_idx=%g. Need reference z range >= %g.", nz, nk, delta,
+                      (nk - 1) * std::fabs(delta) + 1);
+        set_error(buf);
+        return WR_E_ARG;
+    }
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the normalised weights: Hann window (numpy hanning, outer product rounded to
+    // float32, floored at 1e-3) or ones, divided by their sum + eps in float32
+    std::vector<float> wt(hw);
+    std::vector<double> wa(h, 1.0), wb(w, 1.0);
+    if (use_hann) {
+        const double pi = 3.14159265358979323846;
+        if (h > 1)
+            for (int i = 0; i < h; ++i) wa[i] = 0.5 + 0.5 * std::cos(pi * (1 - h + 2 * i) / (h - 1));
+        if (w > 1)
+            for (int j = 0; j < w; ++j) wb[j] = 0.5 + 0.5 * std::cos(pi * (1 - w + 2 * j) / (w - 1));
+    }
+    double sum = 0.0;
+    for (int j = 0; j < w; ++j)
+        for (int i = 0; i < h; ++i) {
+            float v = use_hann ? std::max(static_cast<float>(wa[i] * wb[j]), 1e-3f) : 1.0f;
+            wt[static_cast<size_t>(j) * h + i] = v;
+            sum += v;
+        }
+    const float den = static_cast<float>(sum) + static_cast<float>(eps);
+    for (float& v : wt) v = v / den;
+    const size_t bytes = align256(hw * sizeof(float)) + (use_gradient ? align256(hw * (nk + nz) * sizeof(float)) : 0) +
+                         4 * align256((nk + nz) * sizeof(double)) + align256(static_cast<size_t>(nk) * nz * sizeof(float));
+    WR_TRY(grow(c.xsel, bytes));
+    Carve cv{c.xsel.as<char>()};
+    float* dwt = cv.take<float>(hw);
+    const float* mf = mov;
+    const float* rf = ref;
+    if (use_gradient) {
+        float* g = cv.take<float>(hw * (nk + nz));
+        k_fz_gradmag<<<blocks_for(hw * nk), kT, 0, s>>>(mov, h, w, hw * nk, g);
+        k_fz_gradmag<<<blocks_for(hw * nz), kT, 0, s>>>(ref, h, w, hw * nz, g + hw * nk);
+        mf = g;
+        rf = g + hw * nk;
+    }
+    double* mm = cv.take<double>(nk);
+    double* mv = cv.take<double>(nk);
+    double* rm = cv.take<double>(nz);
+    double* rv = cv.take<double>(nz);
+    float* sc = cv.take<float>(static_cast<size_t>(nk) * nz);
+    WR_HIP_CHECK(hipMemcpyAsync(dwt, wt.data(), hw * sizeof(float), hipMemcpyHostToDevice, s));
+    k_fz_stats<<<nk, 1024, 0, s>>>(mf, dwt, hw, mm, mv);
+    k_fz_stats<<<nz, 1024, 0, s>>>(rf, dwt, hw, rm, rv);
+    k_fz_zncc<<<nk * nz, 1024, 0, s>>>(mf, rf, dwt, hw, nz, mm, mv, rm, rv, eps, sc);
+    WR_HIP_CHECK(hipGetLastError());
+    std::vector<float> hs(static_cast<size_t>(nk) * nz);
+    WR_HIP_CHECK(hipMemcpyAsync(hs.data(), sc, hs.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    WR_HIP_CHECK(hipStreamSynchronize(s));
+    // the search (:432-446): first start with the largest float32 score sum
+    float best = -INFINITY;
+    int bz = z0_min;
+    for (int z0 = z0_min; z0 <= z0_max; ++z0) {
+        float acc = 0.0f;
+        for (int k = 0; k < nk; ++k) {
+            const int zi = static_cast<int>(std::nearbyint(static_cast<float>(z0) + off[k]));
+            acc += hs[static_cast<size_t>(k) * nz + zi];
+        }
+        if (acc > best) {
+            best = acc;
+            bz = z0;
+        }
+    }
+    for (int k = 0; k < nk; ++k) z_init[k] = static_cast<uint16_t>(static_cast<float>(bz) + off[k]);
+    if (scores_out) std::copy(hs.begin(), hs.end(), scores_out);
+    if (best_score) *best_score = best;
+    if (best_z0) *best_z0 = bz;
     return WR_OK;
 }
 
