@@ -626,6 +626,34 @@ def learned_measure(n_templates=16, per_template=HELD_OUT_FRAMES, reps=3, weight
     out.update(conc)
     out["speedup_learned_bf16_4ctx"] = conc["plain_fp32_4ctx"]["ms_per_frame"] / conc["learned_bf16_4ctx"]["ms_per_frame"]
     del frames_first, batch, wo, mo
+    # the first template's 16 held-out frames as one batch: the plain method's
+    # batched launch sequence against the learned start's (one network pass of
+    # batch 16, then one batched sequence warm-started by the control points)
+    clean0, tmpl0 = T.make_template(HELD_OUT_TEMPLATES[0], dev)
+    base0 = dict(r=8, zRatio=1.0, movRange=5.0, smoothPenalty=T.penalty(tmpl0), tol=1e-3)
+    ref_l0 = D.pyramid_level(tmpl0, Lr.LEVEL)
+    movs = torch.stack([T.make_moving(clean0, HELD_OUT_TEMPLATES[0] + i, dev)[0] for i in range(1, 17)])
+    mo = torch.empty(tuple(movs.shape) + (3,), dtype=torch.float32, device=dev)
+    mo_l = torch.empty_like(mo)
+    wo = torch.empty(tuple(movs.shape), dtype=torch.float32, device=dev)
+    runs = {"plain_fp32_batched16": lambda: D.get_motion_batch(movs, tmpl0, layer=4, iter=15, motion_out=mo,
+                                                                warp_src=movs, warped_out=wo, **base0),
+            "learned_bf16_batched16": lambda: Lr.register_learned_batch(
+                movs, tmpl0, net, ref_level=ref_l0, precision="bf16", chunk=16, motion_out=mo_l, warp_src=movs,
+                warped_out=wo, **Lr.SCHEDULE, **base0)}
+    for name, fn in runs.items():
+        _, ms = timed(fn, 3)
+        ms /= movs.shape[0]
+        out[name] = {"value": n0 / (ms * 1e-3), "unit": "registered voxels/s", "ms_per_frame": ms,
+                     "frames": int(movs.shape[0])}
+    e = torch.sqrt(((mo_l - mo) ** 2).sum(-1)).mean(dim=(1, 2, 3)).cpu().numpy()
+    out["learned_bf16_batched16"]["frames_within_gate"] = int((e <= 0.1).sum())
+    out["learned_bf16_batched16"]["epe_vs_plain_mean"] = float(e.mean())
+    out["speedup_learned_bf16_batched16"] = (out["plain_fp32_batched16"]["ms_per_frame"] /
+                                             out["learned_bf16_batched16"]["ms_per_frame"])
+    out["speedup_learned_bf16_batched16_vs_plain_single"] = (out["plain_fp32"]["ms_per_frame"] /
+                                                             out["learned_bf16_batched16"]["ms_per_frame"])
+    del movs, mo, mo_l, wo
     m_l = D.pyramid_level(T.make_moving(T.make_template(HELD_OUT_TEMPLATES[0], dev)[0], HELD_OUT_TEMPLATES[0] + 1,
                                         dev)[0], Lr.LEVEL)
     ref_l = D.pyramid_level(T.make_template(HELD_OUT_TEMPLATES[0], dev)[1], Lr.LEVEL)

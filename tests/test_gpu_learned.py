@@ -136,6 +136,31 @@ def test_learned_frames_on_concurrent_contexts_equal_single_runs(setup):
         assert single.records == res.records
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_batched_learned_frames_equal_single_runs(setup, precision):
+    """register_learned_batch (one network pass of batch 3, one batched
+    launch sequence warm-started by each frame's control points, the warp
+    formed with the field) gives every frame the field, schedule and warp of
+    register_learned on the frame alone."""
+    import torch
+
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    s = setup
+    movs = torch.stack([T.make_moving(s["clean"], seed, s["tmpl"].device)[0] for seed in (90006, 90007, 90008)])
+    mo = torch.empty(tuple(movs.shape) + (3,), dtype=torch.float32, device=movs.device)
+    wo = torch.empty(tuple(movs.shape), dtype=torch.float32, device=movs.device)
+    res = Lr.register_learned_batch(movs, s["tmpl"], s["net"], ref_level=s["ref_l"], precision=precision, chunk=3,
+                                    motion_out=mo, warp_src=movs, warped_out=wo, **Lr.SCHEDULE, **s["params"])
+    for f in range(movs.shape[0]):
+        w1 = torch.empty_like(wo[f])
+        single, _ = Lr.register_learned(movs[f], s["tmpl"], s["net"], ref_level=s["ref_l"], precision=precision,
+                                        warp_src=movs[f], warped_out=w1, **Lr.SCHEDULE, **s["params"])
+        assert torch.equal(single.motion, mo[f])
+        assert single.records == res[f].records and single.updates == res[f].updates
+        assert torch.equal(w1, wo[f])
+
+
 def test_learned_start_on_uneven_crops(setup):
     """Crops whose level-3 image is not a multiple of the network's (4, 4, 32)
     grid run on an edge-padded grid; crops on concurrent contexts equal their

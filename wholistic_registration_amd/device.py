@@ -248,13 +248,15 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
 def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                      mask_ref=None, mask_mov=None, motion_init=None, motion_out=None, tile_patches_x=0,
                      timing=False, stop_kernels=False, precision="fp32", reuse_template=False, context=None,
-                     stream=None, schedule=None, logs=True, warp_src=None, warped_out=None):
+                     stream=None, schedule=None, logs=True, warp_src=None, warped_out=None, motion_init_cp=None):
     """getMotion of every frame of ``movs`` (F, Z, Y, X) against ``ref`` in one
     launch sequence (wr_get_motion_batch): one launch per kernel and
     level-iteration over all frames, each frame with its own stop rules.
     Every frame's field equals ``get_motion(movs[f], ref, ...)`` bit for bit.
     ``mask_mov`` (F, Z, Y, X) float32 and ``motion_init`` (F, Z, Y, X, 3)
-    are per frame; ``motion_out`` is (F, Z, Y, X, 3).  Returns one
+    are per frame (or ``motion_init_cp`` (F, 3, Z, ncy, ncx): each frame's
+    coarsest-level control-point field, as in ``get_motion``); ``motion_out``
+    is (F, Z, Y, X, 3).  Returns one
     MotionResult per frame (its ``motion`` a view of ``motion_out``).  With
     ``warp_src`` (F, Z, Y, X) float32 or uint16 (e.g. ``movs``) is warped by
     each frame's field into ``warped_out`` (F, Z, Y, X) float32 in the kernel
@@ -279,12 +281,24 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
     if motion_init is not None and (tuple(motion_init.shape) != (nf, nz, ny, nx, 3)
                                     or motion_init.dtype != torch.float32 or not motion_init.is_contiguous()):
         raise ValueError("motion_init must be contiguous float32 (F, Z, Y, X, 3)")
+    init_stride = 3 * nz * ny * nx
+    if motion_init_cp is not None:
+        if motion_init is not None:
+            raise ValueError("give motion_init or motion_init_cp, not both")
+        ncx, ncy = cp_grid((nz, ny, nx), r, layer)
+        if (tuple(motion_init_cp.shape) != (nf, 3, nz, ncy, ncx) or motion_init_cp.dtype != torch.float32
+                or not motion_init_cp.is_contiguous()):
+            raise ValueError(f"motion_init_cp must be contiguous float32 {(nf, 3, nz, ncy, ncx)}")
+        init_stride = 3 * nz * ncy * ncx
     if motion_out is None:
         motion_out = torch.empty((nf, nz, ny, nx, 3), dtype=torch.float32, device=movs.device)
     elif tuple(motion_out.shape) != (nf, nz, ny, nx, 3) or not motion_out.is_contiguous():
         raise ValueError("motion_out must be contiguous float32 (F, Z, Y, X, 3)")
     prm, keep = _params(layer, iter, r, zRatio, movRange, smoothPenalty, tol, tile_patches_x, stop_kernels,
                         precision, reuse_template, timing, schedule)
+    if motion_init_cp is not None:
+        prm.init_cp = 1
+        motion_init = motion_init_cp
     if warp_src is not None:
         if (not _is_tensor(warp_src) or tuple(warp_src.shape) != (nf, nz, ny, nx) or not warp_src.is_contiguous()
                 or warp_src.dtype not in (torch.float32, torch.uint16)):
@@ -308,7 +322,7 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
     strm = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
     nvox = nz * ny * nx
     rc = lib.wr_get_motion_batch(ctx, _ptr(movs), nvox, _ptr(ref), dtype_code(movs), nx, ny, nz, nf,
-                                 _ptr(mask_ref), _ptr(mask_mov), nvox, _ptr(motion_init), 3 * nvox,
+                                 _ptr(mask_ref), _ptr(mask_mov), nvox, _ptr(motion_init), init_stride,
                                  ctypes.byref(prm), _ptr(motion_out), _ptr(warp_src),
                                  dtype_code(warp_src) if warp_src is not None else 0, nvox, _ptr(warped_out),
                                  logs, strm)

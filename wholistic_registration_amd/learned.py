@@ -347,6 +347,36 @@ class InitNet:
                                               int(level), PSCALE, _p(out), self._s()), "wr_net_output_cp")
         return out
 
+    def predict_cp_batch(self, mov_ls, ref_l, full_shape, r, level=0, out=None):
+        """``predict_cp`` of B frames in one forward pass of batch B:
+        ``mov_ls`` (B, D, H, W) level images of the frames, ``ref_l`` the
+        template's; returns (B, 3, Z, ncy, ncx).  Each frame's field equals
+        its own ``predict_cp`` (a convolution's sums do not depend on the
+        batch)."""
+        torch = _torch()
+        nb, d, h, w = mov_ls.shape
+        nz, ny, nx = full_shape
+        if d != nz or int(nx / 2 ** LEVEL) != w or int(ny / 2 ** LEVEL) != h:
+            raise ValueError(f"level images must be the full volume's level {LEVEL} (x, y / 2^{LEVEL}, same z)")
+        if tuple(ref_l.shape) != (d, h, w) or mov_ls.dtype != torch.float32 or ref_l.dtype != torch.float32:
+            raise ValueError("level images must be float32 volumes of one shape")
+        ncx, ncy = _dev.cp_grid(full_shape, r, level)
+        gd, gh, gw = self.grid()
+        dp, hp, wp = -(-d // gd) * gd, -(-h // gh) * gh, -(-w // gw) * gw
+        x = torch.empty((nb, dp, hp, wp, 8), dtype=torch.bfloat16, device=self.dev)
+        ref_c = ref_l.contiguous()
+        mov_c = mov_ls.contiguous()
+        for b in range(nb):
+            self._check(self.lib.wr_net_input_padded(self.ctx, _p(mov_c[b]), _p(ref_c), d, h, w, dp, hp, wp,
+                                                     _p(x[b]), self._s()), "wr_net_input_padded")
+        p = self.forward(x)
+        if out is None:
+            out = torch.empty((nb, 3, nz, ncy, ncx), dtype=torch.float32, device=self.dev)
+        for b in range(nb):
+            self._check(self.lib.wr_net_output_cp(self.ctx, _p(p[b]), 16, d, h, w, hp, wp, LEVEL, ncx, ncy, int(r),
+                                                  int(level), PSCALE, _p(out[b]), self._s()), "wr_net_output_cp")
+        return out
+
     # ---------------------------------------------------------------- storage
 
     def params(self):
@@ -454,3 +484,38 @@ def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothP
     elif fallback is not None:
         raise ValueError("fallback must be None or 'plain'")
     return res, info
+
+
+def register_learned_batch(movs, ref, net, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol, chunk=16,
+                           ref_level=None, motion_out=None, warp_src=None, warped_out=None, context=None,
+                           stream=None, **kw):
+    """``register_learned`` (control-point start, no fallback) of every frame
+    of ``movs`` (F, Z, Y, X) against ``ref``, ``chunk`` frames at a time: the
+    level images of a chunk, one network pass of batch ``chunk``, then one
+    batched launch sequence (``device.get_motion_batch``) warm-started by
+    each frame's control-point field.  Each frame's field equals
+    ``register_learned(movs[f], ...)``'s.  ``motion_out`` (F, Z, Y, X, 3) and,
+    with ``warp_src`` (F, Z, Y, X), ``warped_out`` are filled; returns the
+    per-frame ``MotionResult`` list."""
+    torch = _torch()
+    nf = int(movs.shape[0])
+    full = tuple(int(v) for v in movs.shape[1:])
+    if motion_out is None:
+        motion_out = torch.empty((nf,) + full + (3,), dtype=torch.float32, device=movs.device)
+    if warp_src is not None and warped_out is None:
+        warped_out = torch.empty((nf,) + full, dtype=torch.float32, device=movs.device)
+    out = []
+    with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(movs.device)):
+        r_l = ref_level if ref_level is not None else _dev.pyramid_level(ref, LEVEL, context=context, stream=stream)
+        for f0 in range(0, nf, chunk):
+            f1 = min(nf, f0 + chunk)
+            m_ls = torch.stack([_dev.pyramid_level(movs[f], LEVEL, context=context, stream=stream)
+                                for f in range(f0, f1)])
+            cps = net.predict_cp_batch(m_ls, r_l, full, r, level=layer)
+            out += _dev.get_motion_batch(movs[f0:f1], ref, layer=layer, iter=iter, r=r, zRatio=zRatio,
+                                         movRange=movRange, smoothPenalty=smoothPenalty, tol=tol,
+                                         motion_init_cp=cps, motion_out=motion_out[f0:f1],
+                                         warp_src=None if warp_src is None else warp_src[f0:f1],
+                                         warped_out=None if warp_src is None else warped_out[f0:f1],
+                                         context=context, stream=stream, **kw)
+    return out
