@@ -115,6 +115,12 @@ typedef struct wr_params {
     int maxdiff_gather;     /* 0 = automatic; 1 = the |delta motion| maximum
                                of a level's first update always runs in the
                                gather kernel, not the LDS-staged one (testing) */
+    void* init_event;       /* NULL, or a hipEvent_t recorded where motion_init
+                               is produced (e.g. on another stream): the
+                               iterations wait for it, the pyramid build on the
+                               context's side stream does not, so the level
+                               images are made while motion_init is still being
+                               computed.  No counterpart in the reference. */
 } wr_params;
 
 /* Per-level iteration log, filled on the host when wr_get_motion returns
@@ -438,6 +444,11 @@ int wr_net_input(wr_ctx* ctx, const float* mov, const float* ref, long long n, v
  * are those of the d * h * w template voxels. */
 int wr_net_input_padded(wr_ctx* ctx, const float* mov, const float* ref, int d, int h, int w, int dp, int hp, int wp,
                         void* out, void* stream);
+/* wr_net_input_padded of nb frames in one call: movs (nb, d, h, w) float32
+ * against the one template ref (its statistics once); out (nb, dp, hp, wp,
+ * 8) bfloat16.  No counterpart in the reference (a learned-start helper). */
+int wr_net_input_padded_batch(wr_ctx* ctx, const float* movs, int nb, const float* ref, int d, int h, int w, int dp,
+                              int hp, int wp, void* out, void* stream);
 /* field (n, 3) float32 = scale * p[:, 0:3] of p (n, cp). */
 int wr_net_output(wr_ctx* ctx, const void* p, int cp, long long n, double scale, float* field, void* stream);
 /* The network's field (p: nz planes of hs x ws voxels of cp channels, of
@@ -448,6 +459,12 @@ int wr_net_output(wr_ctx* ctx, const void* p, int cp, long long n, double scale,
  * wr_params.init_cp takes. */
 int wr_net_output_cp(wr_ctx* ctx, const void* p, int cp, int nz, int h, int w, int hs, int ws, int q, int ncx,
                      int ncy, int r, int level, double scale, float* cp_out, void* stream);
+/* wr_net_output_cp of nb frames in one call: p (nb, dp, hs, ws, cp) with dp
+ * >= nz stored planes per frame (planes past nz are padding), cp_out (nb,
+ * 3, nz, ncy, ncx).  No counterpart in the reference (a learned-start
+ * helper). */
+int wr_net_output_cp_batch(wr_ctx* ctx, const void* p, int nb, int cp, int nz, int dp, int h, int w, int hs, int ws,
+                           int q, int ncx, int ncy, int r, int level, double scale, float* cp_out, void* stream);
 
 /* ------------------------------------------------------------------------
  * Cross-resolution registration (getMotion_v2,

@@ -124,6 +124,8 @@ def test_smooth_penalty_null_arguments_are_rejected():
     ("wr_net_output", (None, None, 16, 10, 4.0, None, None)),
     ("wr_net_output_cp", (None, None, 16, 4, 8, 32, 8, 32, 3, 4, 4, 8, 0, 4.0, None, None)),
     ("wr_net_input_padded", (None, None, None, 4, 8, 32, 4, 8, 32, None, None)),
+    ("wr_net_input_padded_batch", (None, None, 2, None, 4, 8, 32, 4, 8, 32, None, None)),
+    ("wr_net_output_cp_batch", (None, None, 2, 16, 4, 4, 8, 32, 8, 32, 3, 4, 4, 8, 0, 4.0, None, None)),
 ])
 def test_network_entry_points_reject_null_arguments(name, args):
     """The learned initialisation's entry points validate before touching a

@@ -42,6 +42,7 @@ class WrParams(ctypes.Structure):
         ("init_cp", ctypes.c_int),
         ("resample_start", ctypes.c_int),
         ("maxdiff_gather", ctypes.c_int),
+        ("init_event", ctypes.c_void_p),
     ]
 
 
@@ -150,6 +151,9 @@ _SIGNATURES = {
     "wr_net_input_padded": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_net_output": ([_VP, _VP, _I, ctypes.c_longlong, ctypes.c_double, _VP, _VP], _I),
     "wr_net_output_cp": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double, _VP, _VP], _I),
+    "wr_net_input_padded_batch": ([_VP, _VP, _I, _VP, _I, _I, _I, _I, _I, _I, _VP, _VP], _I),
+    "wr_net_output_cp_batch": ([_VP, _VP, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, ctypes.c_double,
+                                _VP, _VP], _I),
     "wr_spline_coef": ([_VP, _VP, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_spline_sample": ([_VP, _VP, _I, _I, _I, _VP, _LL, _D, _VP, _VP], _I),
     "wr_xres_optimize": ([_VP, ctypes.POINTER(WrXresLayer), _VP, _VP, _VP, _VP, _VP, _VP, _VP,
@@ -216,14 +220,16 @@ def check(rc, what):
     raise RuntimeError(msg)
 
 
-def context(device=0):
-    """The library context of one HIP device (created on first use)."""
+def context(device=0, slot=0):
+    """The library context of one HIP device (created on first use).  Slot 1
+    is a second context of the device for work enqueued on another stream at
+    the same time as slot 0's (each context has its own workspace)."""
     lib = load()
     with _lock:
-        h = _ctx.get(device)
+        h = _ctx.get((device, slot))
         if h is None:
             h = lib.wr_create(device)
             if not h:
                 raise RuntimeError(f"wr_create({device}): {last_error()}")
-            _ctx[device] = h
+            _ctx[(device, slot)] = h
         return h

@@ -612,6 +612,9 @@ static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int 
     };
     auto starts_at_cp = [&](int L) { return L == layer && motion_init && prm->init_cp; };
 
+    // the start field may come from another stream: the iterations wait for
+    // it, the side stream's level images (forked above) do not
+    if (prm->init_event) WR_HIP_CHECK(hipStreamWaitEvent(s, static_cast<hipEvent_t>(prm->init_event), 0));
     for (int L = layer; L >= 0; --L) {
         const LevelGeom& g = geo[L];
         const LkTile& t = tiles[L];
@@ -1298,6 +1301,39 @@ int wr_net_input_padded(wr_ctx* ctx, const float* mov, const float* ref, int d, 
     WR_TRY(harness_small(c, &part, &ms));
     launch_mean_std(ref, static_cast<long long>(d) * h * w, part, ms, s);
     launch_net_input_padded(mov, ref, d, h, w, dp, hp, wp, ms, static_cast<uint16_t*>(out), s);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_input_padded_batch(wr_ctx* ctx, const float* movs, int nb, const float* ref, int d, int h, int w, int dp,
+                              int hp, int wp, void* out, void* stream) {
+    WR_ARG(ctx && movs && ref && out, "wr_net_input_padded_batch: null argument");
+    WR_ARG(nb > 0 && nb <= 65535, "wr_net_input_padded_batch: frame count out of range");
+    WR_ARG(d > 0 && h > 0 && w > 0 && dp >= d && hp >= h && wp >= w, "wr_net_input_padded_batch: bad sizes");
+    WR_ARG(al16(out), "wr_net_input_padded_batch: out must be 16-byte aligned");
+    Context& c = ctx->c;
+    WR_HIP_CHECK(hipSetDevice(c.device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* part;
+    float* ms;
+    WR_TRY(harness_small(c, &part, &ms));
+    launch_mean_std(ref, static_cast<long long>(d) * h * w, part, ms, s);
+    launch_net_input_padded(movs, ref, d, h, w, dp, hp, wp, ms, static_cast<uint16_t*>(out), s, nb);
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
+int wr_net_output_cp_batch(wr_ctx* ctx, const void* p, int nb, int cp, int nz, int dp, int h, int w, int hs, int ws,
+                           int q, int ncx, int ncy, int r, int level, double scale, float* cp_out, void* stream) {
+    WR_ARG(ctx && p && cp_out, "wr_net_output_cp_batch: null argument");
+    WR_ARG(nb > 0 && nb <= 65535, "wr_net_output_cp_batch: frame count out of range");
+    WR_ARG(cp >= 3 && nz > 0 && dp >= nz && h > 0 && w > 0 && hs >= h && ws >= w && ncx > 0 && ncy > 0 && r >= 0,
+           "wr_net_output_cp_batch: bad sizes");
+    WR_ARG(q >= 0 && q < 16 && level >= 0 && level < WR_MAX_LEVELS, "wr_net_output_cp_batch: bad level");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    launch_net_output_cp(static_cast<const uint16_t*>(p), cp, h, w, hs, ws, q, nz, ncx, ncy, r, level,
+                         static_cast<float>(scale), cp_out, static_cast<hipStream_t>(stream), nb,
+                         static_cast<long long>(dp) * hs * ws * cp);
     WR_HIP_CHECK(hipGetLastError());
     return WR_OK;
 }

@@ -774,37 +774,96 @@ __global__ __launch_bounds__(256) void k_make_records(PVol v, unsigned char* __r
 }
 
 // Records of a level that equals the input (level 0) straight from the
-// dense uint16 / float32 input, indices clamped instead of padded: four
-// consecutive records per thread, rows (y, z) of nx + 1 records.
+// dense uint16 / float32 input, indices clamped instead of padded.  A thread
+// makes the 8 consecutive records x .. x + 7 of one row (y, z) of nx + 1
+// records; the last thread of a row also the repeated record nx (or, where
+// nx is not a multiple of 8, the records x .. nx).  The threads of a row are
+// consecutive lanes, so the x neighbours x - 1 and x + 8 come from the
+// adjacent lanes' loads (loaded only at a wave's edges); each row of eight
+// inputs is one 16-byte load (two for float32) and each pair of 16-bit
+// records one 16-byte store.  bfloat16 pairs are rounded by the packed
+// conversion instruction (round to nearest even, as to_bf16 for finite
+// values).
 template <int IN_T>
 __device__ __forceinline__ float in_at(const void* in, size_t i) {
     if constexpr (IN_T == E_U16) return static_cast<float>(static_cast<const uint16_t*>(in)[i]);
     else return static_cast<const float*>(in)[i];
 }
 template <int IN_T>
-__device__ __forceinline__ void in_row4(const void* in, size_t i, float o[4]) {
+__device__ __forceinline__ void in_row8(const void* in, size_t i, float o[8]) {
     if constexpr (IN_T == E_U16) {
-        const u2v w = *reinterpret_cast<const u2v*>(static_cast<const uint16_t*>(in) + i);
-        o[0] = static_cast<float>(w.x & 0xffffu);
-        o[1] = static_cast<float>(w.x >> 16);
-        o[2] = static_cast<float>(w.y & 0xffffu);
-        o[3] = static_cast<float>(w.y >> 16);
+        const u4v w = *reinterpret_cast<const u4v*>(static_cast<const uint16_t*>(in) + i);
+        const unsigned q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            o[2 * k] = static_cast<float>(q[k] & 0xffffu);
+            o[2 * k + 1] = static_cast<float>(q[k] >> 16);
+        }
     } else {
-        const f4v w = *reinterpret_cast<const f4v*>(static_cast<const float*>(in) + i);
-        o[0] = w.x;
-        o[1] = w.y;
-        o[2] = w.z;
-        o[3] = w.w;
+        const f4v a = *reinterpret_cast<const f4v*>(static_cast<const float*>(in) + i);
+        const f4v b = *reinterpret_cast<const f4v*>(static_cast<const float*>(in) + i + 4);
+        o[0] = a.x;
+        o[1] = a.y;
+        o[2] = a.z;
+        o[3] = a.w;
+        o[4] = b.x;
+        o[5] = b.y;
+        o[6] = b.z;
+        o[7] = b.w;
+    }
+}
+// Inputs x .. x + 7 of a row, clamped to the row's last voxel.
+template <int IN_T>
+__device__ __forceinline__ void in_row8_clamped(const void* in, size_t r, int x, int nx, bool full, float o[8]) {
+    if (full) {
+        in_row8<IN_T>(in, r + x, o);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = in_at<IN_T>(in, r + min(x + k, nx - 1));
+    }
+}
+
+// Two 16-bit values of storage S in one word (lo in the low half).
+template <int S>
+__device__ __forceinline__ unsigned pack2(float lo, float hi) {
+    if constexpr (S == ST_BF16) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+        return __builtin_bit_cast(unsigned, __builtin_convertvector(f2{lo, hi}, b2));
+    } else {
+        return to_f16(lo) | (static_cast<unsigned>(to_f16(hi)) << 16);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void put_record(unsigned char* o, float q0, float q1, float q2, float q3) {
+    if constexpr (S == ST_F32) {
+        f4a r;
+        r.x = q0;
+        r.y = q1;
+        r.z = q2;
+        r.w = q3;
+        *reinterpret_cast<f4a*>(o) = r;
+    } else {
+        u2v r;
+        r.x = pack2<S>(q0, q1);
+        r.y = pack2<S>(q2, q3);
+        *reinterpret_cast<u2v*>(o) = r;
     }
 }
 
 template <int IN_T, int S>
 __global__ __launch_bounds__(256) void k_make_records_in(const void* __restrict__ in, int nx, int ny, int nz,
-                                                         unsigned char* __restrict__ out, unsigned rowB, int xblocks) {
-    const unsigned b = blockIdx.x;
-    const unsigned row = b / static_cast<unsigned>(xblocks);
-    const int x = (static_cast<int>(b - row * static_cast<unsigned>(xblocks)) * 256 + static_cast<int>(threadIdx.x)) * 4;
-    if (x > nx) return;
+                                                         unsigned char* __restrict__ out, unsigned rowB, unsigned tpr,
+                                                         unsigned items) {
+    const unsigned item = blockIdx.x * 256u + threadIdx.x;
+    const bool live = item < items;
+    // lanes past the end work on the last item (no stores) so that every lane
+    // takes part in the shuffles
+    const unsigned it = live ? item : items - 1u;
+    const unsigned row = it / tpr;
+    const unsigned t = it - row * tpr;
+    const int x = static_cast<int>(t) * 8;
     const int y = static_cast<int>(row % static_cast<unsigned>(ny + 1));
     const int z = static_cast<int>(row / static_cast<unsigned>(ny + 1));
     const int yc = min(y, ny - 1), zc = min(z, nz - 1);
@@ -812,49 +871,52 @@ __global__ __launch_bounds__(256) void k_make_records_in(const void* __restrict_
     const size_t r0 = (static_cast<size_t>(zc) * ny + yc) * nx;
     const size_t rym = r0 - (yc > 0 ? nx : 0), ryp = r0 + (yc < ny - 1 ? nx : 0);
     const size_t rzm = r0 - (zc > 0 ? plane : 0), rzp = r0 + (zc < nz - 1 ? plane : 0);
-    float xv[6], ym[4], yp[4], zm[4], zp[4];
-    if (x >= 4 && x + 4 <= nx - 1) {   // interior of the row: vector loads
-        xv[0] = in_at<IN_T>(in, r0 + x - 1);
-        in_row4<IN_T>(in, r0 + x, xv + 1);
-        xv[5] = in_at<IN_T>(in, r0 + x + 4);
-        in_row4<IN_T>(in, rym + x, ym);
-        in_row4<IN_T>(in, ryp + x, yp);
-        in_row4<IN_T>(in, rzm + x, zm);
-        in_row4<IN_T>(in, rzp + x, zp);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 6; ++i) xv[i] = in_at<IN_T>(in, r0 + clampi(x - 1 + i, 0, nx - 1));
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int xc = min(x + i, nx - 1);
-            ym[i] = in_at<IN_T>(in, rym + xc);
-            yp[i] = in_at<IN_T>(in, ryp + xc);
-            zm[i] = in_at<IN_T>(in, rzm + xc);
-            zp[i] = in_at<IN_T>(in, rzp + xc);
-        }
-    }
+    const bool full = x + 8 <= nx;   // all eight inputs inside the row
+    float xv[10], ym[8], yp[8], zm[8], zp[8];
+    in_row8_clamped<IN_T>(in, r0, x, nx, full, xv + 1);
+    in_row8_clamped<IN_T>(in, rym, x, nx, full, ym);
+    in_row8_clamped<IN_T>(in, ryp, x, nx, full, yp);
+    in_row8_clamped<IN_T>(in, rzm, x, nx, full, zm);
+    in_row8_clamped<IN_T>(in, rzp, x, nx, full, zp);
+    // x - 1 and x + 8 (clamped): the neighbouring lanes' end values where
+    // they hold the same row, else loaded
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const float from_left = __shfl_up(xv[8], 1, 64);
+    const float from_right = __shfl_down(xv[1], 1, 64);
+    if (x == 0) xv[0] = xv[1];
+    else if (lane > 0) xv[0] = from_left;
+    else xv[0] = in_at<IN_T>(in, r0 + x - 1);
+    if (x + 8 > nx - 1) xv[9] = xv[8];   // inputs past the row end repeat its last voxel
+    else if (lane < 63) xv[9] = from_right;
+    else xv[9] = in_at<IN_T>(in, r0 + x + 8);
+    if (!live) return;
     unsigned char* o = out + static_cast<size_t>(row) * rowB + static_cast<size_t>(x) * kRec<S>;
+    if (S != ST_F32 && full) {
+        // eight records of voxels inside the row: four 16-byte stores
+        u4v w[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (x + i > nx) break;
-        // the record past the last voxel repeats the last one
-        const int j = x + i < nx ? i : i - 1;
-        const float q0 = xv[j + 1], q1 = xv[j + 2] - xv[j], q2 = yp[j] - ym[j], q3 = zp[j] - zm[j];
-        if constexpr (S == ST_F32) {
-            f4a r;
-            r.x = q0;
-            r.y = q1;
-            r.z = q2;
-            r.w = q3;
-            reinterpret_cast<f4a*>(o)[i] = r;
-        } else {
-            auto h = [](float f) -> unsigned { return S == ST_BF16 ? to_bf16(f) : to_f16(f); };
-            u2v r;
-            r.x = h(q0) | (h(q1) << 16);
-            r.y = h(q2) | (h(q3) << 16);
-            reinterpret_cast<u2v*>(o)[i] = r;
+        for (int q = 0; q < 4; ++q) {
+            const int a = 2 * q, b = 2 * q + 1;
+            w[q].x = pack2<S>(xv[a + 1], xv[a + 2] - xv[a]);
+            w[q].y = pack2<S>(yp[a] - ym[a], zp[a] - zm[a]);
+            w[q].z = pack2<S>(xv[b + 1], xv[b + 2] - xv[b]);
+            w[q].w = pack2<S>(yp[b] - ym[b], zp[b] - zm[b]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) reinterpret_cast<u4v*>(o)[q] = w[q];
+    } else {
+        // records x .. min(x + 7, nx); the one past the last voxel repeats it
+        const int last = min(x + 8, nx + 1) - x;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (i >= last) break;
+            const int j = x + i < nx ? i : i - 1;
+            put_record<S>(o + static_cast<size_t>(i) * kRec<S>, xv[j + 1], xv[j + 2] - xv[j], yp[j] - ym[j],
+                          zp[j] - zm[j]);
         }
     }
+    if (x + 8 == nx)   // the row's last thread of a multiple-of-8 row: record nx repeats nx - 1
+        put_record<S>(o + static_cast<size_t>(8) * kRec<S>, xv[8], xv[9] - xv[7], yp[7] - ym[7], zp[7] - zm[7]);
 }
 
 QVol make_qvol(const void* base, int store, int nx, int ny, int nz) {
@@ -888,14 +950,17 @@ void launch_moving_records(const float* mov_padded, int nx, int ny, int nz, int 
 void launch_moving_records_input(const void* in, Elem t, int nx, int ny, int nz, int store, void* out,
                                  hipStream_t s) {
     const QVol q = make_qvol(out, store, nx, ny, nz);
-    const int xblocks = (nx + 1 + 1023) / 1024;   // 4 records per thread
-    const dim3 grid(static_cast<unsigned>(static_cast<long long>(ny + 1) * (nz + 1) * xblocks));
+    // 8 records per thread; rows (y, z) of nx + 1 records, y <= ny, z <= nz
+    const unsigned tpr = static_cast<unsigned>((nx + 7) / 8);
+    const unsigned items = static_cast<unsigned>(static_cast<long long>(ny + 1) * (nz + 1) * tpr);
+    const dim3 grid((items + 255u) / 256u);
     unsigned char* o = static_cast<unsigned char*>(out);
-    using Fn = void (*)(const void*, int, int, int, unsigned char*, unsigned, int);
+    using Fn = void (*)(const void*, int, int, int, unsigned char*, unsigned, unsigned, unsigned);
     static const Fn table[2][3] = {
         {k_make_records_in<E_F32, ST_F32>, k_make_records_in<E_F32, ST_BF16>, k_make_records_in<E_F32, ST_F16>},
         {k_make_records_in<E_U16, ST_F32>, k_make_records_in<E_U16, ST_BF16>, k_make_records_in<E_U16, ST_F16>}};
-    hipLaunchKernelGGL(table[t == E_U16 ? 1 : 0][store], grid, dim3(256), 0, s, in, nx, ny, nz, o, q.rowB, xblocks);
+    hipLaunchKernelGGL(table[t == E_U16 ? 1 : 0][store], grid, dim3(256), 0, s, in, nx, ny, nz, o, q.rowB, tpr,
+                       items);
 }
 
 long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool dense,

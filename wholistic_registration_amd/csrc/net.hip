@@ -423,12 +423,15 @@ __global__ void k_net_input(const float* __restrict__ mov, const float* __restri
 
 // The same on a padded grid (dp, hp, wp) >= (d, h, w): voxels past the
 // volume repeat its nearest edge voxel (a thread per padded voxel).
+// Frame blockIdx.y of a batch: mov + y * d*h*w, out + y * dp*hp*wp records.
 __global__ void k_net_input_padded(const float* __restrict__ mov, const float* __restrict__ ref, int d, int h,
                                    int w, int dp, int hp, int wp, const float* __restrict__ ms,
                                    uint16_t* __restrict__ out) {
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     const long long n = static_cast<long long>(dp) * hp * wp;
     if (i >= n) return;
+    mov += static_cast<size_t>(blockIdx.y) * d * h * w;
+    out += static_cast<size_t>(blockIdx.y) * n * 8;
     const int x = min(static_cast<int>(i % wp), w - 1);
     const int y = min(static_cast<int>((i / wp) % hp), h - 1);
     const int z = min(static_cast<int>(i / (static_cast<long long>(wp) * hp)), d - 1);
@@ -461,10 +464,13 @@ __global__ void k_net_output(const uint16_t* __restrict__ P, int cp, long long n
 // planes of nz * ncy * ncx.  A thread per control point.
 __global__ void k_net_output_cp(const uint16_t* __restrict__ P, int cp, int h3, int w3, int hs, int ws, float inv_q,
                                 int nz, int ncx, int ncy, int r, int pstep, float lscale, float scale,
-                                float* __restrict__ out) {
+                                float* __restrict__ out, long long pstride) {
     const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
     const long long ncp = static_cast<long long>(ncx) * ncy * nz;
     if (i >= ncp) return;
+    // frame blockIdx.y of a batch (pstride elements of P per frame)
+    P += static_cast<size_t>(blockIdx.y) * pstride;
+    out += static_cast<size_t>(blockIdx.y) * 3 * ncp;
     const int ix = static_cast<int>(i % ncx);
     const int iy = static_cast<int>((i / ncx) % ncy);
     const int z = static_cast<int>(i / (static_cast<long long>(ncx) * ncy));
@@ -572,17 +578,18 @@ void launch_net_input(const float* mov, const float* ref, long long n, const flo
 }
 
 void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int hs, int ws, int q, int nz, int ncx, int ncy,
-                          int r, int level, float scale, float* out, hipStream_t s) {
+                          int r, int level, float scale, float* out, hipStream_t s, int nb, long long pstride) {
     const long long ncp = static_cast<long long>(ncx) * ncy * nz;
-    hipLaunchKernelGGL(k_net_output_cp, dim3(blocks_of(ncp)), dim3(256), 0, s, p, cp, h3, w3, hs, ws,
+    hipLaunchKernelGGL(k_net_output_cp, dim3(blocks_of(ncp), nb), dim3(256), 0, s, p, cp, h3, w3, hs, ws,
                        1.0f / static_cast<float>(1 << q), nz, ncx, ncy, r, 2 * r + 1,
-                       static_cast<float>(1 << level), scale, out);
+                       static_cast<float>(1 << level), scale, out, pstride);
 }
 
 void launch_net_input_padded(const float* mov, const float* ref, int d, int h, int w, int dp, int hp, int wp,
-                             const float* ms, uint16_t* out, hipStream_t s) {
+                             const float* ms, uint16_t* out, hipStream_t s, int nb) {
     const long long n = static_cast<long long>(dp) * hp * wp;
-    hipLaunchKernelGGL(k_net_input_padded, dim3(blocks_of(n)), dim3(256), 0, s, mov, ref, d, h, w, dp, hp, wp, ms, out);
+    hipLaunchKernelGGL(k_net_input_padded, dim3(blocks_of(n), nb), dim3(256), 0, s, mov, ref, d, h, w, dp, hp, wp, ms,
+                       out);
 }
 
 void launch_net_output(const uint16_t* p, int cp, long long n, float scale, float* field, hipStream_t s) {

@@ -488,9 +488,10 @@ void launch_to_bf16(const float* in, long long n, uint16_t* out, hipStream_t s);
 // (n, 8) bfloat16: (mov - ms[0]) / ms[1], (ref - ms[0]) / ms[1], zeros.
 void launch_net_input(const float* mov, const float* ref, long long n, const float* ms, uint16_t* out,
                       hipStream_t s);
-// The same input on a padded (dp, hp, wp) grid, edge voxels repeated.
+// The same input on a padded (dp, hp, wp) grid, edge voxels repeated; nb
+// frames of mov (frame stride d*h*w) into nb consecutive padded grids.
 void launch_net_input_padded(const float* mov, const float* ref, int d, int h, int w, int dp, int hp, int wp,
-                             const float* ms, uint16_t* out, hipStream_t s);
+                             const float* ms, uint16_t* out, hipStream_t s, int nb = 1);
 // field (n, 3) float32 = scale * p[:, 0:3] of p (n, cp) bfloat16.
 void launch_net_output(const uint16_t* p, int cp, long long n, float scale, float* field, hipStream_t s);
 // The field of p (nz, h3, w3, cp) (level q of the full volume; stored on an
@@ -498,7 +499,8 @@ void launch_net_output(const uint16_t* p, int cp, long long n, float scale, floa
 // points (ncx x ncy per plane, half-width r) of pyramid level `level`, in that
 // level's units: out 3 planes of nz * ncy * ncx float32.
 void launch_net_output_cp(const uint16_t* p, int cp, int h3, int w3, int hs, int ws, int q, int nz, int ncx, int ncy,
-                          int r, int level, float scale, float* out, hipStream_t s);
+                          int r, int level, float scale, float* out, hipStream_t s,
+                          int nb = 1, long long pstride = 0);
 
 // ---------------------------------------------------------------- context
 

@@ -248,7 +248,8 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
 def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol,
                      mask_ref=None, mask_mov=None, motion_init=None, motion_out=None, tile_patches_x=0,
                      timing=False, stop_kernels=False, precision="fp32", reuse_template=False, context=None,
-                     stream=None, schedule=None, logs=True, warp_src=None, warped_out=None, motion_init_cp=None):
+                     stream=None, schedule=None, logs=True, warp_src=None, warped_out=None, motion_init_cp=None,
+                     init_event=None):
     """getMotion of every frame of ``movs`` (F, Z, Y, X) against ``ref`` in one
     launch sequence (wr_get_motion_batch): one launch per kernel and
     level-iteration over all frames, each frame with its own stop rules.
@@ -264,7 +265,10 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
     ``logs=False`` the call only enqueues the work on ``stream`` and returns
     ``motion_out`` without waiting: repeated with the same tensors after a
     first call, it allocates nothing and can be captured in a CUDA/HIP graph
-    (``torch.cuda.graph``)."""
+    (``torch.cuda.graph``).  ``init_event`` (a ``torch.cuda.Event``) marks
+    where ``motion_init`` / ``motion_init_cp`` is written on another stream:
+    the iterations wait for it, the level images are built without waiting
+    (``wr_params.init_event``)."""
     torch = _torch()
     if not _is_tensor(movs) or movs.dim() != 4 or not movs.is_cuda or not movs.is_contiguous():
         raise ValueError("movs must be a contiguous (F, Z, Y, X) device tensor")
@@ -299,6 +303,8 @@ def get_motion_batch(movs, ref, *, layer, iter, r, zRatio, movRange, smoothPenal
     if motion_init_cp is not None:
         prm.init_cp = 1
         motion_init = motion_init_cp
+    if init_event is not None:
+        prm.init_event = ctypes.c_void_p(init_event.cuda_event)
     if warp_src is not None:
         if (not _is_tensor(warp_src) or tuple(warp_src.shape) != (nf, nz, ny, nx) or not warp_src.is_contiguous()
                 or warp_src.dtype not in (torch.float32, torch.uint16)):

@@ -110,7 +110,9 @@ class InitNet:
             raise RuntimeError("wholistic_registration_amd needs a HIP device (none is visible)")
         self.dev = torch.device("cuda", device)
         self.lib = _lib.load()
-        self.ctx = _lib.context(device)
+        # a context of its own: the batched start runs the network on a second
+        # stream while the registration's context builds the level images
+        self.ctx = _lib.context(device, slot=1)
         if params is None:
             params = init_params(seed, channels)
         self.channels = channels_of(params)
@@ -366,15 +368,16 @@ class InitNet:
         x = torch.empty((nb, dp, hp, wp, 8), dtype=torch.bfloat16, device=self.dev)
         ref_c = ref_l.contiguous()
         mov_c = mov_ls.contiguous()
-        for b in range(nb):
-            self._check(self.lib.wr_net_input_padded(self.ctx, _p(mov_c[b]), _p(ref_c), d, h, w, dp, hp, wp,
-                                                     _p(x[b]), self._s()), "wr_net_input_padded")
+        self._check(self.lib.wr_net_input_padded_batch(self.ctx, _p(mov_c), nb, _p(ref_c), d, h, w, dp, hp, wp,
+                                                       _p(x), self._s()), "wr_net_input_padded_batch")
         p = self.forward(x)
         if out is None:
             out = torch.empty((nb, 3, nz, ncy, ncx), dtype=torch.float32, device=self.dev)
-        for b in range(nb):
-            self._check(self.lib.wr_net_output_cp(self.ctx, _p(p[b]), 16, d, h, w, hp, wp, LEVEL, ncx, ncy, int(r),
-                                                  int(level), PSCALE, _p(out[b]), self._s()), "wr_net_output_cp")
+        elif not out.is_contiguous() or tuple(out.shape) != (nb, 3, nz, ncy, ncx):
+            raise ValueError("out must be a contiguous (B, 3, Z, ncy, ncx) float32 tensor")
+        self._check(self.lib.wr_net_output_cp_batch(self.ctx, _p(p), nb, 16, d, dp, h, w, hp, wp, LEVEL, ncx, ncy,
+                                                    int(r), int(level), PSCALE, _p(out), self._s()),
+                    "wr_net_output_cp_batch")
         return out
 
     # ---------------------------------------------------------------- storage
@@ -496,7 +499,12 @@ def register_learned_batch(movs, ref, net, *, layer, iter, r, zRatio, movRange, 
     each frame's control-point field.  Each frame's field equals
     ``register_learned(movs[f], ...)``'s.  ``motion_out`` (F, Z, Y, X, 3) and,
     with ``warp_src`` (F, Z, Y, X), ``warped_out`` are filled; returns the
-    per-frame ``MotionResult`` list."""
+    per-frame ``MotionResult`` list.
+
+    Per chunk, the level images and the network run on a second stream
+    (with the network's own library context) while the registration's
+    stream builds the level-0 images of the chunk; the iterations wait for
+    the network through ``init_event``."""
     torch = _torch()
     nf = int(movs.shape[0])
     full = tuple(int(v) for v in movs.shape[1:])
@@ -505,17 +513,39 @@ def register_learned_batch(movs, ref, net, *, layer, iter, r, zRatio, movRange, 
     if warp_src is not None and warped_out is None:
         warped_out = torch.empty((nf,) + full, dtype=torch.float32, device=movs.device)
     out = []
-    with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(movs.device)):
-        r_l = ref_level if ref_level is not None else _dev.pyramid_level(ref, LEVEL, context=context, stream=stream)
+    main = stream if stream is not None else torch.cuda.current_stream(movs.device)
+    side = _net_stream(movs.device)
+    with torch.cuda.stream(main):
+        r_l = ref_level if ref_level is not None else _dev.pyramid_level(ref, LEVEL, context=context)
         for f0 in range(0, nf, chunk):
             f1 = min(nf, f0 + chunk)
-            m_ls = torch.stack([_dev.pyramid_level(movs[f], LEVEL, context=context, stream=stream)
-                                for f in range(f0, f1)])
-            cps = net.predict_cp_batch(m_ls, r_l, full, r, level=layer)
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                # the level images of the chunk in one call: the resampling is
+                # x-y only, so F frames of Z planes are one volume of F*Z planes
+                chunk_vol = movs[f0:f1].reshape((f1 - f0) * full[0], full[1], full[2])
+                m_ls = _dev.pyramid_level(chunk_vol, LEVEL, context=net.ctx)
+                m_ls = m_ls.view(f1 - f0, full[0], m_ls.shape[1], m_ls.shape[2])
+                cps = net.predict_cp_batch(m_ls, r_l, full, r, level=layer)
+                ready = torch.cuda.Event()
+                ready.record(side)
+            cps.record_stream(main)
             out += _dev.get_motion_batch(movs[f0:f1], ref, layer=layer, iter=iter, r=r, zRatio=zRatio,
                                          movRange=movRange, smoothPenalty=smoothPenalty, tol=tol,
-                                         motion_init_cp=cps, motion_out=motion_out[f0:f1],
+                                         motion_init_cp=cps, init_event=ready, motion_out=motion_out[f0:f1],
                                          warp_src=None if warp_src is None else warp_src[f0:f1],
                                          warped_out=None if warp_src is None else warped_out[f0:f1],
-                                         context=context, stream=stream, **kw)
+                                         context=context, stream=main, **kw)
     return out
+
+
+_NET_STREAMS = {}
+
+
+def _net_stream(device):
+    """The stream the batched start runs its network on (default priority)."""
+    torch = _torch()
+    key = device.index or 0
+    if key not in _NET_STREAMS:
+        _NET_STREAMS[key] = torch.cuda.Stream(device=device)
+    return _NET_STREAMS[key]
