@@ -71,8 +71,11 @@ __device__ __forceinline__ unsigned tap_mask(int d, int h, int w, const NetGeom&
 }
 
 // NT MFMA tiles of 16 voxels per wave (4 on large volumes, 1 on small ones
-// for 4x the waves).
-template <int NT>
+// for 4x the waves) times CT tiles of 16 output channels: each gathered
+// input fragment feeds CT matrix operations (the gathers bound the kernel).
+// Every output tile receives the same operations in the same order whatever
+// NT and CT, so the results do not depend on them.
+template <int NT, int CT>
 __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X, const uint16_t* __restrict__ Wt,
                                                   const float* __restrict__ bias, uint16_t* __restrict__ Y,
                                                   NetGeom g, int cin, int cout, int act, float slope) {
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
     __syncthreads();
     const int l = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long nv = static_cast<long long>(g.d) * g.h * g.w;
-    const int co0 = blockIdx.y * 16;
+    const int co0 = blockIdx.y * 16 * CT;
     // workgroups go round-robin to the 8 XCDs; give each XCD a contiguous run
     // of voxel chunks so the neighbour planes its taps share stay in its L2
     // (gridDim.x is a multiple of 8 on the network's volumes, so the XCD of a
@@ -108,10 +111,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
         vo[j] = static_cast<int>(vv) * cin;
         vm[j] = ok ? tap_mask(d, h, w, g) : 0u;
     }
-    f4 acc[NT];
+    f4 acc[CT][NT];
 #pragma unroll
-    for (int j = 0; j < NT; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[c][j] = f4{0.f, 0.f, 0.f, 0.f};
     const uint16_t* wrow = Wt + static_cast<size_t>(co0 + (l & 15)) * ksz + 8 * (l >> 4);
+    const size_t wct = static_cast<size_t>(16) * ksz;   // next tile of 16 output channels
     // this lane's reduction index k0 + 8 (l / 16) = tap * cin + ci, advanced by 32 per step;
     // two steps per trip so that the loads of both are in flight together
     int tap = (8 * (l >> 4)) / cin;
@@ -123,8 +129,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
             ++tap1;
         }
         const bool kin0 = tap < 27, kin1 = tap1 < 27;
-        const bf16x8 a0 = kin0 ? load8(wrow + k0) : zero8();
-        const bf16x8 a1 = kin1 ? load8(wrow + k0 + 32) : zero8();
+        bf16x8 a0[CT], a1[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            a0[c] = kin0 ? load8(wrow + c * wct + k0) : zero8();
+            a1[c] = kin1 ? load8(wrow + c * wct + k0 + 32) : zero8();
+        }
         const int off0 = (kin0 ? toff[tap] : 0) + ci;
         const int off1 = (kin1 ? toff[tap1] : 0) + ci1;
         bf16x8 b0[NT], b1[NT];
@@ -136,9 +146,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
             b1[j] = in1 ? load8(xb + vo[j] + off1) : zero8();
         }
 #pragma unroll
-        for (int j = 0; j < NT; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0[j], acc[j], 0, 0, 0);
+        for (int c = 0; c < CT; ++c)
 #pragma unroll
-        for (int j = 0; j < NT; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[j], acc[j], 0, 0, 0);
+            for (int j = 0; j < NT; ++j)
+                acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[c], b0[j], acc[c][j], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[c], b1[j], acc[c][j], 0, 0, 0);
         tap = tap1;
         ci = ci1 + 32;
         while (ci >= cin) {
@@ -146,25 +162,28 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const uint16_t* __restrict__ X
             ++tap;
         }
     }
-    const int m0 = co0 + 4 * (l >> 4);
-    float bs[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) bs[i] = bias ? bias[m0 + i] : 0.0f;
+    for (int c = 0; c < CT; ++c) {
+        const int m0 = co0 + 16 * c + 4 * (l >> 4);
+        float bs[4];
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-        const long long v = vb + 16 * j + (l & 15);
-        if (v >= nv) continue;
-        uint16_t o[4];
+        for (int i = 0; i < 4; ++i) bs[i] = bias ? bias[m0 + i] : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float y = acc[j][i] + bs[i];
-            if (act && y < 0.0f) y *= slope;
-            o[i] = f2bf(y);
+        for (int j = 0; j < NT; ++j) {
+            const long long v = vb + 16 * j + (l & 15);
+            if (v >= nv) continue;
+            uint16_t o[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float y = acc[c][j][i] + bs[i];
+                if (act && y < 0.0f) y *= slope;
+                o[i] = f2bf(y);
+            }
+            uint2 pk;
+            pk.x = static_cast<unsigned>(o[0]) | (static_cast<unsigned>(o[1]) << 16);
+            pk.y = static_cast<unsigned>(o[2]) | (static_cast<unsigned>(o[3]) << 16);
+            *reinterpret_cast<uint2*>(Y + yb + static_cast<size_t>(v) * cout + m0) = pk;
         }
-        uint2 pk;
-        pk.x = static_cast<unsigned>(o[0]) | (static_cast<unsigned>(o[1]) << 16);
-        pk.y = static_cast<unsigned>(o[2]) | (static_cast<unsigned>(o[3]) << 16);
-        *reinterpret_cast<uint2*>(Y + yb + static_cast<size_t>(v) * cout + m0) = pk;
     }
 }
 
@@ -503,15 +522,26 @@ inline unsigned blocks_of(long long n, int per = 256) { return static_cast<unsig
 void launch_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, NetGeom g, int batch,
                      int cin, int cout, int act, float slope, hipStream_t s) {
     const long long nv = static_cast<long long>(g.d) * g.h * g.w;
-    // four tiles per wave while that still gives 512 workgroups (measured:
-    // 65k-voxel layers are faster with four tiles, 8k-voxel ones with one)
-    if (static_cast<long long>(blocks_of(nv, 256)) * (cout / 16) * batch >= 512) {
-        dim3 grid(blocks_of(nv, 256), cout / 16, batch);
-        hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, s, x, w, bias, y, g, cin, cout, act, slope);
-    } else {
-        dim3 grid(blocks_of(nv, 64), cout / 16, batch);
-        hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, s, x, w, bias, y, g, cin, cout, act, slope);
+    // four voxel tiles per wave while that still gives 512 workgroups
+    // (measured: 65k-voxel layers are faster with four tiles, 8k-voxel ones
+    // with one), then as many output-channel tiles per wave (up to 4) as keep
+    // 512 workgroups
+    const int ncot = cout / 16;
+    const int nt = static_cast<long long>(blocks_of(nv, 256)) * ncot * batch >= 512 ? 4 : 1;
+    const long long vblocks = blocks_of(nv, 64 * nt);
+    int ct = 1;
+    for (int c : {4, 2}) {
+        if (ncot % c == 0 && vblocks * (ncot / c) * batch >= 512) {
+            ct = c;
+            break;
+        }
     }
+    dim3 grid(static_cast<unsigned>(vblocks), ncot / ct, batch);
+    using Fn = void (*)(const uint16_t*, const uint16_t*, const float*, uint16_t*, NetGeom, int, int, int, float);
+    static const Fn table[2][3] = {{k_conv_fwd<1, 1>, k_conv_fwd<1, 2>, k_conv_fwd<1, 4>},
+                                   {k_conv_fwd<4, 1>, k_conv_fwd<4, 2>, k_conv_fwd<4, 4>}};
+    hipLaunchKernelGGL(table[nt == 4 ? 1 : 0][ct == 4 ? 2 : ct - 1], grid, dim3(256), 0, s, x, w, bias, y, g, cin,
+                       cout, act, slope);
 }
 
 void launch_flip_weights(const uint16_t* w, int cin, int cout, uint16_t* wf, hipStream_t s) {
