@@ -8,9 +8,12 @@
 // the device.  The host waits once, at the end, for the logs.
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 
 #include "wr_internal.h"
 
@@ -239,6 +242,7 @@ void wr_destroy(wr_ctx* ctx) {
     for (hipEvent_t e : c.events) (void)hipEventDestroy(e);
     for (hipEvent_t e : c.lvl_events) (void)hipEventDestroy(e);
     if (c.side) (void)hipStreamDestroy(c.side);
+    if (c.hflags) (void)hipHostFree(c.hflags);
     delete ctx;
 }
 
@@ -361,6 +365,65 @@ struct FrameArgs {
     bool async = false;   // no logs: return once everything is enqueued
     WarpOut warp;         // warp.src set: correctMotion of it in the level-0 end
 };
+
+// ---- the host's view of a level's progress
+//
+// The stop rules run on the device, so the host enqueues a level's `iter`
+// iterations without waiting, and the launches after the level has stopped
+// exit at once (a few microseconds each).  Where the call waits for its
+// results anyway, the host instead keeps at most kAhead iterations in flight
+// and reads, from mapped host memory the kernels write (Ctl::hflag), how far
+// every frame's level has come and whether it is done; it stops enqueueing a
+// level once every frame's is.  The launches and their order are the same
+// either way, only the ones that would exit at once are not made.
+
+constexpr int kAhead = 2;
+constexpr unsigned kGenMask = (1u << 21) - 1;
+
+int ensure_hflags(Context& c, int frames) {
+    if (c.hflags && c.hflags_frames >= frames) return WR_OK;
+    if (c.hflags) WR_HIP_CHECK(hipHostFree(c.hflags));
+    c.hflags = nullptr;
+    c.hflags_dev = nullptr;
+    c.hflags_frames = 0;
+    void* h = nullptr;
+    WR_HIP_CHECK(hipHostMalloc(&h, sizeof(unsigned) * 2 * frames, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(h, 0, sizeof(unsigned) * 2 * frames);
+    void* d = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        ::wr::set_error(std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+        return WR_E_HIP;
+    }
+    c.hflags = static_cast<unsigned*>(h);
+    c.hflags_dev = static_cast<unsigned*>(d);
+    c.hflags_frames = frames;
+    return WR_OK;
+}
+
+// Waits until every frame's level has ended the LK pass of its first `need`
+// iterations or is done.  Returns 1 when every frame's level is done, 0 to go
+// on, -1 when nothing moved for a long time (the caller stops watching).
+int wait_progress(const Context& c, int frames, unsigned gen, int need) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0;; ++spin) {
+        bool all_done = true, ready = true;
+        for (int f = 0; f < frames; ++f) {
+            const unsigned pr = __atomic_load_n(c.hflags + 2 * f, __ATOMIC_ACQUIRE);
+            const unsigned dn = __atomic_load_n(c.hflags + 2 * f + 1, __ATOMIC_ACQUIRE);
+            if (dn == gen) continue;
+            all_done = false;
+            if ((pr >> 10) != gen || static_cast<int>(pr & 1023u) < need) ready = false;
+        }
+        if (all_done) return 1;
+        if (ready) return 0;
+        if ((spin & 1023u) == 1023u) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) return -1;
+            std::this_thread::yield();
+        }
+    }
+}
 
 static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int dtype, int nx, int ny, int nz,
                       const float* mask_ref, const float* mask_mov, const float* motion_init, const wr_params* prm,
@@ -612,6 +675,14 @@ static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int 
     };
     auto starts_at_cp = [&](int L) { return L == layer && motion_init && prm->init_cp; };
 
+    // watch the levels' progress (see wait_progress) where the call waits for
+    // its results anyway and the stream is not being captured into a graph
+    bool observe = !fa.async && iter < 1024;
+    if (observe) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        observe = hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone &&
+                  ensure_hflags(c, fa.n) == WR_OK;
+    }
     // the start field may come from another stream: the iterations wait for
     // it, the side stream's level images (forked above) do not
     if (prm->init_event) WR_HIP_CHECK(hipStreamWaitEvent(s, static_cast<hipEvent_t>(prm->init_event), 0));
@@ -658,8 +729,14 @@ static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int 
         if (!cp_start) launch_cp_init(g, m0, cp0, fr, s);
         const int frec = prm->forced_records ? prm->forced_records[L] : -1;
         const int fupd = prm->forced_updates ? prm->forced_updates[L] : -1;
+        unsigned gen = 0;
+        if (observe) {
+            c.gen = (c.gen + 1) & kGenMask;
+            if (c.gen == 0) c.gen = 1;
+            gen = c.gen;
+        }
         launch_ctl_reset(ctl, iter, frec, fupd, prm->tol, prm->smoothPenalty * p * p, static_cast<double>(nv), fr,
-                         s);
+                         s, observe ? c.hflags_dev : nullptr, gen);
 
         if (L - 2 >= 0) WR_TRY(build_level(L - 2));
 
@@ -685,8 +762,17 @@ static int get_motion_frames(wr_ctx* ctx, const void* mov, const void* ref, int 
         b.events = timing ? &c.events[static_cast<size_t>(L) * iter * 2] : nullptr;
         b.cp_start = cp_start ? 1 : 0;
         b.maxdiff_gather = prm->maxdiff_gather ? 1 : 0;
-        for (int k = 0; k < iter; ++k) WR_TRY(launch_iteration(g, t, b, k, prm->stop_kernels != 0, s));
-        c.stats.lk_launches[L] = iter;
+        int launched = 0;
+        for (int k = 0; k < iter; ++k) {
+            if (observe && gen && k >= kAhead) {
+                const int w = wait_progress(c, fa.n, gen, k - kAhead + 1);
+                if (w == 1) break;        // every frame's level is done
+                if (w < 0) gen = 0;       // no progress seen: enqueue the rest
+            }
+            WR_TRY(launch_iteration(g, t, b, k, prm->stop_kernels != 0, s));
+            ++launched;
+        }
+        c.stats.lk_launches[L] = launched;
         c.stats.lk_blocks[L] = static_cast<int>(t.blocks);
         if (L == 0) {
             c.stats.tile_cols = t.cols;

@@ -41,10 +41,12 @@ namespace {
 // ---------------------------------------------------------------- CP kernels
 
 __global__ __launch_bounds__(256) void k_ctl_reset(Ctl* c, int iter, int frec, int fupd, double tol,
-                                                    double s_pen, double n_vox, long long fs) {
+                                                    double s_pen, double n_vox, long long fs, unsigned* hflags, unsigned gen) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     c = at_frame(c, fs);
     c->done = 0;
+    c->hflag = hflags ? hflags + 2 * blockIdx.y : nullptr;
+    c->gen = gen;
     c->rec = 0;
     c->upd = 0;
     c->nan = 0;
@@ -104,6 +106,8 @@ __global__ __launch_bounds__(256) void k_solve(Ctl* __restrict__ c, const float*
                                                 double* __restrict__ log, int* __restrict__ counts, long long fs) {
     c = at_frame(c, fs);
     if (c->done) return;
+    // the LK pass of iteration k has ended: progress for a watching host
+    if (c->hflag && blockIdx.x == 0 && threadIdx.x == 0) host_word(c->hflag, (c->gen << 10) | (k + 1));
     sums = at_frame(sums, fs);
     dn = at_frame(dn, fs);
     cp = at_frame(cp, fs);
@@ -606,8 +610,8 @@ LkTile plan_tile(const LevelGeom& g, int np_x) {
 }
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen, double n_vox,
-                      const Frames& fr, hipStream_t s) {
-    k_ctl_reset<<<dim3(1, fr.n), 64, 0, s>>>(ctl, iter, frec, fupd, tol, s_pen, n_vox, fr.fs);
+                      const Frames& fr, hipStream_t s, unsigned* hflags, unsigned gen) {
+    k_ctl_reset<<<dim3(1, fr.n), 64, 0, s>>>(ctl, iter, frec, fupd, tol, s_pen, n_vox, fr.fs, hflags, gen);
 }
 
 void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, const Frames& fr, hipStream_t s) {

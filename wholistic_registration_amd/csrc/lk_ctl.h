@@ -64,6 +64,18 @@ __device__ __forceinline__ void block_sum2(double& a, double& b) {
     }
 }
 
+// A word of mapped host memory, written through to the system.
+__device__ __forceinline__ void host_word(unsigned* p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The level is done: every later kernel of the level exits; the host, where
+// it watches, stops enqueueing them.
+__device__ __forceinline__ void mark_done(Ctl* __restrict__ c) {
+    c->done = 1;
+    if (c->hflag) host_word(c->hflag + 1, c->gen);
+}
+
 // The error-based stop rule of iteration k (calFlow3d_Wei_v1.py:426,
 // 502-530) from the current error and the last three accepted errors, kept
 // in old_err[k & 1] (the shifted triple goes to old_err[(k + 1) & 1], so a
@@ -95,7 +107,7 @@ __device__ __forceinline__ void err_record(Ctl* __restrict__ c, int k, double di
         n[2] = cur;
     }
     c->maxdiff = 0u;
-    if (stop) c->done = 1;
+    if (stop) mark_done(c);
 }
 
 // Sums of the It^2 and |D|^2 partials and the error terms of iteration k; the
@@ -156,7 +168,7 @@ __device__ __forceinline__ void check_record(Ctl* __restrict__ c, int k, float m
     counts[1] = k + 1;
     counts[2] = c->nan;
     counts[3] = c->solve64;
-    if (stop) c->done = 1;
+    if (stop) mark_done(c);
 }
 
 // Run by one workgroup: the test of update k over `nparts` maxima.

@@ -171,6 +171,11 @@ struct Ctl {
     double s_pen;        // smoothPenalty * (2r+1)^2
     double inv_n;        // x*y*z of the level (the error terms divide by it)
     double last_err;
+    unsigned* hflag;     // NULL, or this frame's two words of host memory: [0] the
+                         // progress (gen << 10 | iterations whose LK pass ended),
+                         // [1] gen once the level is done (the host stops
+                         // enqueueing a level's iterations when it sees it)
+    unsigned gen;        // the level's generation (21 bits, never 0)
 };
 
 struct LevelGeom {
@@ -263,7 +268,8 @@ long long launch_lk_pass(const LevelGeom& g, const LkTile& t, const LkBuffers& b
                          int check_parts, hipStream_t s);
 
 void launch_ctl_reset(Ctl* ctl, int iter, int frec, int fupd, double tol, double s_pen,
-                      double inv_n, const Frames& fr, hipStream_t s);
+                      double inv_n, const Frames& fr, hipStream_t s,
+                      unsigned* hflags = nullptr, unsigned gen = 0);
 void launch_cp_init(const LevelGeom& g, const float* m0, float* cp, const Frames& fr, hipStream_t s);
 int launch_iteration(const LevelGeom& g, const LkTile& t, const LkBuffers& b, int k, bool separate_stop,
                      hipStream_t s);
@@ -518,6 +524,12 @@ struct Context {
     DevBuf netw;                         // learned initialisation: flipped weights, input statistics
     DevBuf xws, xsel, xroi;              // cross-resolution solver: per-iteration scratch, selection, trap ROI
     std::vector<hipEvent_t> events;
+    // progress and stop words of each frame's level in mapped host memory
+    // (Ctl::hflag), read by the host to stop enqueueing a finished level
+    unsigned* hflags = nullptr;          // host address
+    unsigned* hflags_dev = nullptr;      // device address of the same memory
+    int hflags_frames = 0;
+    unsigned gen = 0;
     hipStream_t side = nullptr;          // pyramid builds, ahead of the iterations
     std::vector<hipEvent_t> lvl_events;  // [L]: level L's images are ready; [layer+1]: fork
     wr_stats stats{};
