@@ -525,13 +525,15 @@ void launch_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, ui
     // four voxel tiles per wave while that still gives 512 workgroups
     // (measured: 65k-voxel layers are faster with four tiles, 8k-voxel ones
     // with one), then as many output-channel tiles per wave (up to 4) as keep
-    // 512 workgroups
+    // 256 workgroups (measured at batch 1 and 16: better than 512; 128 and 64
+    // gain nothing more)
     const int ncot = cout / 16;
+    const long long kCtMinBlocks = 256;
     const int nt = static_cast<long long>(blocks_of(nv, 256)) * ncot * batch >= 512 ? 4 : 1;
     const long long vblocks = blocks_of(nv, 64 * nt);
     int ct = 1;
     for (int c : {4, 2}) {
-        if (ncot % c == 0 && vblocks * (ncot / c) * batch >= 512) {
+        if (ncot % c == 0 && vblocks * (ncot / c) * batch >= kCtMinBlocks) {
             ct = c;
             break;
         }
