@@ -292,20 +292,23 @@ __global__ void k_lrelu_bwd(uint16_t* __restrict__ gr, const uint16_t* __restric
 
 // Y (B, D/2, H/2, W/2, C) = scale * sum of the 8 children in X (B, D, H, W, C);
 // a thread per (output voxel, 8 channels).
+// I: the index type (32-bit unsigned where the element count allows; its
+// divisions are a few instructions, 64-bit ones dozens).
+template <typename I>
 __global__ void k_down2(const uint16_t* __restrict__ X, NetGeom g, int c, long long n_out8, float scale,
                         uint16_t* __restrict__ Y) {
-    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n_out8) return;
-    const int c8 = c / 8;
+    const I i = static_cast<I>(blockIdx.x) * 256 + threadIdx.x;
+    if (static_cast<long long>(i) >= n_out8) return;
+    const I c8 = static_cast<I>(c / 8);
     const int q = static_cast<int>(i % c8);
-    long long v = i / c8;
-    const int d2 = g.d / 2, h2 = g.h / 2, w2 = g.w / 2;
+    I v = i / c8;
+    const I d2 = static_cast<I>(g.d / 2), h2 = static_cast<I>(g.h / 2), w2 = static_cast<I>(g.w / 2);
     const int x = static_cast<int>(v % w2);
     v /= w2;
     const int y = static_cast<int>(v % h2);
     v /= h2;
     const int z = static_cast<int>(v % d2);
-    const long long b = v / d2;
+    const long long b = static_cast<long long>(v / d2);
     float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -324,19 +327,20 @@ __global__ void k_down2(const uint16_t* __restrict__ X, NetGeom g, int c, long l
 
 // Y (B, D, H, W, C) (+)= scale * X at the parent voxel (B, D/2, H/2, W/2, C);
 // a thread per (output voxel, 8 channels).
+template <typename I>
 __global__ void k_up2(const uint16_t* __restrict__ X, NetGeom g, int c, long long n_out8, float scale, int accumulate,
                       uint16_t* __restrict__ Y) {
-    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n_out8) return;
-    const int c8 = c / 8;
+    const I i = static_cast<I>(blockIdx.x) * 256 + threadIdx.x;
+    if (static_cast<long long>(i) >= n_out8) return;
+    const I c8 = static_cast<I>(c / 8);
     const int q = static_cast<int>(i % c8);
-    long long v = i / c8;
-    const int x = static_cast<int>(v % g.w);
-    v /= g.w;
-    const int y = static_cast<int>(v % g.h);
-    v /= g.h;
-    const int z = static_cast<int>(v % g.d);
-    const long long b = v / g.d;
+    I v = i / c8;
+    const int x = static_cast<int>(v % static_cast<I>(g.w));
+    v /= static_cast<I>(g.w);
+    const int y = static_cast<int>(v % static_cast<I>(g.h));
+    v /= static_cast<I>(g.h);
+    const int z = static_cast<int>(v % static_cast<I>(g.d));
+    const long long b = static_cast<long long>(v / static_cast<I>(g.d));
     const size_t src = (((static_cast<size_t>(b) * (g.d / 2) + z / 2) * (g.h / 2) + y / 2) * (g.w / 2) + x / 2) * c +
                        8 * q;
     Pack8 p, o;
@@ -351,26 +355,28 @@ __global__ void k_up2(const uint16_t* __restrict__ X, NetGeom g, int c, long lon
 }
 
 // Y[v] = (A[v] (ca channels), B[v] (cb channels)); thread per (voxel, 8 channels of Y)
+template <typename I>
 __global__ void k_concat(const uint16_t* __restrict__ A, int ca, const uint16_t* __restrict__ B, int cb,
                          long long n8, uint16_t* __restrict__ Y) {
-    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n8) return;
-    const int c8 = (ca + cb) / 8;
+    const I i = static_cast<I>(blockIdx.x) * 256 + threadIdx.x;
+    if (static_cast<long long>(i) >= n8) return;
+    const I c8 = static_cast<I>((ca + cb) / 8);
     const int q = static_cast<int>(i % c8);
-    const long long v = i / c8;
+    const long long v = static_cast<long long>(i / c8);
     const uint4 val = 8 * q < ca ? *reinterpret_cast<const uint4*>(A + static_cast<size_t>(v) * ca + 8 * q)
                                  : *reinterpret_cast<const uint4*>(B + static_cast<size_t>(v) * cb + 8 * q - ca);
     reinterpret_cast<uint4*>(Y)[i] = val;
 }
 
 // dA[v] = G[v][0:ca], dB[v] = G[v][ca:ca+cb]
+template <typename I>
 __global__ void k_split(const uint16_t* __restrict__ G, int ca, int cb, long long n8, uint16_t* __restrict__ dA,
                         uint16_t* __restrict__ dB) {
-    const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-    if (i >= n8) return;
-    const int c8 = (ca + cb) / 8;
+    const I i = static_cast<I>(blockIdx.x) * 256 + threadIdx.x;
+    if (static_cast<long long>(i) >= n8) return;
+    const I c8 = static_cast<I>((ca + cb) / 8);
     const int q = static_cast<int>(i % c8);
-    const long long v = i / c8;
+    const long long v = static_cast<long long>(i / c8);
     const uint4 val = reinterpret_cast<const uint4*>(G)[i];
     if (8 * q < ca)
         *reinterpret_cast<uint4*>(dA + static_cast<size_t>(v) * ca + 8 * q) = val;
@@ -569,23 +575,27 @@ void launch_lrelu_bwd(uint16_t* gr, const uint16_t* y, long long n, float slope,
 
 void launch_down2(const uint16_t* x, NetGeom g, int batch, int c, float scale, uint16_t* y, hipStream_t s) {
     const long long n8 = static_cast<long long>(batch) * (g.d / 2) * (g.h / 2) * (g.w / 2) * (c / 8);
-    hipLaunchKernelGGL(k_down2, dim3(blocks_of(n8)), dim3(256), 0, s, x, g, c, n8, scale, y);
+    hipLaunchKernelGGL(n8 + 256 < (1LL << 32) ? k_down2<unsigned> : k_down2<long long>, dim3(blocks_of(n8)), dim3(256), 0,
+                       s, x, g, c, n8, scale, y);
 }
 
 void launch_up2(const uint16_t* x, NetGeom g, int batch, int c, float scale, int accumulate, uint16_t* y,
                 hipStream_t s) {
     const long long n8 = static_cast<long long>(batch) * g.d * g.h * g.w * (c / 8);
-    hipLaunchKernelGGL(k_up2, dim3(blocks_of(n8)), dim3(256), 0, s, x, g, c, n8, scale, accumulate, y);
+    hipLaunchKernelGGL(n8 + 256 < (1LL << 32) ? k_up2<unsigned> : k_up2<long long>, dim3(blocks_of(n8)), dim3(256), 0, s,
+                       x, g, c, n8, scale, accumulate, y);
 }
 
 void launch_concat(const uint16_t* a, int ca, const uint16_t* b, int cb, long long nvox, uint16_t* y, hipStream_t s) {
     const long long n8 = nvox * ((ca + cb) / 8);
-    hipLaunchKernelGGL(k_concat, dim3(blocks_of(n8)), dim3(256), 0, s, a, ca, b, cb, n8, y);
+    hipLaunchKernelGGL(n8 + 256 < (1LL << 32) ? k_concat<unsigned> : k_concat<long long>, dim3(blocks_of(n8)), dim3(256),
+                       0, s, a, ca, b, cb, n8, y);
 }
 
 void launch_split(const uint16_t* gr, int ca, int cb, long long nvox, uint16_t* da, uint16_t* db, hipStream_t s) {
     const long long n8 = nvox * ((ca + cb) / 8);
-    hipLaunchKernelGGL(k_split, dim3(blocks_of(n8)), dim3(256), 0, s, gr, ca, cb, n8, da, db);
+    hipLaunchKernelGGL(n8 + 256 < (1LL << 32) ? k_split<unsigned> : k_split<long long>, dim3(blocks_of(n8)), dim3(256),
+                       0, s, gr, ca, cb, n8, da, db);
 }
 
 void launch_mse(const uint16_t* p, int cp, const float* t, int ct, long long nvox, float pscale, float scale,
