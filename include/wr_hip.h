@@ -424,6 +424,11 @@ int wr_net_up2(wr_ctx* ctx, const void* x, void* y, int batch, int d, int h, int
                int accumulate, void* stream);
 /* y[v] = (a[v], b[v]) over nvox voxels; wr_net_split is its backward. */
 int wr_net_concat(wr_ctx* ctx, const void* a, int ca, const void* b, int cb, long long nvox, void* y, void* stream);
+/* y (batch, d, h, w, ca + cb) = (the nearest 2x upsampling of x (batch, d/2,
+ * h/2, w/2, ca), b (batch, d, h, w, cb)): wr_net_up2 (scale 1) and
+ * wr_net_concat in one pass, without the upsampled tensor. */
+int wr_net_upcat(wr_ctx* ctx, const void* x, int ca, const void* b, int cb, int batch, int d, int h, int wd, void* y,
+                 void* stream);
 int wr_net_split(wr_ctx* ctx, const void* g, int ca, int cb, long long nvox, void* da, void* db, void* stream);
 /* *loss (float32, device) += scale * sum over voxels and channels c < ct of
  * (pscale * p[v][c] - t[v][c])^2, t float32 (nvox, ct); dp (nvox, cp) = the

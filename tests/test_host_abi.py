@@ -116,6 +116,7 @@ def test_smooth_penalty_null_arguments_are_rejected():
     ("wr_net_down2", (None, None, None, 1, 4, 4, 8, 8, 0.125, None)),
     ("wr_net_up2", (None, None, None, 1, 4, 4, 8, 8, 1.0, 0, None)),
     ("wr_net_concat", (None, None, 8, None, 8, 16, None, None)),
+    ("wr_net_upcat", (None, None, 8, None, 8, 1, 4, 4, 8, None, None)),
     ("wr_net_split", (None, None, 8, 8, 16, None, None, None)),
     ("wr_net_mse", (None, None, 16, None, 3, 16, 4.0, 1.0, None, None, None)),
     ("wr_net_adam", (None, None, None, None, None, None, 10, 1e-3, 0.9, 0.999, 1e-8, 1, None)),

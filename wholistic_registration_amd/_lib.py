@@ -141,6 +141,7 @@ _SIGNATURES = {
     "wr_net_down2": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, ctypes.c_double, _VP], _I),
     "wr_net_up2": ([_VP, _VP, _VP, _I, _I, _I, _I, _I, ctypes.c_double, _I, _VP], _I),
     "wr_net_concat": ([_VP, _VP, _I, _VP, _I, ctypes.c_longlong, _VP, _VP], _I),
+    "wr_net_upcat": ([_VP, _VP, _I, _VP, _I, _I, _I, _I, _I, _VP, _VP], _I),
     "wr_net_split": ([_VP, _VP, _I, _I, ctypes.c_longlong, _VP, _VP, _VP], _I),
     "wr_net_mse": ([_VP, _VP, _I, _VP, _I, ctypes.c_longlong, ctypes.c_double, ctypes.c_double, _VP, _VP, _VP],
                    _I),

@@ -1305,6 +1305,24 @@ int wr_net_concat(wr_ctx* ctx, const void* a, int ca, const void* b, int cb, lon
     return WR_OK;
 }
 
+int wr_net_upcat(wr_ctx* ctx, const void* x, int ca, const void* b, int cb, int batch, int d, int h, int wd, void* y,
+                 void* stream) {
+    WR_ARG(ctx && x && b && y, "wr_net_upcat: null argument");
+    WR_ARG(ca > 0 && cb > 0 && ca % 8 == 0 && cb % 8 == 0, "wr_net_upcat: bad channel counts");
+    WR_ARG(batch > 0 && d > 0 && h > 0 && wd > 0 && d % 2 == 0 && h % 2 == 0 && wd % 2 == 0,
+           "wr_net_upcat: d, h and w must be positive and even");
+    WR_ARG(al16(x) && al16(b) && al16(y), "wr_net_upcat: buffers must be 16-byte aligned");
+    WR_HIP_CHECK(hipSetDevice(ctx->c.device));
+    NetGeom g{};
+    g.d = d;
+    g.h = h;
+    g.w = wd;
+    launch_upcat(static_cast<const uint16_t*>(x), ca, static_cast<const uint16_t*>(b), cb, g, batch,
+                 static_cast<uint16_t*>(y), static_cast<hipStream_t>(stream));
+    WR_HIP_CHECK(hipGetLastError());
+    return WR_OK;
+}
+
 int wr_net_split(wr_ctx* ctx, const void* g, int ca, int cb, long long nvox, void* da, void* db, void* stream) {
     WR_ARG(ctx && g && da && db, "wr_net_split: null argument");
     WR_ARG(ca > 0 && cb > 0 && ca % 8 == 0 && cb % 8 == 0 && nvox > 0, "wr_net_split: bad channel counts");

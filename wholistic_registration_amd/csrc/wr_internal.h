@@ -482,6 +482,11 @@ void launch_down2(const uint16_t* x, NetGeom g, int batch, int c, float scale, u
 void launch_up2(const uint16_t* x, NetGeom g, int batch, int c, float scale, int accumulate, uint16_t* y,
                 hipStream_t s);
 void launch_concat(const uint16_t* a, int ca, const uint16_t* b, int cb, long long nvox, uint16_t* y, hipStream_t s);
+// The concatenation of the nearest 2x upsampling of x (batch, d/2, h/2, w/2,
+// ca) with b (batch, d, h, w, cb) into y (batch, d, h, w, ca + cb); g holds
+// the full-resolution d, h, w.
+void launch_upcat(const uint16_t* x, int ca, const uint16_t* b, int cb, NetGeom g, int batch, uint16_t* y,
+                  hipStream_t s);
 void launch_split(const uint16_t* gr, int ca, int cb, long long nvox, uint16_t* da, uint16_t* db, hipStream_t s);
 // loss += scale * sum over voxels and channels c < ct of (pscale p - t)^2;
 // dp = d loss / d p (zero in channels ct..cp-1).

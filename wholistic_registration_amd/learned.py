@@ -198,12 +198,10 @@ class InitNet:
         cu = cb
         for i in range(n - 2, -1, -1):       # decoder level i: upsample, concatenate the skip, two convs
             c = self.channels[i]
-            u = self._buf(f"u{i + 1}", (B,) + dims[i] + (cu,))
-            self._check(L.wr_net_up2(self.ctx, _p(h), _p(u), B, *dims[i], cu, 1.0, 0, self._s()), "wr_net_up2")
-            nv = B * dims[i][0] * dims[i][1] * dims[i][2]
+            # nearest upsampling of h and the skip connection, concatenated
             cat = self._buf(f"cat{i + 1}", (B,) + dims[i] + (cu + c,))
-            self._check(L.wr_net_concat(self.ctx, _p(u), cu, _p(a[f"enc{i + 1}b"]), c, nv, _p(cat), self._s()),
-                        "wr_net_concat")
+            self._check(L.wr_net_upcat(self.ctx, _p(h), cu, _p(a[f"enc{i + 1}b"]), c, B, *dims[i], _p(cat),
+                                       self._s()), "wr_net_upcat")
             h = self._conv(f"dec{i + 1}a", cat, self._buf(f"dec{i + 1}a", (B,) + dims[i] + (c,)), dims[i], B)
             h = self._conv(f"dec{i + 1}b", h, self._buf(f"dec{i + 1}b", (B,) + dims[i] + (c,)), dims[i], B)
             cu = c
