@@ -161,6 +161,30 @@ def test_batched_learned_frames_equal_single_runs(setup, precision):
         assert torch.equal(w1, wo[f])
 
 
+def test_batched_learned_on_the_registration_context_equals_single_runs(setup):
+    """With the network forked onto the registration's own context (one
+    workspace), register_learned_batch runs the network on the registration's
+    stream instead of a second one; the fields still equal the single runs."""
+    import torch
+
+    from wholistic_registration_amd import _lib
+    from wholistic_registration_amd import learned as Lr
+    from wholistic_registration_amd import train_init as T
+    s = setup
+    dev = s["tmpl"].device
+    ctx = _lib.context(dev.index or 0)
+    net = s["net"].fork(ctx)
+    movs = torch.stack([T.make_moving(s["clean"], seed, dev)[0] for seed in (90009, 90010)])
+    mo = torch.empty(tuple(movs.shape) + (3,), dtype=torch.float32, device=dev)
+    res = Lr.register_learned_batch(movs, s["tmpl"], net, ref_level=s["ref_l"], precision="bf16", chunk=2,
+                                    motion_out=mo, context=ctx, **Lr.SCHEDULE, **s["params"])
+    for f in range(movs.shape[0]):
+        single, _ = Lr.register_learned(movs[f], s["tmpl"], s["net"], ref_level=s["ref_l"], precision="bf16",
+                                        **Lr.SCHEDULE, **s["params"])
+        assert torch.equal(single.motion, mo[f])
+        assert single.records == res[f].records
+
+
 def test_learned_start_on_uneven_crops(setup):
     """Crops whose level-3 image is not a multiple of the network's (4, 4, 32)
     grid run on an edge-padded grid; crops on concurrent contexts equal their

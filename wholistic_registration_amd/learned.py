@@ -512,7 +512,12 @@ def register_learned_batch(movs, ref, net, *, layer, iter, r, zRatio, movRange, 
         warped_out = torch.empty((nf,) + full, dtype=torch.float32, device=movs.device)
     out = []
     main = stream if stream is not None else torch.cuda.current_stream(movs.device)
-    side = _net_stream(movs.device)
+    # the network's work overlaps the registration's only on a context of its
+    # own (two streams must not share one context's workspace); a network
+    # forked onto the registration's context runs on the registration's stream
+    reg_ctx = context if context is not None else _lib.context(movs.device.index or 0)
+    own = _ctx_value(net.ctx) != _ctx_value(reg_ctx)
+    side = _net_stream(movs.device) if own else main
     with torch.cuda.stream(main):
         r_l = ref_level if ref_level is not None else _dev.pyramid_level(ref, LEVEL, context=context)
         for f0 in range(0, nf, chunk):
@@ -525,9 +530,12 @@ def register_learned_batch(movs, ref, net, *, layer, iter, r, zRatio, movRange, 
                 m_ls = _dev.pyramid_level(chunk_vol, LEVEL, context=net.ctx)
                 m_ls = m_ls.view(f1 - f0, full[0], m_ls.shape[1], m_ls.shape[2])
                 cps = net.predict_cp_batch(m_ls, r_l, full, r, level=layer)
-                ready = torch.cuda.Event()
-                ready.record(side)
-            cps.record_stream(main)
+                ready = None
+                if own:
+                    ready = torch.cuda.Event()
+                    ready.record(side)
+            if own:
+                cps.record_stream(main)
             out += _dev.get_motion_batch(movs[f0:f1], ref, layer=layer, iter=iter, r=r, zRatio=zRatio,
                                          movRange=movRange, smoothPenalty=smoothPenalty, tol=tol,
                                          motion_init_cp=cps, init_event=ready, motion_out=motion_out[f0:f1],
@@ -538,6 +546,11 @@ def register_learned_batch(movs, ref, net, *, layer, iter, r, zRatio, movRange, 
 
 
 _NET_STREAMS = {}
+
+
+def _ctx_value(h):
+    """The address of a library context handle (an int or a ctypes pointer)."""
+    return getattr(h, "value", h)
 
 
 def _net_stream(device):
