@@ -150,7 +150,7 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
                schedule=None, tile_patches_x=0, motion_out=None, device=None, timing=False,
                stop_kernels=False, precision="fp32", reuse_template=False, context=None, stream=None,
                motion_init_cp=None, resample_start=False, maxdiff_gather=False, warp_src=None,
-               warped_out=None):
+               warped_out=None, init_event=None):
     """Pyramid LK registration of ``mov`` onto ``ref`` on the GPU.
 
     Replaces ``getMotion`` (calFlow3d_Wei_v1.py:295-673); the stop rules run on
@@ -174,6 +174,9 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
     that writes the field (wr_get_motion_warp).  ``motion_init`` may have
     another x-y size than the volume (full-resolution voxel units; the
     reference's warm start resamples it to the coarsest level).
+    ``init_event`` (a ``torch.cuda.Event``): where ``motion_init`` /
+    ``motion_init_cp`` is written on another stream; the iterations wait for
+    it, the level images are built without waiting (``wr_params.init_event``).
     ``motion_init_cp`` (instead of ``motion_init``): the coarsest level's
     control-point field itself, (3, Z, ncy, ncx) float32 in that level's
     voxel units (``cp_grid``); the level starts from its interpolant.
@@ -209,6 +212,8 @@ def get_motion(mov, ref, *, layer, iter, r, zRatio, movRange, smoothPenalty, tol
 
     prm, keep = _params(layer, iter, r, zRatio, movRange, smoothPenalty, tol, tile_patches_x, stop_kernels,
                         precision, reuse_template, timing, schedule, resample_start, maxdiff_gather)
+    if init_event is not None:     # motion_init / motion_init_cp written on another stream
+        prm.init_event = ctypes.c_void_p(init_event.cuda_event)
     if motion_init_cp is not None:
         prm.init_cp = 1
         motion_init = motion_init_cp

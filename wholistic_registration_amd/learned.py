@@ -456,12 +456,27 @@ def register_learned(mov, ref, net, *, layer, iter, r, zRatio, movRange, smoothP
                   context=context, stream=stream, **kw)
     if start not in ("cp", "dense"):
         raise ValueError("start must be 'cp' or 'dense'")
-    with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(mov.device)):
-        m_l, r_l = level_images(mov, ref, ref_level=ref_level, context=context, stream=stream)
-        if start == "cp":
-            init = {"motion_init_cp": net.predict_cp(m_l, r_l, tuple(mov.shape), r, level=layer)}
-        else:
-            init = {"motion_init": net.predict(m_l, r_l)}
+    main = stream if stream is not None else torch.cuda.current_stream(mov.device)
+    reg_ctx = context if context is not None else _lib.context(mov.device.index or 0)
+    if start == "cp" and _ctx_value(net.ctx) != _ctx_value(reg_ctx):
+        # the network (on a context of its own) runs on a second stream while
+        # getMotion builds its level images; the iterations wait for it
+        side = _net_stream(mov.device)
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            m_l, r_l = level_images(mov, ref, ref_level=ref_level, context=net.ctx)
+            cp_init = net.predict_cp(m_l, r_l, tuple(mov.shape), r, level=layer)
+            ready = torch.cuda.Event()
+            ready.record(side)
+        cp_init.record_stream(main)
+        init = {"motion_init_cp": cp_init, "init_event": ready}
+    else:
+        with torch.cuda.stream(main):
+            m_l, r_l = level_images(mov, ref, ref_level=ref_level, context=context, stream=stream)
+            if start == "cp":
+                init = {"motion_init_cp": net.predict_cp(m_l, r_l, tuple(mov.shape), r, level=layer)}
+            else:
+                init = {"motion_init": net.predict(m_l, r_l)}
     res = _dev.get_motion(mov, ref, iter=iter, **init, **params)
     info = {"used": "learned", "learned_error": res.final_error}
     if fallback == "plain":
